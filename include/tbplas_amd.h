@@ -1,0 +1,180 @@
+/*
+ * tbplas_amd -- C ABI of the MI355X TBPM hot path.
+ *
+ * Plain pointers and sizes only.  Unless stated otherwise every array
+ * argument is a DEVICE pointer on the handle's device; scalars and the small
+ * coefficient tables (Bessel, Fermi, energies) are HOST pointers.
+ * Complex numbers are interleaved (re, im) doubles, i.e. numpy complex128.
+ * Every call is ordered on the handle's stream and returns after that stream
+ * has finished the call's work.
+ *
+ * Status codes: 0 ok; TBG_ENORM the wave-function norm check failed (the
+ * step is written to *fail_step, the outputs hold every step computed);
+ * TBG_EINVAL bad argument; TBG_EOOM device allocation failed; TBG_EHIP a HIP
+ * runtime call failed.  The message is available from tbg_last_error().
+ * No call ever terminates the process (the reference STOPs,
+ * tbplas/fortran/funcs.f90:175-192).
+ */
+#ifndef TBPLAS_AMD_H
+#define TBPLAS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TBG_OK 0
+#define TBG_ENORM 1
+#define TBG_EINVAL 2
+#define TBG_EOOM 3
+#define TBG_EHIP 4
+
+#define TBG_FP64 0 /* complex128 storage and arithmetic */
+#define TBG_FP32 1 /* complex64 storage and arithmetic, FP64 correlations */
+
+typedef struct tbg_handle tbg_handle;
+typedef struct tbg_ham tbg_ham;
+
+/* ---- library / handle --------------------------------------------------- */
+int tbg_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char *tbg_last_error(void);
+/* Create a handle on `device` with its own default-priority stream. */
+int tbg_create(tbg_handle **h, int device);
+int tbg_destroy(tbg_handle *h);
+/* 1: replay each time step from a captured hipGraph (default); 0: launch. */
+int tbg_set_graph(tbg_handle *h, int enable);
+
+/* ---- CSR assembly --------------------------------------------------------
+ * Replaces the Cython build_ham_dxy_fast (tbplas/cython/sample.pyx:148-257,
+ * called from tbplas/builder/sample.py:452-458).  Insertion order: within a
+ * row, the kept diagonal (|orb_eng| >= cutoff) first, then for each hop ih
+ * in order the entry (hop_i, hop_j, hop_v, +dr) and (hop_j, hop_i,
+ * conj(hop_v), -dr).  Bit-exact in indptr/indices/hop/dx/dy.
+ * Two phases: count writes indptr (n+1, device) and *nnz (host); the caller
+ * allocates indices/hop/dx/dy with *nnz elements and calls fill.
+ * dr is (m, 3) row-major float64; hop_v/hop are complex128.
+ */
+int tbg_ham_dxy_count(tbg_handle *h, int64_t n, int64_t m,
+                      const double *orb_eng, const int64_t *hop_i,
+                      const int64_t *hop_j, double cutoff, int64_t *indptr,
+                      int64_t *nnz);
+int tbg_ham_dxy_fill(tbg_handle *h, int64_t n, int64_t m,
+                     const double *orb_eng, const int64_t *hop_i,
+                     const int64_t *hop_j, const double *hop_v,
+                     const double *dr, double cutoff, const int64_t *indptr,
+                     int64_t *indices, double *hop, double *dx, double *dy);
+
+/* Replaces Sample.build_ham_csr (tbplas/builder/sample.py:375-391), the
+ * scipy sum dia(orb_eng) + csr(hop) + csr(hop)^H in canonical form: columns
+ * sorted, duplicates summed ((dia + half) + half^H), exact zeros dropped.
+ * Same two-phase protocol. */
+int tbg_ham_csr_count(tbg_handle *h, int64_t n, int64_t m,
+                      const double *orb_eng, const int64_t *hop_i,
+                      const int64_t *hop_j, const double *hop_v,
+                      int64_t *indptr, int64_t *nnz);
+int tbg_ham_csr_fill(tbg_handle *h, int64_t n, int64_t m,
+                     const double *orb_eng, const int64_t *hop_i,
+                     const int64_t *hop_j, const double *hop_v,
+                     const int64_t *indptr, int64_t *indices, double *data);
+
+/* Replaces get_rescale (tbplas/cython/sample.pyx:75-114): the maximum over
+ * rows of orb_eng[i] + sum |hop| of the hops touching row i.  Host result. */
+int tbg_get_rescale(tbg_handle *h, int64_t n, int64_t m,
+                    const double *orb_eng, const int64_t *hop_i,
+                    const int64_t *hop_j, const double *hop_v,
+                    double *rescale);
+
+/* Replaces set_mag_field (tbplas/cython/sample.pyx:18-72): Peierls phases
+ * for a field along -z, applied in place to hop_v.  HOST pointers (this is
+ * O(m) host work done once per sample, kept on the host so the phases use
+ * the same libm cos/sin as the reference).  gauge 0: (By, 0, 0);
+ * 1: (0, -Bx, 0); 2: the half sum. */
+int tbg_peierls_host(int64_t m, const int64_t *hop_i, const int64_t *hop_j,
+                     double *hop_v, const double *dr, const double *orb_pos,
+                     double intensity, int gauge);
+
+/* ---- prepared Hamiltonian ------------------------------------------------
+ * Stands in for make_csr_matrix (tbplas/fortran/csr.F90:45-68) and
+ * current_coefficient (tbplas/fortran/funcs.f90:12-33): takes the CSR arrays
+ * the reference passes to f2py (indptr/indices int64, hop complex128, all
+ * device) and builds the device layout used by the propagation kernels.
+ * dx/dy may be NULL; when given, the current operators
+ * J_{x,y} = (0, cur_scale) * hop * d{x,y} are built on the same structure.
+ * precision: TBG_FP64 or TBG_FP32.  Requires n < 2^31.
+ */
+int tbg_ham_create(tbg_handle *h, int64_t n, const int64_t *indptr,
+                   const int64_t *indices, const double *hop,
+                   const double *dx, const double *dy, double cur_scale,
+                   int precision, tbg_ham **out);
+int tbg_ham_destroy(tbg_ham *H);
+/* Stored elements (with slice padding) and maximum row width. */
+int tbg_ham_info(const tbg_ham *H, int64_t *n, int64_t *nnz,
+                 int64_t *padded, int *max_width);
+
+/* ---- correlation functions ----------------------------------------------
+ * psi0: K initial states, complex128 (K, n) row-major (state k contiguous),
+ * supplied by the caller (the reference draws them inside Fortran,
+ * tbplas/fortran/random.f90:36-69).  Outputs are PER STATE; averaging over
+ * states (the reference's corr += corrval / n_ran_samples) is the caller's.
+ * bes: host Bessel series J_0..J_{n_bes-1} of the scaled time step
+ * (tbplas/tbpm/solver.py:194-224), n_bes >= 3.
+ * chk_step/chk_thr: norm check as tbplas/fortran/funcs.f90:175-192 at the
+ * step the reference checks (chk_step outside the loop range disables it).
+ */
+
+/* Replaces f2py tbpm_dos (tbplas/fortran/tbpm.f90:6-93; called from
+ * tbplas/tbpm/solver.py:271-281).  corr: complex128 (K, T+1), t = 0..T. */
+int tbg_corr_dos(tbg_handle *h, const tbg_ham *H, const double *bes,
+                 int n_bes, const double *psi0, int K, int T, int chk_step,
+                 double chk_thr, double *corr, int *fail_step);
+
+/* Replaces f2py tbpm_accond (tbplas/fortran/tbpm.f90:193-351; called from
+ * tbplas/tbpm/solver.py:335-349).  H must carry current operators built with
+ * cur_scale = rescale.  coef_F / coef_omF: host Chebyshev coefficients of the
+ * Fermi and one-minus-Fermi operators (tbplas/fortran/funcs.f90:60-123).
+ * corr: complex128 (K, 4, T) with columns xx, xy, yx, yy; column t=1 is zero
+ * as in the reference (tbpm.f90:283-302 never stores it). */
+int tbg_corr_ac(tbg_handle *h, const tbg_ham *H, const double *bes, int n_bes,
+                const double *coef_F, int n_F, const double *coef_omF,
+                int n_omF, const double *psi0, int K, int T, int chk_step,
+                double chk_thr, double *corr, int *fail_step);
+
+/* Replaces f2py tbpm_dccond (tbplas/fortran/tbpm.f90:497-718; called from
+ * tbplas/tbpm/solver.py:422-436).  energies: host grid; en_inds: host 0-based
+ * indices of the n_en quasi-eigenstate energies.  dos_corr: complex128
+ * (K, T), t = 1..T.  dc_corr: complex128 (K, 2, n_en, T). */
+int tbg_corr_dc(tbg_handle *h, const tbg_ham *H, const double *bes, int n_bes,
+                const double *psi0, int K, int T, double t_step,
+                const double *energies, const int64_t *en_inds, int n_en,
+                int chk_step, double chk_thr, double *dos_corr,
+                double *dc_corr, int *fail_step);
+
+/* One propagation step psi <- exp(-+ i H~ tau) psi of K states in place
+ * (cheb_wf_timestep, tbplas/fortran/propagation.f90:12-49).  fwd=1: forward.
+ * psi: complex128 (K, n). */
+int tbg_cheb_step(tbg_handle *h, const tbg_ham *H, const double *bes,
+                  int n_bes, double *psi, int K, int fwd);
+
+/* Apply a real Chebyshev polynomial sum_k c_k T_k(H~) to K states in place
+ * (Fermi, tbplas/fortran/propagation.f90:52-82).  psi: complex128 (K, n). */
+int tbg_cheb_poly(tbg_handle *h, const tbg_ham *H, const double *coef,
+                  int n_coef, double *psi, int K);
+
+/* ---- batched uneven instances --------------------------------------------
+ * No reference counterpart (the reference runs one sample at a time): DOS
+ * correlations of several independent Hamiltonians in one launch per
+ * Chebyshev term.  hams: n_inst prepared Hamiltonians (same precision);
+ * psi0[i]: device complex128 (K, n_i); corr[i]: device complex128 (K, T+1).
+ */
+int tbg_corr_dos_batch(tbg_handle *h, int n_inst, const tbg_ham *const *hams,
+                       const double *bes, int n_bes,
+                       const double *const *psi0, int K, int T, int chk_step,
+                       double chk_thr, double *const *corr, int *fail_inst,
+                       int *fail_step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TBPLAS_AMD_H */

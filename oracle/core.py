@@ -1,0 +1,674 @@
+"""Oracle implementation -- test infrastructure only (see package docstring)."""
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+__all__ = [
+    "oracle_lib", "ref_lib", "ref_available",
+    "orc_tbpm_dos", "orc_tbpm_accond", "orc_tbpm_dccond", "orc_step",
+    "orc_fermi_coef", "orc_csr_mv",
+    "ref_random_state", "ref_tbpm_dos", "ref_tbpm_accond", "ref_tbpm_dccond",
+    "ref_fermi_coef",
+    "bessel_series", "build_ham_csr", "build_ham_dxy_loop",
+    "build_ham_dxy_stable", "graphene_hops_loop", "get_rescale_loop",
+    "set_mag_field_loop", "kspace_graphene_corr", "dense_corr",
+    "analyzer_calc_dos", "analyzer_calc_ac_cond", "analyzer_calc_dc_cond",
+    "window_hanning", "window_exp",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ORC_SO = os.path.join(_HERE, "_build", "liboracle.so")
+_REF_SO = os.path.join(_HERE, "_ref", "libtbpm_ref.so")
+_c_i64p = ctypes.POINTER(ctypes.c_int64)
+_c_dp = ctypes.POINTER(ctypes.c_double)
+_c_ip = ctypes.POINTER(ctypes.c_int)
+_libs = {}
+
+
+def _ptr(a, ctype=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def oracle_lib():
+    """Load (building if needed) the C restatement library."""
+    if "orc" not in _libs:
+        if not os.path.exists(_ORC_SO):
+            subprocess.run(["make", "-C", _HERE, "oracle"], check=True,
+                           stdout=subprocess.DEVNULL)
+        _libs["orc"] = ctypes.CDLL(_ORC_SO)
+    return _libs["orc"]
+
+
+def ref_available():
+    return os.path.exists(_REF_SO)
+
+
+def ref_lib():
+    """Load the reference Fortran TBPM library compiled from its sources."""
+    if "ref" not in _libs:
+        if not ref_available():
+            raise FileNotFoundError(
+                f"{_REF_SO} missing: run `make -C oracle ref` where the "
+                "reference sources exist")
+        _libs["ref"] = ctypes.CDLL(_REF_SO)
+    return _libs["ref"]
+
+
+def _csr_args(indptr, indices, hop):
+    indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+    indices = np.ascontiguousarray(indices, dtype=np.int64)
+    hop = np.ascontiguousarray(hop, dtype=np.complex128)
+    return indptr, indices, hop
+
+
+def _check_rc(rc, fail_step, what):
+    if rc == 1:
+        raise FloatingPointError(f"{what}: wave function norm broken at step "
+                                 f"{fail_step.value}")
+    if rc != 0:
+        raise RuntimeError(f"{what}: oracle returned {rc}")
+
+
+# ---------------------------------------------------------------------------
+# C restatement (oracle/tbpm_oracle.c)
+# ---------------------------------------------------------------------------
+def orc_tbpm_dos(indptr, indices, hop, bes, psi0, n_timestep,
+                 chk_step=128, chk_thr=1e-9):
+    """tbpm_dos (tbpm.f90:6-93) with injected psi0 (S, n).
+
+    Returns the per-sample correlation (S, T+1); the reference's output is
+    the running average ``sum_s c_s / S`` of these rows.
+    """
+    lib = oracle_lib()
+    indptr, indices, hop = _csr_args(indptr, indices, hop)
+    psi0 = np.ascontiguousarray(np.atleast_2d(psi0), dtype=np.complex128)
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    S, n = psi0.shape
+    corr = np.zeros((S, n_timestep + 1), dtype=np.complex128)
+    fs = ctypes.c_int(0)
+    rc = lib.orc_tbpm_dos(
+        ctypes.c_int64(n), _ptr(indptr, ctypes.c_int64),
+        _ptr(indices, ctypes.c_int64), _ptr(hop), _ptr(bes),
+        ctypes.c_int(len(bes)), _ptr(psi0), ctypes.c_int(S),
+        ctypes.c_int(n_timestep), ctypes.c_int(chk_step),
+        ctypes.c_double(chk_thr), _ptr(corr), ctypes.byref(fs))
+    _check_rc(rc, fs, "orc_tbpm_dos")
+    return corr
+
+
+def orc_tbpm_accond(indptr, indices, hop, dx, dy, rescale, beta_re, mu_re,
+                    bes, psi0, n_timestep, nr_fermi=2 ** 15,
+                    fermi_prec=1e-12, chk_step=128, chk_thr=1e-9):
+    """tbpm_accond (tbpm.f90:193-351); per-sample (S, 4, T)."""
+    lib = oracle_lib()
+    indptr, indices, hop = _csr_args(indptr, indices, hop)
+    dx = np.ascontiguousarray(dx, dtype=np.float64)
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    psi0 = np.ascontiguousarray(np.atleast_2d(psi0), dtype=np.complex128)
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    S, n = psi0.shape
+    corr = np.zeros((S, 4, n_timestep), dtype=np.complex128)
+    fs = ctypes.c_int(0)
+    rc = lib.orc_tbpm_accond(
+        ctypes.c_int64(n), _ptr(indptr, ctypes.c_int64),
+        _ptr(indices, ctypes.c_int64), _ptr(hop), _ptr(dx), _ptr(dy),
+        ctypes.c_double(rescale), ctypes.c_double(beta_re),
+        ctypes.c_double(mu_re), ctypes.c_int64(nr_fermi),
+        ctypes.c_double(fermi_prec), _ptr(bes), ctypes.c_int(len(bes)),
+        _ptr(psi0), ctypes.c_int(S), ctypes.c_int(n_timestep),
+        ctypes.c_int(chk_step), ctypes.c_double(chk_thr), _ptr(corr),
+        ctypes.byref(fs))
+    _check_rc(rc, fs, "orc_tbpm_accond")
+    return corr
+
+
+def orc_tbpm_dccond(indptr, indices, hop, dx, dy, rescale, bes, psi0,
+                    n_timestep, t_step, energies, en_inds, chk_step=128,
+                    chk_thr=1e-9):
+    """tbpm_dccond (tbpm.f90:497-718).
+
+    Returns per-sample ``dos (S, T)`` and ``dc (S, 2, n_en, T)``.
+    """
+    lib = oracle_lib()
+    indptr, indices, hop = _csr_args(indptr, indices, hop)
+    dx = np.ascontiguousarray(dx, dtype=np.float64)
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    psi0 = np.ascontiguousarray(np.atleast_2d(psi0), dtype=np.complex128)
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    energies = np.ascontiguousarray(energies, dtype=np.float64)
+    en_inds = np.ascontiguousarray(en_inds, dtype=np.int64)
+    S, n = psi0.shape
+    n_en = len(en_inds)
+    dos = np.zeros((S, n_timestep), dtype=np.complex128)
+    dc = np.zeros((S, 2, n_en, n_timestep), dtype=np.complex128)
+    fs = ctypes.c_int(0)
+    rc = lib.orc_tbpm_dccond(
+        ctypes.c_int64(n), _ptr(indptr, ctypes.c_int64),
+        _ptr(indices, ctypes.c_int64), _ptr(hop), _ptr(dx), _ptr(dy),
+        ctypes.c_double(rescale), _ptr(bes), ctypes.c_int(len(bes)),
+        _ptr(psi0), ctypes.c_int(S), ctypes.c_int(n_timestep),
+        ctypes.c_double(t_step), _ptr(energies),
+        _ptr(en_inds, ctypes.c_int64), ctypes.c_int(n_en),
+        ctypes.c_int(chk_step), ctypes.c_double(chk_thr), _ptr(dos),
+        _ptr(dc), ctypes.byref(fs))
+    _check_rc(rc, fs, "orc_tbpm_dccond")
+    return dos, dc
+
+
+def orc_step(indptr, indices, hop, bes, psi, fwd=True):
+    """One cheb_wf_timestep (propagation.f90:12-49)."""
+    lib = oracle_lib()
+    indptr, indices, hop = _csr_args(indptr, indices, hop)
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    out = np.zeros_like(psi)
+    rc = lib.orc_step(ctypes.c_int64(len(psi)), _ptr(indptr, ctypes.c_int64),
+                      _ptr(indices, ctypes.c_int64), _ptr(hop), _ptr(bes),
+                      ctypes.c_int(len(bes)), _ptr(psi), _ptr(out),
+                      ctypes.c_int(1 if fwd else 0))
+    if rc:
+        raise RuntimeError(f"orc_step returned {rc}")
+    return out
+
+
+def orc_csr_mv(indptr, indices, val, x):
+    """csr_mv (csr.F90:71-101)."""
+    lib = oracle_lib()
+    indptr, indices, val = _csr_args(indptr, indices, val)
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(x)
+    lib.orc_csr_mv(ctypes.c_int64(len(x)), _ptr(indptr, ctypes.c_int64),
+                   _ptr(indices, ctypes.c_int64), _ptr(val), _ptr(x),
+                   _ptr(out))
+    return out
+
+
+def orc_fermi_coef(nr_fermi, beta_re, mu_re, one_minus, eps):
+    """get_Fermi_cheb_coef (funcs.f90:60-123); returns the kept coefficients."""
+    lib = oracle_lib()
+    coef = np.zeros(nr_fermi, dtype=np.float64)
+    nc = ctypes.c_int64(0)
+    rc = lib.orc_fermi_coef(ctypes.c_int64(nr_fermi), ctypes.c_double(beta_re),
+                            ctypes.c_double(mu_re),
+                            ctypes.c_int(1 if one_minus else 0),
+                            ctypes.c_double(eps), _ptr(coef), ctypes.byref(nc))
+    if rc:
+        raise RuntimeError(f"orc_fermi_coef returned {rc}")
+    return coef[:nc.value].copy()
+
+
+# ---------------------------------------------------------------------------
+# The reference Fortran, compiled from tbplas/fortran (oracle/_ref)
+# ---------------------------------------------------------------------------
+def _fstr(s):
+    b = s.encode()
+    return ctypes.c_char_p(b), ctypes.c_int64(len(b))
+
+
+def ref_random_state(n, iseed):
+    """random_state (random.f90:36-69) as compiled here (flang intrinsic RNG).
+
+    tbpm_* call it with ``iseed = seed * (i_sample + rank * n_ran_samples)``.
+    """
+    lib = ref_lib()
+    wf = np.zeros(n, dtype=np.complex128)
+    lib._QMrandomPrandom_state(_ptr(wf), ctypes.byref(ctypes.c_int(n)),
+                               ctypes.byref(ctypes.c_int(iseed)))
+    return wf
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def ref_tbpm_dos(bes, indptr, indices, hop, seed, n_timestep, n_samples,
+                 chk_step=128, chk_thr=1e-9):
+    """The reference tbpm_dos (tbpm.f90:6-93), sample-averaged (T+1,)."""
+    lib = ref_lib()
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    ip, ix = _i32(indptr), _i32(indices)
+    hop = np.ascontiguousarray(hop, dtype=np.complex128)
+    corr = np.zeros(n_timestep + 1, dtype=np.complex128)
+    fn, fl = _fstr("unused")
+    I = ctypes.c_int
+    lib.tbpm_dos_(_ptr(bes), ctypes.byref(I(len(bes))), _ptr(ip, I),
+                  ctypes.byref(I(len(ip))), _ptr(ix, I),
+                  ctypes.byref(I(len(ix))), _ptr(hop),
+                  ctypes.byref(I(len(hop))), ctypes.byref(I(seed)),
+                  ctypes.byref(I(n_timestep)), ctypes.byref(I(n_samples)),
+                  fn, _ptr(corr), ctypes.byref(I(0)),
+                  ctypes.byref(I(chk_step)),
+                  ctypes.byref(ctypes.c_double(chk_thr)), fl)
+    return corr
+
+
+def ref_tbpm_accond(bes, beta_re, mu_re, indptr, indices, hop, rescale, dx, dy,
+                    seed, n_timestep, n_samples, nr_fermi=2 ** 15,
+                    fermi_prec=1e-12, chk_step=128, chk_thr=1e-9):
+    """The reference tbpm_accond (tbpm.f90:193-351), averaged (4, T)."""
+    lib = ref_lib()
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    ip, ix = _i32(indptr), _i32(indices)
+    hop = np.ascontiguousarray(hop, dtype=np.complex128)
+    dx = np.ascontiguousarray(dx, dtype=np.float64)
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    corr = np.zeros((4, n_timestep), dtype=np.complex128, order="F")
+    fn, fl = _fstr("unused")
+    I, D = ctypes.c_int, ctypes.c_double
+    r = ctypes.byref
+    lib.tbpm_accond_(_ptr(bes), r(I(len(bes))), r(D(beta_re)), r(D(mu_re)),
+                     _ptr(ip, I), r(I(len(ip))), _ptr(ix, I), r(I(len(ix))),
+                     _ptr(hop), r(I(len(hop))), r(D(rescale)), _ptr(dx),
+                     r(I(len(dx))), _ptr(dy), r(I(len(dy))), r(I(seed)),
+                     r(I(n_timestep)), r(I(n_samples)), r(I(nr_fermi)),
+                     r(D(fermi_prec)), fn, _ptr(corr), r(I(0)), r(I(chk_step)),
+                     r(D(chk_thr)), fl)
+    return np.ascontiguousarray(corr)
+
+
+def ref_tbpm_dccond(bes, beta_re, mu_re, indptr, indices, hop, rescale, dx, dy,
+                    seed, n_timestep, n_samples, t_step, energies, en_inds,
+                    chk_step=128, chk_thr=1e-9):
+    """The reference tbpm_dccond (tbpm.f90:497-718): (T,), (2, nE_grid, T)."""
+    lib = ref_lib()
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    ip, ix = _i32(indptr), _i32(indices)
+    hop = np.ascontiguousarray(hop, dtype=np.complex128)
+    dx = np.ascontiguousarray(dx, dtype=np.float64)
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    energies = np.ascontiguousarray(energies, dtype=np.float64)
+    en = _i32(en_inds)
+    dos = np.zeros(n_timestep, dtype=np.complex128)
+    dc = np.zeros((2, len(energies), n_timestep), dtype=np.complex128,
+                  order="F")
+    f1, l1 = _fstr("unused1")
+    f2, l2 = _fstr("unused2")
+    I, D = ctypes.c_int, ctypes.c_double
+    r = ctypes.byref
+    lib.tbpm_dccond_(_ptr(bes), r(I(len(bes))), r(D(beta_re)), r(D(mu_re)),
+                     _ptr(ip, I), r(I(len(ip))), _ptr(ix, I), r(I(len(ix))),
+                     _ptr(hop), r(I(len(hop))), r(D(rescale)), _ptr(dx),
+                     r(I(len(dx))), _ptr(dy), r(I(len(dy))), r(I(seed)),
+                     r(I(n_timestep)), r(I(n_samples)), r(D(t_step)),
+                     _ptr(energies), r(I(len(energies))), _ptr(en, I),
+                     r(I(len(en))), f1, f2, _ptr(dos), _ptr(dc), r(I(0)),
+                     r(I(chk_step)), r(D(chk_thr)), l1, l2)
+    return dos, np.ascontiguousarray(dc)
+
+
+def ref_fermi_coef(nr_fermi, beta_re, mu_re, one_minus, eps):
+    """The reference get_Fermi_cheb_coef (funcs.f90:60-123)."""
+    lib = ref_lib()
+    coef = np.zeros(nr_fermi + 1, dtype=np.float64)
+    nc = ctypes.c_int(0)
+    fn = getattr(lib, "_QMfuncsPget_fermi_cheb_coef")
+    fn(_ptr(coef), ctypes.byref(nc), ctypes.byref(ctypes.c_int(nr_fermi)),
+       ctypes.byref(ctypes.c_double(beta_re)),
+       ctypes.byref(ctypes.c_double(mu_re)),
+       ctypes.byref(ctypes.c_int(1 if one_minus else 0)),
+       ctypes.byref(ctypes.c_double(eps)))
+    return coef[:nc.value].copy()
+
+
+# ---------------------------------------------------------------------------
+# Host-side restatements
+# ---------------------------------------------------------------------------
+def bessel_series(time_scaled, precision=1e-14, bessel_max=250):
+    """Solver._get_bessel_series (solver.py:194-224) via scipy.special.jv.
+
+    ``time_scaled = time_step * rescale``.  Keeps J_i while |J_i| > precision
+    or |J_{i+1}| > precision; raises if Bessel_max is reached first.
+    """
+    import scipy.special as spec
+    out = []
+    for i in range(bessel_max):
+        v = spec.jv(i, time_scaled)
+        if abs(v) > precision:
+            out.append(float(v))
+        else:
+            if abs(spec.jv(i + 1, time_scaled)) > precision:
+                out.append(float(v))
+            else:
+                return out
+    raise ValueError("Bessel_max too low")
+
+
+def build_ham_csr(orb_eng, hop_i, hop_j, hop_v):
+    """Sample.build_ham_csr (sample.py:375-391): scipy dia + csr + csr^H.
+
+    scipy is the reference's own third-party dependency here; its canonical
+    form (sorted columns, duplicates summed, explicit zeros dropped) is the
+    contract.  Returns (indptr, indices, data) as int64/int64/complex128.
+    """
+    from scipy.sparse import dia_matrix, csr_matrix
+    n = len(orb_eng)
+    shape = (n, n)
+    ham_dia = dia_matrix((orb_eng, 0), shape=shape)
+    ham_half = csr_matrix((hop_v, (hop_i, hop_j)), shape=shape)
+    ham = ham_dia + ham_half + ham_half.getH()
+    return (ham.indptr.astype(np.int64), ham.indices.astype(np.int64),
+            ham.data.astype(np.complex128))
+
+
+def build_ham_dxy_loop(orb_eng, hop_i, hop_j, hop_v, dr, cutoff=1e-5):
+    """build_ham_dxy_fast (sample.pyx:148-257), loop for loop (small inputs)."""
+    n, m = len(orb_eng), len(hop_i)
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    for io in range(n):
+        if abs(orb_eng[io]) >= cutoff:
+            indptr[io + 1] += 1
+    for ih in range(m):
+        indptr[hop_i[ih] + 1] += 1
+        indptr[hop_j[ih] + 1] += 1
+    for io in range(1, n + 1):
+        indptr[io] = indptr[io - 1] + indptr[io]
+    nnz = indptr[n]
+    indices = np.zeros(nnz, dtype=np.int64)
+    hop = np.zeros(nnz, dtype=np.complex128)
+    dx = np.zeros(nnz, dtype=np.float64)
+    dy = np.zeros(nnz, dtype=np.float64)
+    cnt = np.zeros(n, dtype=np.int64)
+
+    def free_ptr(i):  # _get_free_ptr (sample.pyx:117-145)
+        p = indptr[i] + cnt[i]
+        cnt[i] += 1
+        return p
+
+    for io in range(n):
+        if abs(orb_eng[io]) >= cutoff:
+            p = free_ptr(io)
+            indices[p] = io
+            hop[p] = orb_eng[io]
+    for ih in range(m):
+        ii, jj = hop_i[ih], hop_j[ih]
+        p = free_ptr(ii)
+        indices[p] = jj
+        hop[p] = hop_v[ih]
+        dx[p] = dr[ih, 0]
+        dy[p] = dr[ih, 1]
+        p = free_ptr(jj)
+        indices[p] = ii
+        hop[p] = np.conj(hop_v[ih])
+        dx[p] = -dr[ih, 0]
+        dy[p] = -dr[ih, 1]
+    return indptr, indices, hop, dx, dy
+
+
+def build_ham_dxy_stable(orb_eng, hop_i, hop_j, hop_v, dr, cutoff=1e-5):
+    """Same order as build_ham_dxy_loop, vectorised with a stable sort.
+
+    The reference fills each row in arrival order: every kept diagonal first
+    (row io), then for each hop ih the entry (ii, jj) followed by (jj, ii).
+    A stable sort by row of that emission stream reproduces it exactly.
+    Pinned against build_ham_dxy_loop on small inputs.
+    """
+    n = len(orb_eng)
+    hop_i = np.asarray(hop_i, dtype=np.int64)
+    hop_j = np.asarray(hop_j, dtype=np.int64)
+    keep = np.nonzero(np.abs(orb_eng) >= cutoff)[0].astype(np.int64)
+    m = len(hop_i)
+    rows = np.empty(len(keep) + 2 * m, dtype=np.int64)
+    cols = np.empty_like(rows)
+    vals = np.empty(len(rows), dtype=np.complex128)
+    dxs = np.zeros(len(rows))
+    dys = np.zeros(len(rows))
+    k = len(keep)
+    rows[:k] = keep
+    cols[:k] = keep
+    vals[:k] = np.asarray(orb_eng)[keep]
+    rows[k::2], rows[k + 1::2] = hop_i, hop_j
+    cols[k::2], cols[k + 1::2] = hop_j, hop_i
+    vals[k::2], vals[k + 1::2] = hop_v, np.conj(hop_v)
+    dxs[k::2], dxs[k + 1::2] = dr[:, 0], -dr[:, 0]
+    dys[k::2], dys[k + 1::2] = dr[:, 1], -dr[:, 1]
+    order = np.argsort(rows, kind="stable")
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(rows, minlength=n), out=indptr[1:])
+    return indptr, cols[order], vals[order], dxs[order], dys[order]
+
+
+def graphene_hops_loop(L1, L2, t=-2.7, pbc=(True, True), a=0.246):
+    """Graphene supercell hop list, loop for loop.
+
+    Primitive cell: make_graphene_diamond (materials/graphene.py:16-31),
+    lattice a=0.246 nm at gamma=60 deg, orbitals at (0,0) and (1/3,1/3).
+    Reduced hop order from check_conj (builder/base.py:76-92) on the
+    insertion order of add_hopping: [(0,0,0,0,1), (1,0,0,1,0), (0,1,0,1,0)].
+    Supercell order: build_hop_pbc (super.pyx:860-986) loops ia, ib, ic, ih;
+    orbital ids from _id_pc2sc (super.pyx:19-43); positions build_orb_pos
+    (super.pyx:569-612); dr includes the lattice shift of wrapped cells
+    (_fast_div, super.pyx:496-539).  Open directions drop hops leaving the
+    cell (split_pc_hop / build_hop_gen, super.pyx:643-857), which for this
+    cell keeps the same relative order.
+    Returns hop_i, hop_j, hop_v, dr, orb_pos.
+    """
+    a1 = np.array([a, 0.0, 0.0])
+    gam = 60.0 / 180 * math.pi     # gen_lattice_vectors (base/lattice.py:36-41)
+    a2 = np.array([a * math.cos(gam), a * math.sin(gam), 0.0])
+    a3 = np.array([0.0, 0.0, 1.0])
+    lat = np.array([a1, a2, a3])
+    frac = np.array([[0.0, 0.0, 0.0], [1 / 3., 1 / 3., 0.0]])
+    pc_hops = [(0, 0, 0, 0, 1), (1, 0, 0, 1, 0), (0, 1, 0, 1, 0)]
+    dim = (L1, L2, 1)
+    n = 2 * L1 * L2
+    orb_pos = np.zeros((n, 3))
+    for ia in range(L1):
+        for ib in range(L2):
+            for io in range(2):
+                idx = (ia * L2 + ib) * 2 + io
+                ra, rb, rc = frac[io, 0] + ia, frac[io, 1] + ib, frac[io, 2]
+                orb_pos[idx] = ra * lat[0] + rb * lat[1] + rc * lat[2]
+    sc_lat = np.array([lat[0] * L1, lat[1] * L2, lat[2]])
+    hi, hj, dr = [], [], []
+    per = (bool(pbc[0]), bool(pbc[1]))
+    # split_pc_hop (super.pyx:786-857): a hop is "free" if it crosses an open
+    # direction; periodic hops are emitted first by build_hop_pbc, free ones
+    # afterwards by build_hop_gen (super.py:703-741).
+    hp = [h for h in pc_hops if all(per[d] or h[d] == 0 for d in range(2))]
+    hf = [h for h in pc_hops if h not in hp]
+
+    def emit(ia, ib, h):
+        ra, rb, rc, oi, oj = h
+        ja0, jb0 = ia + ra, ib + rb
+        if (not per[0] and not 0 <= ja0 < L1) or \
+                (not per[1] and not 0 <= jb0 < L2):
+            return
+        ja, jb = ja0 % L1, jb0 % L2
+        i = (ia * L2 + ib) * 2 + oi
+        j = (ja * L2 + jb) * 2 + oj
+        na, nb = ja0 // L1, jb0 // L2
+        hi.append(i)
+        hj.append(j)
+        dr.append(orb_pos[j] - orb_pos[i] + na * sc_lat[0] + nb * sc_lat[1])
+
+    for ia in range(L1):                 # build_hop_pbc: ia, ib, (ic), ih
+        for ib in range(L2):
+            for h in hp:
+                emit(ia, ib, h)
+    for io in range(n):                  # build_hop_gen: io, ih (same orb_i)
+        ia, ib, oo = io // (2 * L2), (io // 2) % L2, io % 2
+        for h in hf:
+            if h[3] == oo:
+                emit(ia, ib, h)
+    m = len(hi)
+    return (np.array(hi, dtype=np.int64), np.array(hj, dtype=np.int64),
+            np.full(m, t, dtype=np.complex128), np.array(dr).reshape(m, 3),
+            orb_pos)
+
+
+def get_rescale_loop(orb_eng, hop_i, hop_j, hop_v):
+    """get_rescale (sample.pyx:75-114): max row sum of |H| (on-site signed)."""
+    s = np.array(orb_eng, dtype=np.float64).copy()
+    for ih in range(len(hop_i)):
+        v = abs(hop_v[ih])
+        s[hop_i[ih]] += v
+        s[hop_j[ih]] += v
+    return float(np.max(s))
+
+
+def set_mag_field_loop(hop_i, hop_j, hop_v, dr, orb_pos, intensity, gauge=0):
+    """set_mag_field (sample.pyx:18-72), in place on a copy of hop_v."""
+    factor = math.pi / 4135.666734
+    out = np.array(hop_v, dtype=np.complex128).copy()
+    for ih in range(len(hop_i)):
+        ii, jj = hop_i[ih], hop_j[ih]
+        if gauge == 0:
+            phase = factor * intensity * dr[ih, 0] * (orb_pos[jj, 1] + orb_pos[ii, 1])
+        elif gauge == 1:
+            phase = -factor * intensity * dr[ih, 1] * (orb_pos[jj, 0] + orb_pos[ii, 0])
+        else:
+            sx = orb_pos[jj, 0] + orb_pos[ii, 0]
+            sy = orb_pos[jj, 1] + orb_pos[ii, 1]
+            phase = factor * 0.5 * intensity * (dr[ih, 0] * sx - dr[ih, 1] * sy)
+        out[ih] = out[ih] * (math.cos(phase) + 1j * math.sin(phase))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Exact oracles (independent of the Chebyshev method)
+# ---------------------------------------------------------------------------
+def kspace_graphene_corr(L1, L2, hop, psi0, tau, steps, fwd=True):
+    """Exact <psi0| exp(-i Ht tau k) |psi0> for pristine periodic graphene.
+
+    ``hop`` is the (rescaled) nearest-neighbour hopping, ``tau`` the scaled
+    time step (time_step * rescale), orbital order (ia*L2+ib)*2+io as in
+    graphene_hops_loop.  H is block diagonal in k with the 2x2 block
+    [[0, f], [f*, 0]], f = hop + hop* (e^{-i k1} + e^{-i k2}); the weights of
+    the two bands come from one 2-D FFT of each sublattice of psi0.
+    ``steps`` is an iterable of integer time indices.
+    """
+    psi0 = np.asarray(psi0).reshape(L1, L2, 2)
+    A = np.fft.fft2(psi0[:, :, 0])
+    B = np.fft.fft2(psi0[:, :, 1])
+    k1 = 2 * np.pi * np.fft.fftfreq(L1)[:, None]
+    k2 = 2 * np.pi * np.fft.fftfreq(L2)[None, :]
+    f = hop + np.conj(hop) * (np.exp(-1j * k1) + np.exp(-1j * k2))
+    e = np.abs(f)
+    ph = np.where(e > 0, f / np.where(e > 0, e, 1), 1.0)
+    # eigenvectors (1, +-conj(ph))/sqrt2 for eigenvalues +-e
+    up = (A + ph * B) / np.sqrt(2)        # <v+|.> with v+ = (1, conj(ph))/sqrt2
+    dn = (A - ph * B) / np.sqrt(2)
+    N = L1 * L2
+    wu = (np.abs(up) ** 2).ravel() / N
+    wd = (np.abs(dn) ** 2).ravel() / N
+    e = e.ravel()
+    sgn = -1.0 if fwd else 1.0
+    out = []
+    for k in steps:
+        th = sgn * tau * k * e
+        out.append(np.sum(wu * np.exp(1j * th)) + np.sum(wd * np.exp(-1j * th)))
+    return np.array(out, dtype=np.complex128)
+
+
+def dense_corr(indptr, indices, data, psi0, tau, steps):
+    """Exact <psi0|exp(-i H tau k)|psi0> by dense eigendecomposition (n<=4000)."""
+    n = len(indptr) - 1
+    Hd = np.zeros((n, n), dtype=np.complex128)
+    for r in range(n):
+        for p in range(indptr[r], indptr[r + 1]):
+            Hd[r, indices[p]] += data[p]
+    w, v = np.linalg.eigh(Hd)
+    c = v.conj().T @ np.asarray(psi0)
+    wt = np.abs(c) ** 2
+    return np.array([np.sum(wt * np.exp(-1j * w * tau * k)) for k in steps])
+
+
+# ---------------------------------------------------------------------------
+# Analyzer restatements (analysis.py), loop for loop
+# ---------------------------------------------------------------------------
+def window_hanning(i, tnr):
+    """analysis.py:20-28"""
+    return 0.5 * (1 + math.cos(math.pi * i / tnr))
+
+
+def window_exp(i, tnr):
+    """analysis.py:31-39"""
+    return math.exp(-2. * (i / tnr) ** 2)
+
+
+def analyzer_calc_dos(corr_dos, tnr, en_range, correct_spin=False,
+                      window=window_hanning):
+    """Analyzer.calc_dos (analysis.py:89-143)."""
+    en_step = 0.5 * en_range / tnr
+    energies = np.array([0.5 * i * en_range / tnr - en_range / 2.
+                         for i in range(tnr * 2)], dtype=float)
+    dos = np.zeros(tnr * 2, dtype=float)
+    cn = np.empty(tnr * 2, dtype=complex)
+    cn[tnr - 1] = corr_dos.item(0)
+    cn[2 * tnr - 1] = window(tnr - 1, tnr) * corr_dos.item(tnr)
+    for i in range(tnr - 1):
+        cn[tnr + i] = window(i, tnr) * corr_dos.item(i + 1)
+        cn[tnr - i - 2] = window(i, tnr) * corr_dos.item(i + 1).conjugate()
+    cf = np.fft.ifft(cn)
+    for i in range(tnr):
+        dos[i + tnr] = abs(cf.item(i))
+    for i in range(tnr, 2 * tnr):
+        dos[i - tnr] = abs(cf.item(i))
+    dos = dos / (np.sum(dos) * en_step)
+    if correct_spin:
+        dos = 2. * dos
+    return energies, dos
+
+
+def analyzer_calc_ac_cond(corr_ac, tnr, en_range, beta, ac_prefactor,
+                          correct_spin=False, window=window_exp):
+    """Analyzer.calc_ac_cond (analysis.py:161-243)."""
+    from scipy.signal import hilbert
+    t_step = math.pi / en_range
+    omegas = np.array([i * en_range / tnr for i in range(tnr)], dtype=float)
+    ac_real = np.zeros((4, tnr), dtype=float)
+    for j in range(4):
+        for i in range(tnr):
+            omega = omegas.item(i)
+            acv = 0.
+            for k in range(tnr):
+                acv += 2. * window(k + 1, tnr) * math.sin(omega * k * t_step) \
+                    * corr_ac.item(j, k).imag
+            if omega == 0.:
+                acv = 0.
+            else:
+                acv = ac_prefactor * t_step * acv \
+                    * (math.exp(-beta * omega) - 1) / omega
+            ac_real[j, i] = acv
+    ac_imag = np.zeros((4, tnr), dtype=float)
+    for j in range(4):
+        sigma = np.zeros(2 * tnr, dtype=float)
+        for i in range(tnr):
+            sigma[tnr + i] = ac_real.item(j, i)
+            sigma[tnr - i] = ac_real.item(j, i)
+        ac_imag[j, :] = np.imag(hilbert(sigma))[tnr:2 * tnr]
+    ac = ac_real + 1j * ac_imag
+    if correct_spin:
+        ac = 2. * ac
+    return omegas, ac
+
+
+def analyzer_calc_dc_cond(corr_dos, corr_dc, tnr, en_range, energy_limits,
+                          dc_prefactor, correct_spin=False,
+                          window_dos=window_hanning, window_dc=window_exp):
+    """Analyzer.calc_dc_cond (analysis.py:375-458)."""
+    corr_dos = np.insert(corr_dos, 0, 1.0)
+    energies_dos, dos = analyzer_calc_dos(corr_dos, tnr, en_range, False,
+                                          window_dos)
+    t_step = 2 * math.pi / en_range
+    qe = np.where((energies_dos >= energy_limits[0]) &
+                  (energies_dos <= energy_limits[1]))[0]
+    energies = energies_dos[qe]
+    dc = np.zeros((2, len(qe)))
+    for i in range(2):
+        for j in range(len(qe)):
+            en = energies.item(j)
+            dos_val = dos.item(qe.item(j))
+            dc_val = 0.
+            for k in range(tnr):
+                w = window_dc(k + 1, tnr)
+                phi = k * t_step * en
+                c_exp = math.cos(phi) - 1j * math.sin(phi)
+                dc_val += w * (c_exp * corr_dc.item(i, j, k)).real
+            dc[i, j] = dc_prefactor * t_step * dos_val * dc_val
+    if correct_spin:
+        dc = 2. * dc
+    return energies, dc

@@ -1,0 +1,166 @@
+"""ctypes binding of the C ABI in ``include/tbplas_amd.h``.
+
+Device buffers are PyTorch tensors on the GPU, used only as storage: their
+``data_ptr()`` is what crosses the boundary.  There is no CPU fallback: if
+the HIP library is missing or no GPU is visible the calls raise.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libtbplas_amd.so")
+
+TBG_OK, TBG_ENORM, TBG_EINVAL, TBG_EOOM, TBG_EHIP = 0, 1, 2, 3, 4
+TBG_FP64, TBG_FP32 = 0, 1
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+_SIGS = {
+    "tbg_version": [],
+    "tbg_last_error": [],
+    "tbg_create": [ctypes.POINTER(_P), _I],
+    "tbg_destroy": [_P],
+    "tbg_set_graph": [_P, _I],
+    "tbg_ham_dxy_count": [_P, _I64, _I64, _P, _P, _P, _D, _P, ctypes.POINTER(_I64)],
+    "tbg_ham_dxy_fill": [_P, _I64, _I64, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P, _P],
+    "tbg_ham_csr_count": [_P, _I64, _I64, _P, _P, _P, _P, _P, ctypes.POINTER(_I64)],
+    "tbg_ham_csr_fill": [_P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P],
+    "tbg_get_rescale": [_P, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(_D)],
+    "tbg_ham_create": [_P, _I64, _P, _P, _P, _P, _P, _D, _I, ctypes.POINTER(_P)],
+    "tbg_ham_destroy": [_P],
+    "tbg_ham_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                     ctypes.POINTER(_I64), ctypes.POINTER(_I)],
+    "tbg_corr_dos": [_P, _P, _P, _I, _P, _I, _I, _I, _D, _P, ctypes.POINTER(_I)],
+    "tbg_corr_ac": [_P, _P, _P, _I, _P, _I, _P, _I, _P, _I, _I, _I, _D, _P,
+                    ctypes.POINTER(_I)],
+    "tbg_corr_dc": [_P, _P, _P, _I, _P, _I, _I, _D, _P, _P, _I, _I, _D, _P, _P,
+                    ctypes.POINTER(_I)],
+    "tbg_cheb_step": [_P, _P, _P, _I, _P, _I, _I],
+    "tbg_cheb_poly": [_P, _P, _P, _I, _P, _I],
+    "tbg_corr_dos_batch": [_P, _I, _P, _P, _I, _P, _I, _I, _I, _D, _P,
+                           ctypes.POINTER(_I), ctypes.POINTER(_I)],
+    "tbg_peierls_host": [_I64, _P, _P, _P, _P, _P, _D, _I],
+}
+_RESTYPE = {"tbg_last_error": ctypes.c_char_p}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class TBGError(RuntimeError):
+    """A library call failed (status code in ``.code``)."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class NormError(TBGError):
+    """The wave-function norm check failed (the reference STOPs here,
+    tbplas/fortran/funcs.f90:175-192)."""
+
+    def __init__(self, code, msg, step):
+        super().__init__(code, msg)
+        self.step = step
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"{LIB_PATH} not found: build the HIP extension first "
+                    "(python -m tbplas_amd.build)")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, args in _SIGS.items():
+                fn = getattr(L, name)
+                fn.argtypes = args
+                fn.restype = _RESTYPE.get(name, ctypes.c_int)
+            _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc, fail_step=None):
+    if rc == TBG_OK:
+        return
+    msg = lib().tbg_last_error().decode(errors="replace")
+    if rc == TBG_ENORM:
+        raise NormError(rc, msg, None if fail_step is None else fail_step.value)
+    raise TBGError(rc, msg)
+
+
+def torch_cuda():
+    """Return torch after checking a GPU is visible (no CPU fallback)."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("tbplas_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch
+
+
+class Handle:
+    """One library handle (device + stream).  Calls are synchronous."""
+
+    def __init__(self, device=0, use_graph=True):
+        torch_cuda()
+        h = _P()
+        check(lib().tbg_create(ctypes.byref(h), int(device)))
+        self._h = h
+        self.device = int(device)
+        self.set_graph(use_graph)
+
+    @property
+    def ptr(self):
+        return self._h
+
+    def set_graph(self, enable):
+        check(lib().tbg_set_graph(self._h, 1 if enable else 0))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tbg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_handles = {}
+
+
+def default_handle(device=0):
+    h = _handles.get(device)
+    if h is None:
+        h = Handle(device)
+        _handles[device] = h
+    return h
+
+
+def dev(a, dtype=None, device=0):
+    """Host array -> contiguous device tensor (storage only)."""
+    torch = torch_cuda()
+    a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype=dtype))
+    return torch.from_numpy(a).to(f"cuda:{device}")
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr() if t is not None and t.numel() > 0 else 0)
+
+
+def sync(device=0):
+    torch_cuda().cuda.synchronize(device)

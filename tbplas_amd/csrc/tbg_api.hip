@@ -1,0 +1,435 @@
+// C ABI of the library (declared in include/tbplas_amd.h) and the DOS,
+// step, polynomial and batched drivers.
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
+
+#include "tbg_cond.h"
+#include "tbg_csr.h"
+#include "tbg_engine.h"
+
+namespace tbg {
+
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+
+template <typename F>
+static int guard(F&& f) {
+    try {
+        f();
+        g_err.clear();
+        return TBG_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host out of memory");
+        return TBG_EOOM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return TBG_EINVAL;
+    }
+}
+
+static void check_handle(tbg_handle* h) {
+    TBG_REQUIRE(h != nullptr, "null handle");
+    TBG_HIP(hipSetDevice(h->device));
+}
+
+// Target number of persistent blocks: enough waves to cover latency on all
+// CUs without making chunks smaller than one 4-slice tile.
+static int chunk_target(tbg_handle* h, int64_t nslices) {
+    const int64_t by_cu = (int64_t)h->num_cus * 8;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(by_cu, (nslices + 3) / 4));
+}
+
+// Norm check of the reference (funcs.f90:175-192): |n - n_ref| > thr or NaN.
+static bool norm_broken(double n, double ref, double thr) {
+    const double d = std::fabs(n - ref);
+    return std::isnan(d) || !std::isfinite(d) || d > thr;
+}
+
+template <typename R>
+static int corr_dos_impl(tbg_handle* h, int n_inst, const tbg_ham* const* hams, const double* bes,
+                         int nb, const double* const* psi0, int K, int T, int chk_step,
+                         double chk_thr, double* const* corr, int* fail_inst, int* fail_step) {
+    // Concatenate instances (block diagonal) when more than one.
+    std::vector<int64_t> inst_slices(1, 0);
+    for (int i = 0; i < n_inst; ++i) inst_slices.push_back(inst_slices.back() + hams[i]->nslices);
+    Concat cat;
+    MatView mv;
+    if (n_inst == 1) {
+        mv = view_of(hams[0]);
+    } else {
+        concat_hams(h, n_inst, hams, cat);
+        mv = cat.view();
+    }
+    Engine<R> e;
+    e.init(h, mv, inst_slices, K, chunk_target(h, mv.nslices));
+    const int Kp = e.Kp;
+    const int nv = 3 * Kp;
+    DevBuf psi((size_t)e.vbytes);
+    launch_fill_zero(psi.p, e.vbytes, h->stream);
+    for (int i = 0; i < n_inst; ++i) {
+        char* base = reinterpret_cast<char*>(psi.p) +
+                     (size_t)inst_slices[i] * 64 * Kp * sizeof(typename CplxT<R>::T);
+        launch_pack<R>(psi0[i], K, hams[i]->n, base, Kp, 0, 0, h->stream);
+    }
+    TBG_HIP(hipMemcpyAsync(e.X.p, psi.p, e.vbytes, hipMemcpyDeviceToDevice, h->stream));
+    const int64_t stride = (int64_t)(T + 1) * nv;
+    DevBuf cbuf((size_t)n_inst * stride * sizeof(double));
+    DevBuf counter(sizeof(int));
+    TBG_HIP(hipMemsetAsync(counter.p, 0, sizeof(int), h->stream));
+    launch_dotnorm<R>(e.plan, psi.p, psi.p, Kp, Kp, e.part.template as<double>(), h->stream);
+    launch_reduce(e.part.template as<double>(), nv, e.plan.inst_chunk_dev.template as<int>(), n_inst,
+                  cbuf.as<double>(), stride, counter.as<int>(), 0, h->stream);
+    const StepPoly sp = time_poly(bes, nb);
+    auto after = [&]() {
+        launch_reduce(e.part.template as<double>(), nv, e.plan.inst_chunk_dev.template as<int>(),
+                      n_inst, cbuf.as<double>(), stride, counter.as<int>(), 0, h->stream);
+    };
+    e.run(T, sp, 0, 0, F_DOT | F_NORM, psi.p, counter.as<int>(), after);
+    std::vector<double> hb((size_t)n_inst * stride);
+    TBG_HIP(hipMemcpyAsync(hb.data(), cbuf.p, hb.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+    TBG_HIP(hipStreamSynchronize(h->stream));
+    int rc = TBG_OK;
+    for (int i = 0; i < n_inst; ++i) {
+        std::vector<double> out((size_t)K * (T + 1) * 2);
+        const double* b = hb.data() + (size_t)i * stride;
+        for (int k = 0; k < K; ++k)
+            for (int t = 0; t <= T; ++t) {
+                out[((size_t)k * (T + 1) + t) * 2 + 0] = b[(size_t)t * nv + 3 * k + 0];
+                out[((size_t)k * (T + 1) + t) * 2 + 1] = b[(size_t)t * nv + 3 * k + 1];
+            }
+        TBG_HIP(hipMemcpyAsync(corr[i], out.data(), out.size() * sizeof(double),
+                               hipMemcpyHostToDevice, h->stream));
+        TBG_HIP(hipStreamSynchronize(h->stream));
+        // tbpm.f90:73-78: norm_ref after step 1, checked on the state of
+        // step chk_step-1 at loop index chk_step (2..T).
+        if (rc == TBG_OK && chk_step >= 2 && chk_step <= T) {
+            for (int k = 0; k < K; ++k) {
+                const double ref = b[(size_t)1 * nv + 3 * k + 2];
+                const double nn = b[(size_t)(chk_step - 1) * nv + 3 * k + 2];
+                if (norm_broken(nn, ref, chk_thr)) {
+                    rc = TBG_ENORM;
+                    if (fail_inst) *fail_inst = i;
+                    if (fail_step) *fail_step = chk_step;
+                    set_error("wave function norm broken at step " + std::to_string(chk_step) +
+                              " (state " + std::to_string(k) + ", instance " +
+                              std::to_string(i) + "): |" + std::to_string(nn) + " - " +
+                              std::to_string(ref) + "| > threshold; check the Hermiticity of "
+                              "the Hamiltonian or increase the rescale factor");
+                    break;
+                }
+            }
+        }
+    }
+    if (rc != TBG_OK) throw Error{rc, g_err};
+    return rc;
+}
+
+template <typename R>
+static void step_impl(tbg_handle* h, const tbg_ham* H, const StepPoly& sp, double* psi, int K,
+                      uint64_t bwd) {
+    Engine<R> e;
+    e.init(h, view_of(H), {0, H->nslices}, K, chunk_target(h, H->nslices));
+    launch_pack<R>(psi, K, H->n, e.X.p, e.Kp, 0, 0, h->stream);
+    e.step(sp, bwd, 0, 0, nullptr, nullptr);
+    launch_unpack<R>(e.X.p, e.Kp, 0, H->n, psi, K, h->stream);
+    TBG_HIP(hipStreamSynchronize(h->stream));
+}
+
+static void check_psi_args(const tbg_ham* H, const double* psi0, int K, int T) {
+    TBG_REQUIRE(H != nullptr, "null Hamiltonian");
+    TBG_REQUIRE(psi0 != nullptr, "null initial states");
+    TBG_REQUIRE(K >= 1 && K <= 64, "number of states must be in [1, 64]");
+    TBG_REQUIRE(T >= 1, "number of time steps must be >= 1");
+}
+
+}  // namespace tbg
+
+using namespace tbg;
+
+extern "C" {
+
+int tbg_version(void) { return 10000; }
+
+const char* tbg_last_error(void) { return g_err.c_str(); }
+
+int tbg_create(tbg_handle** out, int device) {
+    return guard([&] {
+        TBG_REQUIRE(out != nullptr, "null output pointer");
+        int ndev = 0;
+        TBG_HIP(hipGetDeviceCount(&ndev));
+        TBG_REQUIRE(device >= 0 && device < ndev, "device index out of range");
+        std::unique_ptr<tbg_handle> h(new tbg_handle());
+        h->device = device;
+        TBG_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        TBG_HIP(hipGetDeviceProperties(&prop, device));
+        h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        TBG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        *out = h.release();
+    });
+}
+
+int tbg_destroy(tbg_handle* h) {
+    return guard([&] {
+        if (!h) return;
+        (void)hipSetDevice(h->device);
+        if (h->stream) {
+            (void)hipStreamSynchronize(h->stream);
+            (void)hipStreamDestroy(h->stream);
+        }
+        delete h;
+    });
+}
+
+int tbg_set_graph(tbg_handle* h, int enable) {
+    return guard([&] {
+        check_handle(h);
+        h->use_graph = enable ? 1 : 0;
+    });
+}
+
+int tbg_ham_dxy_count(tbg_handle* h, int64_t n, int64_t m, const double* orb_eng,
+                      const int64_t* hop_i, const int64_t* hop_j, double cutoff, int64_t* indptr,
+                      int64_t* nnz) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(orb_eng && indptr && nnz && (m == 0 || (hop_i && hop_j)), "null argument");
+        DevBuf tmp;
+        *nnz = place_count(h, n, m, orb_eng, 1, cutoff, hop_i, hop_j, indptr, tmp);
+    });
+}
+
+int tbg_ham_dxy_fill(tbg_handle* h, int64_t n, int64_t m, const double* orb_eng,
+                     const int64_t* hop_i, const int64_t* hop_j, const double* hop_v,
+                     const double* dr, double cutoff, const int64_t* indptr, int64_t* indices,
+                     double* hop, double* dx, double* dy) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(orb_eng && indptr && indices && hop && dx && dy &&
+                        (m == 0 || (hop_i && hop_j && hop_v && dr)),
+                    "null argument");
+        dxy_fill(h, n, m, orb_eng, hop_i, hop_j, hop_v, dr, cutoff, indptr, indices, hop, dx, dy);
+    });
+}
+
+int tbg_ham_csr_count(tbg_handle* h, int64_t n, int64_t m, const double* orb_eng,
+                      const int64_t* hop_i, const int64_t* hop_j, const double* hop_v,
+                      int64_t* indptr, int64_t* nnz) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(orb_eng && indptr && nnz && (m == 0 || (hop_i && hop_j && hop_v)),
+                    "null argument");
+        TBG_REQUIRE(n > 0 && m >= 0, "bad sizes");
+        *nnz = canon_build(h, n, m, orb_eng, hop_i, hop_j, hop_v, indptr, nullptr, nullptr,
+                           nullptr);
+    });
+}
+
+int tbg_ham_csr_fill(tbg_handle* h, int64_t n, int64_t m, const double* orb_eng,
+                     const int64_t* hop_i, const int64_t* hop_j, const double* hop_v,
+                     const int64_t* indptr, int64_t* indices, double* data) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(orb_eng && indptr && indices && data && (m == 0 || (hop_i && hop_j && hop_v)),
+                    "null argument");
+        TBG_REQUIRE(n > 0 && m >= 0, "bad sizes");
+        canon_build(h, n, m, orb_eng, hop_i, hop_j, hop_v, nullptr, indptr, indices, data);
+    });
+}
+
+int tbg_get_rescale(tbg_handle* h, int64_t n, int64_t m, const double* orb_eng,
+                    const int64_t* hop_i, const int64_t* hop_j, const double* hop_v,
+                    double* rescale) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(orb_eng && rescale && (m == 0 || (hop_i && hop_j && hop_v)), "null argument");
+        TBG_REQUIRE(n > 0 && m >= 0, "bad sizes");
+        *rescale = get_rescale(h, n, m, orb_eng, hop_i, hop_j, hop_v);
+    });
+}
+
+// Host code: same operation order and libm cos/sin as set_mag_field
+// (tbplas/cython/sample.pyx:18-72), no contraction into FMA.
+int tbg_peierls_host(int64_t m, const int64_t* hop_i, const int64_t* hop_j, double* hop_v,
+                     const double* dr, const double* orb_pos, double intensity, int gauge) {
+#pragma clang fp contract(off)
+    return guard([&] {
+        TBG_REQUIRE(m >= 0 && (m == 0 || (hop_i && hop_j && hop_v && dr && orb_pos)),
+                    "null argument");
+        TBG_REQUIRE(gauge >= 0 && gauge <= 2, "gauge must be 0, 1 or 2");
+        const double factor = M_PI / 4135.666734;
+        for (int64_t ih = 0; ih < m; ++ih) {
+            const int64_t ii = hop_i[ih], jj = hop_j[ih];
+            double phase;
+            if (gauge == 0) {
+                const double sy = orb_pos[jj * 3 + 1] + orb_pos[ii * 3 + 1];
+                phase = factor * intensity * dr[ih * 3 + 0] * sy;
+            } else if (gauge == 1) {
+                const double sx = orb_pos[jj * 3 + 0] + orb_pos[ii * 3 + 0];
+                phase = -factor * intensity * dr[ih * 3 + 1] * sx;
+            } else {
+                const double sx = orb_pos[jj * 3 + 0] + orb_pos[ii * 3 + 0];
+                const double sy = orb_pos[jj * 3 + 1] + orb_pos[ii * 3 + 1];
+                const double p1 = dr[ih * 3 + 0] * sx;
+                const double p2 = dr[ih * 3 + 1] * sy;
+                phase = factor * 0.5 * intensity * (p1 - p2);
+            }
+            const double c = std::cos(phase), s = std::sin(phase);
+            const double hr = hop_v[ih * 2 + 0], hi = hop_v[ih * 2 + 1];
+            const double t1 = hr * c, t2 = hi * s, t3 = hr * s, t4 = hi * c;
+            hop_v[ih * 2 + 0] = t1 - t2;
+            hop_v[ih * 2 + 1] = t3 + t4;
+        }
+    });
+}
+
+int tbg_ham_create(tbg_handle* h, int64_t n, const int64_t* indptr, const int64_t* indices,
+                   const double* hop, const double* dx, const double* dy, double cur_scale,
+                   int precision, tbg_ham** out) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(out && indptr && indices && hop, "null argument");
+        TBG_REQUIRE(n > 0 && n < (int64_t(1) << 31) - 64, "n must be in (0, 2^31 - 64)");
+        TBG_REQUIRE(precision == TBG_FP64 || precision == TBG_FP32, "unknown precision");
+        TBG_REQUIRE((dx == nullptr) == (dy == nullptr), "dx and dy must be given together");
+        std::unique_ptr<tbg_ham> H(new tbg_ham());
+        H->h = h;
+        H->n = n;
+        H->precision = precision;
+        build_sell(h, H.get(), indptr, indices, hop, dx, dy, cur_scale);
+        *out = H.release();
+    });
+}
+
+int tbg_ham_destroy(tbg_ham* H) {
+    return guard([&] {
+        if (!H) return;
+        (void)hipSetDevice(H->h->device);
+        delete H;
+    });
+}
+
+int tbg_ham_info(const tbg_ham* H, int64_t* n, int64_t* nnz, int64_t* padded, int* max_width) {
+    return guard([&] {
+        TBG_REQUIRE(H != nullptr, "null Hamiltonian");
+        if (n) *n = H->n;
+        if (nnz) *nnz = H->nnz;
+        if (padded) *padded = H->padded;
+        if (max_width) *max_width = H->max_width;
+    });
+}
+
+int tbg_corr_dos(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
+                 const double* psi0, int K, int T, int chk_step, double chk_thr, double* corr,
+                 int* fail_step) {
+    return guard([&] {
+        check_handle(h);
+        check_psi_args(H, psi0, K, T);
+        TBG_REQUIRE(bes && n_bes >= 3 && corr, "need >= 3 Bessel terms and an output");
+        const tbg_ham* hs[1] = {H};
+        const double* ps[1] = {psi0};
+        double* cs[1] = {corr};
+        if (H->precision == TBG_FP32)
+            corr_dos_impl<float>(h, 1, hs, bes, n_bes, ps, K, T, chk_step, chk_thr, cs, nullptr,
+                                 fail_step);
+        else
+            corr_dos_impl<double>(h, 1, hs, bes, n_bes, ps, K, T, chk_step, chk_thr, cs, nullptr,
+                                  fail_step);
+    });
+}
+
+int tbg_corr_dos_batch(tbg_handle* h, int n_inst, const tbg_ham* const* hams, const double* bes,
+                       int n_bes, const double* const* psi0, int K, int T, int chk_step,
+                       double chk_thr, double* const* corr, int* fail_inst, int* fail_step) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(n_inst >= 1 && hams && psi0 && corr, "null argument");
+        TBG_REQUIRE(bes && n_bes >= 3, "need >= 3 Bessel terms");
+        for (int i = 0; i < n_inst; ++i) {
+            check_psi_args(hams[i], psi0[i], K, T);
+            TBG_REQUIRE(corr[i] != nullptr, "null output");
+            TBG_REQUIRE(hams[i]->precision == hams[0]->precision,
+                        "all instances must share one precision");
+        }
+        if (hams[0]->precision == TBG_FP32)
+            corr_dos_impl<float>(h, n_inst, hams, bes, n_bes, psi0, K, T, chk_step, chk_thr, corr,
+                                 fail_inst, fail_step);
+        else
+            corr_dos_impl<double>(h, n_inst, hams, bes, n_bes, psi0, K, T, chk_step, chk_thr,
+                                  corr, fail_inst, fail_step);
+    });
+}
+
+int tbg_cheb_step(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes, double* psi,
+                  int K, int fwd) {
+    return guard([&] {
+        check_handle(h);
+        check_psi_args(H, psi, K, 1);
+        TBG_REQUIRE(bes && n_bes >= 3, "need >= 3 Bessel terms");
+        const StepPoly sp = time_poly(bes, n_bes);
+        const uint64_t bwd = fwd ? 0ull : ~0ull;
+        if (H->precision == TBG_FP32) step_impl<float>(h, H, sp, psi, K, bwd);
+        else step_impl<double>(h, H, sp, psi, K, bwd);
+    });
+}
+
+int tbg_cheb_poly(tbg_handle* h, const tbg_ham* H, const double* coef, int n_coef, double* psi,
+                  int K) {
+    return guard([&] {
+        check_handle(h);
+        check_psi_args(H, psi, K, 1);
+        TBG_REQUIRE(coef && n_coef >= 3, "need >= 3 coefficients");
+        const StepPoly sp = cheb_poly(coef, n_coef, coef, n_coef);
+        if (H->precision == TBG_FP32) step_impl<float>(h, H, sp, psi, K, 0);
+        else step_impl<double>(h, H, sp, psi, K, 0);
+    });
+}
+
+int tbg_corr_ac(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
+                const double* coef_F, int n_F, const double* coef_omF, int n_omF,
+                const double* psi0, int K, int T, int chk_step, double chk_thr, double* corr,
+                int* fail_step) {
+    return guard([&] {
+        check_handle(h);
+        check_psi_args(H, psi0, K, T);
+        TBG_REQUIRE(H->has_current, "Hamiltonian was prepared without current operators");
+        TBG_REQUIRE(bes && n_bes >= 3 && corr, "need >= 3 Bessel terms and an output");
+        TBG_REQUIRE(coef_F && coef_omF && n_F >= 3 && n_omF >= 3, "need >= 3 Fermi coefficients");
+        TBG_REQUIRE(3 * K <= 64, "at most 21 states per AC call");
+        if (H->precision == TBG_FP32)
+            corr_ac_impl<float>(h, H, bes, n_bes, coef_F, n_F, coef_omF, n_omF, psi0, K, T,
+                                chk_step, chk_thr, corr, fail_step);
+        else
+            corr_ac_impl<double>(h, H, bes, n_bes, coef_F, n_F, coef_omF, n_omF, psi0, K, T,
+                                 chk_step, chk_thr, corr, fail_step);
+    });
+}
+
+int tbg_corr_dc(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
+                const double* psi0, int K, int T, double t_step, const double* energies,
+                const int64_t* en_inds, int n_en, int chk_step, double chk_thr, double* dos_corr,
+                double* dc_corr, int* fail_step) {
+    return guard([&] {
+        check_handle(h);
+        check_psi_args(H, psi0, K, T);
+        TBG_REQUIRE(H->has_current, "Hamiltonian was prepared without current operators");
+        TBG_REQUIRE(bes && n_bes >= 3 && dos_corr && dc_corr, "need >= 3 Bessel terms and outputs");
+        TBG_REQUIRE(n_en >= 0 && (n_en == 0 || (energies && en_inds)), "bad energy arguments");
+        TBG_REQUIRE(2 * K <= 64, "at most 32 states per DC call");
+        if (H->precision == TBG_FP32)
+            corr_dc_impl<float>(h, H, bes, n_bes, psi0, K, T, t_step, energies, en_inds, n_en,
+                                chk_step, chk_thr, dos_corr, dc_corr, fail_step);
+        else
+            corr_dc_impl<double>(h, H, bes, n_bes, psi0, K, T, t_step, energies, en_inds, n_en,
+                                 chk_step, chk_thr, dos_corr, dc_corr, fail_step);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,219 @@
+// Step driver shared by the correlation entry points.
+#pragma once
+
+#include <algorithm>
+#include <type_traits>
+
+#include "tbg_prop.h"
+
+namespace tbg {
+
+// Matrix operand of a run: one prepared Hamiltonian or the block-diagonal
+// concatenation of several.
+struct MatView {
+    const int* cols = nullptr;
+    const void* vals = nullptr;
+    const int64_t* soff = nullptr;
+    const int* sw = nullptr;
+    int64_t nslices = 0;
+};
+
+inline MatView view_of(const tbg_ham* H) {
+    MatView m;
+    m.cols = H->cols.as<int>();
+    m.vals = H->vals.p;
+    m.soff = H->slice_off.as<int64_t>();
+    m.sw = H->slice_w.as<int>();
+    m.nslices = H->nslices;
+    return m;
+}
+
+// Vector slots: a power of two >= the vectors needed (lanes per row must
+// divide the wavefront).
+inline int slots_for(int k_needed) {
+    int kp = 1;
+    while (kp < k_needed) kp <<= 1;
+    return kp;
+}
+
+// Polynomial of one step: terms v_0..v_{N-1} with v_1 = alpha1 H v_0 and
+// v_m = alpha H v_{m-1} + beta v_{m-2}; output sum_m c[cls][m] v_m.
+struct StepPoly {
+    std::vector<double> c[2];
+    int N = 0;
+    double ar1 = 0, ai1 = 0, ar = 0, ai = 0, beta = 0;
+};
+
+// exp(-i H~ tau) from the Bessel series (propagation.f90:12-49):
+// c_0 = J_0, c_m = 2 J_m, alpha1 = -i, alpha = -2i, beta = 1.
+inline StepPoly time_poly(const double* bes, int nb) {
+    StepPoly p;
+    p.N = nb;
+    p.c[0].resize(nb);
+    for (int m = 0; m < nb; ++m) p.c[0][m] = m == 0 ? bes[0] : 2.0 * bes[m];
+    p.c[1] = p.c[0];
+    p.ar1 = 0.0; p.ai1 = -1.0; p.ar = 0.0; p.ai = -2.0; p.beta = 1.0;
+    return p;
+}
+
+// sum_m c_m T_m(H~) (propagation.f90:52-82): alpha1 = 1, alpha = 2, beta = -1.
+inline StepPoly cheb_poly(const double* c0, int n0, const double* c1, int n1) {
+    StepPoly p;
+    p.N = std::max(n0, n1);
+    p.c[0].assign(p.N, 0.0);
+    p.c[1].assign(p.N, 0.0);
+    for (int m = 0; m < n0; ++m) p.c[0][m] = c0[m];
+    for (int m = 0; m < n1; ++m) p.c[1][m] = c1[m];
+    p.ar1 = 1.0; p.ai1 = 0.0; p.ar = 2.0; p.ai = 0.0; p.beta = -1.0;
+    return p;
+}
+
+template <typename R>
+struct Engine {
+    tbg_handle* h = nullptr;
+    MatView M;
+    Plan plan;
+    int Kp = 1, lgG = 0;
+    int64_t n_pad = 0;
+    DevBuf X, P, Y, part;
+    size_t vbytes = 0;
+
+    void init(tbg_handle* hh, const MatView& m, const std::vector<int64_t>& inst_slices,
+              int k_needed, int chunk_target) {
+        h = hh;
+        M = m;
+        Kp = slots_for(k_needed);
+        TBG_REQUIRE(Kp <= 64, "too many vectors for one launch (at most 64 slots)");
+        const int V = (std::is_same<R, float>::value && Kp >= 2) ? 2 : 1;
+        lgG = 0;
+        while ((1 << lgG) * V < Kp) ++lgG;
+        make_plan(h, plan, inst_slices, chunk_target);
+        n_pad = m.nslices * 64;
+        vbytes = (size_t)n_pad * Kp * sizeof(typename CplxT<R>::T);
+        X.alloc(vbytes);
+        P.alloc(vbytes);
+        Y.alloc(vbytes);
+        part.alloc((size_t)plan.nchunks * 3 * Kp * sizeof(double) + 64);
+        launch_fill_zero(X.p, vbytes, h->stream);
+        launch_fill_zero(P.p, vbytes, h->stream);
+        launch_fill_zero(Y.p, vbytes, h->stream);
+    }
+
+    TermArgs base_args(uint64_t bwd, uint64_t cls) const {
+        TermArgs a{};
+        a.cols = M.cols;
+        a.vals = M.vals;
+        a.soff = M.soff;
+        a.sw = M.sw;
+        a.chunk = plan.chunk.as<int64_t>();
+        a.nslices = M.nslices;
+        a.K = Kp;
+        a.lgG = lgG;
+        a.bwd_mask = bwd;
+        a.cls_mask = cls;
+        a.nv = 3 * Kp;
+        return a;
+    }
+
+    // One step: X holds the input, the result is left in X (buffers swap).
+    void step(const StepPoly& sp, uint64_t bwd, uint64_t cls, int epi, const void* psi0,
+              int* counter) {
+        const int N = sp.N;
+        TBG_REQUIRE(N >= 3, "at least 3 expansion terms are required");
+        auto buf = [&](int j) { return (j % 2 == 0) ? X.p : P.p; };
+        for (int m = 1; m < N; ++m) {
+            TermArgs a = base_args(bwd, cls);
+            a.acc = Y.p;
+            int fl;
+            if (m == 1) {
+                a.pin = X.p;
+                a.pout = P.p;
+                a.ar[0] = a.ar[1] = sp.ar1;
+                a.ai[0] = a.ai[1] = sp.ai1;
+                a.counter = counter;
+                fl = F_STORE;
+            } else {
+                const bool last = (m == N - 1);
+                a.pin = buf(m - 1);
+                a.pprev = buf(m - 2);
+                a.pout = buf(m);
+                a.ar[0] = a.ar[1] = sp.ar;
+                a.ai[0] = a.ai[1] = sp.ai;
+                a.beta = sp.beta;
+                fl = F_PREV | (last ? 0 : F_STORE);
+                auto setc = [&](double* dst, int idx) {
+                    dst[0] = sp.c[0][idx];
+                    dst[1] = sp.c[1][idx];
+                };
+                if (m == 2) {
+                    fl |= F_ACC | F_ACCPREV | F_MID;
+                    setc(a.cp, 0); setc(a.cm, 1); setc(a.cc, 2);
+                } else if (m % 3 == 2) {
+                    fl |= F_ACC | F_ACCRMW | F_ACCPREV | F_MID;
+                    setc(a.cp, m - 2); setc(a.cm, m - 1); setc(a.cc, m);
+                } else if (last) {
+                    if (m % 3 == 0) {
+                        fl |= F_ACC | F_ACCRMW;
+                        setc(a.cc, m);
+                    } else {
+                        fl |= F_ACC | F_ACCRMW | F_MID;
+                        setc(a.cm, m - 1); setc(a.cc, m);
+                    }
+                }
+                if (last) {
+                    fl |= epi;
+                    a.psi0 = psi0;
+                    a.part = part.as<double>();
+                }
+            }
+            launch_term<R>(fl, a, plan.nchunks, h->stream);
+        }
+        std::swap(X, Y);
+    }
+
+    // Steps with an optional per-step callback (after each step), replayed
+    // from captured graphs of an even number of steps when enabled.
+    template <typename F>
+    void run(int nsteps, const StepPoly& sp, uint64_t bwd, uint64_t cls, int epi,
+             const void* psi0, int* counter, F&& after) {
+        int done = 0;
+        const int per = std::min(16, nsteps & ~1);
+        if (h->use_graph && per >= 2 && nsteps / per >= 1) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            TBG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                for (int s = 0; s < per; ++s) {
+                    step(sp, bwd, cls, epi, psi0, counter);
+                    after();
+                }
+            } catch (...) {
+                hipGraph_t dummy = nullptr;
+                (void)hipStreamEndCapture(h->stream, &dummy);
+                if (dummy) (void)hipGraphDestroy(dummy);
+                throw;
+            }
+            TBG_HIP(hipStreamEndCapture(h->stream, &g));
+            hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                TBG_HIP(e);
+            }
+            const int reps = nsteps / per;
+            for (int r = 0; r < reps; ++r) {
+                e = hipGraphLaunch(ge, h->stream);
+                if (e != hipSuccess) break;
+            }
+            (void)hipGraphExecDestroy(ge);
+            (void)hipGraphDestroy(g);
+            TBG_HIP(e);
+            done = reps * per;
+        }
+        for (; done < nsteps; ++done) {
+            step(sp, bwd, cls, epi, psi0, counter);
+            after();
+        }
+    }
+};
+
+}  // namespace tbg

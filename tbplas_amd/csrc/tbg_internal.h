@@ -1,0 +1,110 @@
+// Internal definitions shared by the tbplas_amd HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/tbplas_amd.h"
+
+namespace tbg {
+
+// ---------------------------------------------------------------------------
+// Errors: every entry point returns a status code; the message of the last
+// failure on the calling thread is kept for tbg_last_error().
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define TBG_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess)                                                  \
+            throw ::tbg::Error{TBG_EHIP, std::string(#call) + ": " +           \
+                                             hipGetErrorString(e_)};           \
+    } while (0)
+
+#define TBG_REQUIRE(cond, msg)                                                 \
+    do {                                                                       \
+        if (!(cond))                                                           \
+            throw ::tbg::Error{TBG_EINVAL, (msg)};                             \
+    } while (0)
+
+// Device memory owned by the library (freed in the destructor).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t b) { alloc(b); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t b) {
+        release();
+        if (b == 0) b = 16;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error{TBG_EOOM, "hipMalloc(" + std::to_string(b) + " bytes) failed: " +
+                                      hipGetErrorString(e)};
+        }
+        bytes = b;
+    }
+    void ensure(size_t b) { if (b > bytes) alloc(b); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Complex storage type of a working precision.
+template <typename R> struct CplxT;
+template <> struct CplxT<float> { using T = float2; };
+template <> struct CplxT<double> { using T = double2; };
+
+// Rows per slice of the sliced-ELL layout: one wavefront owns one slice.
+constexpr int kSlice = 64;
+constexpr int kWave = 64;
+
+}  // namespace tbg
+
+// Library handle: one per host thread; all work is ordered on `stream`.
+struct tbg_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 256;
+    int use_graph = 1;
+    tbg::DevBuf scratch;      // reduction partials and small tables
+    tbg::DevBuf counter;      // step counter for graph-replayed reductions
+};
+
+// Hamiltonian prepared for propagation: sliced-ELL (64-row slices, column
+// major inside a slice), int32 columns, complex values in the working
+// precision; optional current operators on the same structure.
+struct tbg_ham {
+    tbg_handle* h = nullptr;
+    int64_t n = 0;
+    int64_t nnz = 0;
+    int64_t nslices = 0;
+    int64_t padded = 0;       // stored elements including padding
+    int precision = TBG_FP64;
+    int max_width = 0;
+    tbg::DevBuf slice_off;    // int64 [nslices+1] element offsets
+    tbg::DevBuf slice_w;      // int32 [nslices]
+    tbg::DevBuf cols;         // int32 [padded]
+    tbg::DevBuf vals;         // complex<R> [padded]
+    tbg::DevBuf jx, jy;       // complex<R> [padded] current operators (optional)
+    bool has_current = false;
+};

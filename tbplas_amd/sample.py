@@ -1,0 +1,391 @@
+"""Drop-in ``Sample`` for the TBPM path, and sample generators.
+
+``Sample`` holds the same arrays as the reference's Sample
+(tbplas/builder/sample.py:24-106): ``orb_eng``, ``orb_pos``, ``hop_i``,
+``hop_j``, ``hop_v``, ``dr`` and ``_rescale``.  Its Hamiltonian builders run
+on the GPU through the C ABI:
+
+* ``build_ham_csr``  -> scipy-canonical CSR (sample.py:375-391)
+* ``build_ham_dxy``  -> insertion-order CSR + dx, dy (sample.py:417-461)
+* ``rescale_ham``    -> sample.py:272-299 (estimate on the GPU when factor
+  is None, sample.pyx:75-114)
+* ``set_magnetic_field`` -> Peierls phases (sample.py:301-354,
+  sample.pyx:18-72), in the library's host code with libm cos/sin.
+
+A Sample is built from arrays, from a reference Sample (``from_tbplas``), from
+files written by the reference's ``save_array`` (``load_array``), or by the
+graphene generator ``make_graphene_sample`` below.
+"""
+import math
+import os
+
+import numpy as np
+
+from . import _lib
+
+__all__ = ["Sample", "make_graphene_sample", "graphene_hops",
+           "add_anderson_disorder", "apply_uniaxial_strain", "random_vacancies"]
+
+GRAPHENE_A = 0.246      # nm, make_graphene_diamond (materials/graphene.py:16-31)
+GRAPHENE_T = -2.7       # eV
+
+
+def graphene_hops(L1, L2, t=GRAPHENE_T, pbc=(True, True), vacancies=None,
+                  a=GRAPHENE_A):
+    """Graphene supercell arrays in the reference's order, vectorised.
+
+    Primitive cell of make_graphene_diamond (materials/graphene.py:16-31):
+    lattice a at 60 deg, orbitals at (0,0) and (1/3,1/3), reduced hops
+    [(0,0)0->1, (1,0)1->0, (0,1)1->0] (order from check_conj,
+    builder/base.py:76-92).  Orbital order (ia, ib, io) (super.pyx:383-443,
+    vacancies skipped).  Hop order: per cell (ia, ib), the A->B hop then the
+    two B->A hops, as build_hop_pbc / build_hop_gen emit them for this cell
+    (super.pyx:643-986); hops touching a vacancy are dropped; for open
+    directions the crossing hops come after all periodic ones
+    (split_pc_hop, super.pyx:786-857).
+    Returns orb_pos (n,3), hop_i, hop_j, hop_v, dr (m,3).
+    """
+    L1, L2 = int(L1), int(L2)
+    a1 = np.array([a, 0.0, 0.0])
+    gam = 60.0 / 180 * math.pi     # gen_lattice_vectors (base/lattice.py:36-41)
+    a2 = np.array([a * math.cos(gam), a * math.sin(gam), 0.0])
+    lat = np.array([a1, a2, [0.0, 0.0, 1.0]])
+    n0 = 2 * L1 * L2
+    ia = np.repeat(np.arange(L1), L2)
+    ib = np.tile(np.arange(L2), L1)
+    # positions of all orbitals (before removing vacancies), build_orb_pos
+    pos = np.empty((n0, 3))
+    for io, (fa, fb) in enumerate(((0.0, 0.0), (1 / 3., 1 / 3.))):
+        ra = fa + ia
+        rb = fb + ib
+        rc = 0.0
+        pos[io::2] = (ra[:, None] * lat[0] + rb[:, None] * lat[1] + rc * lat[2])
+    sc1, sc2 = lat[0] * L1, lat[1] * L2
+    cell = ia * L2 + ib
+    pc = [(0, 0, 0, 1), (1, 0, 1, 0), (0, 1, 1, 0)]
+    per = (bool(pbc[0]), bool(pbc[1]))
+    periodic = [h for h in pc if (per[0] or h[0] == 0) and (per[1] or h[1] == 0)]
+    free = [h for h in pc if h not in periodic]
+
+    def emit(hops):
+        if not hops:
+            z = np.zeros(0, dtype=np.int64)
+            return z, z, np.zeros((0, 3)), np.zeros(0, dtype=bool)
+        ii, jj, dd, ok = [], [], [], []
+        for (ra, rb, oi, oj) in hops:
+            ja0, jb0 = ia + ra, ib + rb
+            valid = np.ones(L1 * L2, dtype=bool)
+            if not per[0]:
+                valid &= (ja0 >= 0) & (ja0 < L1)
+            if not per[1]:
+                valid &= (jb0 >= 0) & (jb0 < L2)
+            na, nb = np.floor_divide(ja0, L1), np.floor_divide(jb0, L2)
+            ja, jb = ja0 - na * L1, jb0 - nb * L2
+            i = cell * 2 + oi
+            j = (ja * L2 + jb) * 2 + oj
+            d = pos[j] - pos[i] + na[:, None] * sc1 + nb[:, None] * sc2
+            ii.append(i); jj.append(j); dd.append(d); ok.append(valid)
+        # interleave per cell: hop order inside a cell follows the list
+        k = len(hops)
+        I = np.stack(ii, axis=1).reshape(-1)
+        J = np.stack(jj, axis=1).reshape(-1)
+        D = np.stack(dd, axis=1).reshape(-1, 3)
+        V = np.stack(ok, axis=1).reshape(-1)
+        return I, J, D, V
+
+    I1, J1, D1, V1 = emit(periodic)
+    if free:
+        # build_hop_gen loops orbitals then hops with matching orb_i; all
+        # free hops of this cell start on orbital B, so cell order is kept.
+        I2, J2, D2, V2 = emit(free)
+        I = np.concatenate([I1, I2]); J = np.concatenate([J1, J2])
+        D = np.concatenate([D1, D2]); V = np.concatenate([V1, V2])
+    else:
+        I, J, D, V = I1, J1, D1, V1
+    if vacancies is not None and len(vacancies):
+        vac = np.zeros(n0, dtype=bool)
+        vac[np.asarray(vacancies, dtype=np.int64)] = True
+        V &= ~vac[I] & ~vac[J]
+        new_id = np.cumsum(~vac) - 1
+        keep_orb = ~vac
+    else:
+        new_id = np.arange(n0)
+        keep_orb = np.ones(n0, dtype=bool)
+    I, J, D = I[V], J[V], D[V]
+    hop_i = new_id[I].astype(np.int64)
+    hop_j = new_id[J].astype(np.int64)
+    hop_v = np.full(len(hop_i), t, dtype=np.complex128)
+    return pos[keep_orb].copy(), hop_i, hop_j, hop_v, np.ascontiguousarray(D)
+
+
+def random_vacancies(n_orb, fraction, seed):
+    """Sorted orbital indices of a seeded uniform vacancy draw."""
+    rng = np.random.default_rng(seed)
+    nv = int(round(fraction * n_orb))
+    return np.sort(rng.choice(n_orb, nv, replace=False)).astype(np.int64)
+
+
+class Sample:
+    """Arrays of a tight-binding sample plus GPU Hamiltonian builders."""
+
+    def __init__(self, orb_eng=None, orb_pos=None, hop_i=None, hop_j=None,
+                 hop_v=None, dr=None, rescale=1.0, area_unit_cell=None,
+                 volume_unit_cell=None, extended=1, nr_orbitals=None,
+                 device=0):
+        self.orb_eng = None if orb_eng is None else np.ascontiguousarray(orb_eng, dtype=np.float64)
+        self.orb_pos = None if orb_pos is None else np.ascontiguousarray(orb_pos, dtype=np.float64)
+        self.hop_i = None if hop_i is None else np.ascontiguousarray(hop_i, dtype=np.int64)
+        self.hop_j = None if hop_j is None else np.ascontiguousarray(hop_j, dtype=np.int64)
+        self.hop_v = None if hop_v is None else np.ascontiguousarray(hop_v, dtype=np.complex128)
+        self.dr = None if dr is None else np.ascontiguousarray(dr, dtype=np.float64).reshape(-1, 3)
+        self._rescale = float(rescale)
+        self._area_unit_cell = area_unit_cell
+        self._volume_unit_cell = volume_unit_cell
+        self._extended = extended
+        self._nr_orbitals = nr_orbitals
+        self.device = device
+
+    # -- construction -----------------------------------------------------
+    @classmethod
+    def from_tbplas(cls, sample, device=0):
+        """Copy the arrays of a reference ``tbplas.Sample``."""
+        sample.init_orb_eng()
+        sample.init_orb_pos()
+        sample.init_hop()
+        return cls(sample.orb_eng, sample.orb_pos, sample.hop_i, sample.hop_j,
+                   sample.hop_v, sample.dr, sample.rescale,
+                   area_unit_cell=sample.area_unit_cell,
+                   volume_unit_cell=sample.volume_unit_cell,
+                   extended=sample.extended, nr_orbitals=sample.nr_orbitals,
+                   device=device)
+
+    # -- reference API ----------------------------------------------------
+    def init_orb_eng(self, force_init=False):
+        if self.orb_eng is None:
+            raise ValueError("orb_eng not set")
+
+    def init_orb_pos(self, force_init=False):
+        if self.orb_pos is None:
+            raise ValueError("orb_pos not set")
+
+    def init_hop(self, force_init=False):
+        if self.hop_i is None:
+            raise ValueError("hopping arrays not set")
+
+    def save_array(self, data_dir="sample"):
+        """Same files as the reference (sample.py:220-240)."""
+        os.makedirs(data_dir, exist_ok=True)
+        if self.orb_eng is not None:
+            np.save(f"{data_dir}/orb_eng", self.orb_eng)
+        if self.orb_pos is not None:
+            np.save(f"{data_dir}/orb_pos", self.orb_pos)
+        if self.hop_i is not None:
+            np.save(f"{data_dir}/hop_i", self.hop_i)
+            np.save(f"{data_dir}/hop_j", self.hop_j)
+            np.save(f"{data_dir}/hop_v", self.hop_v)
+            np.save(f"{data_dir}/dr", self.dr)
+            np.save(f"{data_dir}/rescale", self._rescale)
+
+    def load_array(self, data_dir="sample"):
+        """Read files written by save_array (sample.py:242-262); numpy
+        loads with allow_pickle=False."""
+        def _ld(name):
+            return np.load(f"{data_dir}/{name}.npy", allow_pickle=False)
+        for name in ("orb_eng", "orb_pos"):
+            try:
+                setattr(self, name, _ld(name))
+            except FileNotFoundError:
+                print(f"Ignoring {data_dir}/{name}.npy")
+        try:
+            self.hop_i, self.hop_j = _ld("hop_i"), _ld("hop_j")
+            self.hop_v, self.dr = _ld("hop_v"), _ld("dr")
+            self._rescale = float(_ld("rescale").item())
+        except FileNotFoundError:
+            print(f"Ignoring {data_dir}/hop_i.npy")
+
+    def rescale_ham(self, factor=None):
+        """sample.py:272-299: divide by factor relative to the originals."""
+        self.init_orb_eng()
+        self.init_hop()
+        if factor is None:
+            factor = self.get_rescale()
+        self.orb_eng *= (self._rescale / factor)
+        self.hop_v *= (self._rescale / factor)
+        self._rescale = factor
+
+    def get_rescale(self):
+        """Max row sum of |H| (sample.pyx:75-114) on the GPU."""
+        import ctypes
+        h = _lib.default_handle(self.device)
+        d = _DevHops(self, self.device)
+        out = ctypes.c_double(0.0)
+        _lib.check(_lib.lib().tbg_get_rescale(
+            h.ptr, self.num_orb, self.num_hop, _lib.ptr(d.orb_eng), _lib.ptr(d.hop_i),
+            _lib.ptr(d.hop_j), _lib.ptr(d.hop_v), ctypes.byref(out)))
+        return out.value
+
+    def set_magnetic_field(self, intensity, gauge=0):
+        """Peierls substitution (sample.py:301-354, sample.pyx:18-72)."""
+        self.init_orb_pos()
+        self.init_hop()
+        if gauge not in (0, 1, 2):
+            raise ValueError(f"Illegal gauge {gauge}")
+        _lib.check(_lib.lib().tbg_peierls_host(
+            self.num_hop, self.hop_i.ctypes.data, self.hop_j.ctypes.data,
+            self.hop_v.ctypes.data, self.dr.ctypes.data, self.orb_pos.ctypes.data,
+            float(intensity), int(gauge)))
+
+    def build_ham_csr_device(self):
+        """Canonical CSR on the GPU: (indptr, indices, data) device tensors."""
+        import ctypes
+        torch = _lib.torch_cuda()
+        self.init_orb_eng()
+        self.init_hop()
+        h = _lib.default_handle(self.device)
+        d = _DevHops(self, self.device)
+        n, m = self.num_orb, self.num_hop
+        dev = f"cuda:{self.device}"
+        indptr = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        nnz = ctypes.c_int64(0)
+        L = _lib.lib()
+        _lib.check(L.tbg_ham_csr_count(h.ptr, n, m, _lib.ptr(d.orb_eng), _lib.ptr(d.hop_i),
+                                       _lib.ptr(d.hop_j), _lib.ptr(d.hop_v), _lib.ptr(indptr),
+                                       ctypes.byref(nnz)))
+        indices = torch.empty(max(nnz.value, 1), dtype=torch.int64, device=dev)
+        data = torch.empty(max(nnz.value, 1), dtype=torch.complex128, device=dev)
+        _lib.check(L.tbg_ham_csr_fill(h.ptr, n, m, _lib.ptr(d.orb_eng), _lib.ptr(d.hop_i),
+                                      _lib.ptr(d.hop_j), _lib.ptr(d.hop_v), _lib.ptr(indptr),
+                                      _lib.ptr(indices), _lib.ptr(data)))
+        return indptr, indices[:nnz.value], data[:nnz.value]
+
+    def build_ham_csr(self):
+        """Canonical sparse Hamiltonian as scipy.sparse.csr_matrix."""
+        from scipy.sparse import csr_matrix
+        ip, ix, dt = self.build_ham_csr_device()
+        n = self.num_orb
+        return csr_matrix((dt.cpu().numpy(), ix.cpu().numpy().astype(np.int32),
+                           ip.cpu().numpy().astype(np.int32)), shape=(n, n))
+
+    def build_ham_dxy_device(self, orb_eng_cutoff=1.0e-5):
+        """Insertion-order CSR + dx, dy on the GPU, as device tensors."""
+        import ctypes
+        torch = _lib.torch_cuda()
+        self.init_orb_eng()
+        self.init_orb_pos()
+        self.init_hop()
+        h = _lib.default_handle(self.device)
+        d = _DevHops(self, self.device, with_dr=True)
+        n, m = self.num_orb, self.num_hop
+        dev = f"cuda:{self.device}"
+        indptr = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        nnz = ctypes.c_int64(0)
+        L = _lib.lib()
+        _lib.check(L.tbg_ham_dxy_count(h.ptr, n, m, _lib.ptr(d.orb_eng), _lib.ptr(d.hop_i),
+                                       _lib.ptr(d.hop_j), float(orb_eng_cutoff),
+                                       _lib.ptr(indptr), ctypes.byref(nnz)))
+        k = max(nnz.value, 1)
+        indices = torch.empty(k, dtype=torch.int64, device=dev)
+        hop = torch.empty(k, dtype=torch.complex128, device=dev)
+        dx = torch.empty(k, dtype=torch.float64, device=dev)
+        dy = torch.empty(k, dtype=torch.float64, device=dev)
+        _lib.check(L.tbg_ham_dxy_fill(h.ptr, n, m, _lib.ptr(d.orb_eng), _lib.ptr(d.hop_i),
+                                      _lib.ptr(d.hop_j), _lib.ptr(d.hop_v), _lib.ptr(d.dr),
+                                      float(orb_eng_cutoff), _lib.ptr(indptr),
+                                      _lib.ptr(indices), _lib.ptr(hop), _lib.ptr(dx),
+                                      _lib.ptr(dy)))
+        z = nnz.value
+        return indptr, indices[:z], hop[:z], dx[:z], dy[:z]
+
+    def build_ham_dxy(self, orb_eng_cutoff=1.0e-5, algo="fast", sort_col=False):
+        """(indptr, indices, hop, dx, dy) numpy arrays (sample.py:417-461).
+
+        ``algo`` is accepted for compatibility; the GPU builder always
+        produces the "fast" (insertion) order.  ``sort_col`` sorts each row
+        by column like sort_col_csr (sample.pyx:385-439).
+        """
+        if algo not in ("fast", "safe"):
+            raise ValueError(f"unknown algo {algo}")
+        out = [t.cpu().numpy() for t in self.build_ham_dxy_device(orb_eng_cutoff)]
+        if sort_col:
+            indptr, indices, hop, dx, dy = out
+            for r in range(len(indptr) - 1):
+                a, b = indptr[r], indptr[r + 1]
+                o = np.argsort(indices[a:b], kind="stable")
+                for arr in (indices, hop, dx, dy):
+                    arr[a:b] = arr[a:b][o]
+        return tuple(out)
+
+    # -- properties (sample.py:644-738) --------------------------------------
+    @property
+    def rescale(self):
+        return self._rescale
+
+    @property
+    def num_orb(self):
+        return int(len(self.orb_eng))
+
+    @property
+    def num_hop(self):
+        return int(len(self.hop_i))
+
+    @property
+    def energy_range(self):
+        return 2.0 * self._rescale
+
+    @property
+    def area_unit_cell(self):
+        return self._area_unit_cell
+
+    @property
+    def volume_unit_cell(self):
+        return self._volume_unit_cell
+
+    @property
+    def extended(self):
+        return self._extended
+
+    @property
+    def nr_orbitals(self):
+        return self._nr_orbitals
+
+
+class _DevHops:
+    """Device copies of a sample's arrays for one builder call."""
+
+    def __init__(self, s, device, with_dr=False):
+        self.orb_eng = _lib.dev(s.orb_eng, np.float64, device)
+        self.hop_i = _lib.dev(s.hop_i, np.int64, device)
+        self.hop_j = _lib.dev(s.hop_j, np.int64, device)
+        self.hop_v = _lib.dev(s.hop_v, np.complex128, device)
+        self.dr = _lib.dev(s.dr, np.float64, device) if with_dr else None
+
+
+def make_graphene_sample(L1, L2=None, t=GRAPHENE_T, pbc=(True, True),
+                         vacancies=None, device=0):
+    """Graphene supercell sample (2 orbitals per cell, a = 0.246 nm)."""
+    L2 = L1 if L2 is None else L2
+    pos, hi, hj, hv, dr = graphene_hops(L1, L2, t, pbc, vacancies)
+    a = GRAPHENE_A
+    area = a * a * math.sin(math.pi / 3)
+    return Sample(np.zeros(len(pos)), pos, hi, hj, hv, dr, 1.0,
+                  area_unit_cell=area, volume_unit_cell=area * 1.0,
+                  extended=1, nr_orbitals=2, device=device)
+
+
+def add_anderson_disorder(sample, W, seed):
+    """orb_eng += U[-W/2, W/2] (seeded), applied before rescale_ham."""
+    rng = np.random.default_rng(seed)
+    sample.orb_eng = sample.orb_eng + rng.uniform(-0.5 * W, 0.5 * W, sample.num_orb)
+
+
+def apply_uniaxial_strain(sample, eps, axis=0, beta=3.37, d0=None):
+    """Stretch coordinates along ``axis`` by (1 + eps) and scale hoppings by
+    exp(-beta (d - d0) / d0), d the strained bond length."""
+    if d0 is None:
+        d0 = GRAPHENE_A / math.sqrt(3.0)
+    sample.orb_pos = sample.orb_pos.copy()
+    sample.orb_pos[:, axis] *= (1.0 + eps)
+    sample.dr = sample.dr.copy()
+    sample.dr[:, axis] *= (1.0 + eps)
+    d = np.linalg.norm(sample.dr, axis=1)
+    sample.hop_v = sample.hop_v * np.exp(-beta * (d - d0) / d0)

@@ -1,0 +1,249 @@
+"""Kernel-level Python API over the C ABI: prepared Hamiltonians and the
+correlation runs, with device tensors as storage.
+
+These functions take and return numpy arrays (host) for convenience; the
+arrays are staged through device tensors and the library works on those.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+
+__all__ = ["Hamiltonian", "bessel_series", "fermi_cheb_coef", "corr_dos",
+           "corr_dos_batch", "corr_ac", "corr_dc", "cheb_step", "cheb_poly",
+           "precision_code"]
+
+
+def precision_code(precision):
+    if precision in ("fp64", _lib.TBG_FP64):
+        return _lib.TBG_FP64
+    if precision in ("fp32", _lib.TBG_FP32):
+        return _lib.TBG_FP32
+    raise ValueError(f"unknown precision {precision!r}")
+
+
+class Hamiltonian:
+    """A prepared device Hamiltonian (tbg_ham_create).
+
+    indptr/indices/hop may be numpy arrays or device tensors (int64, int64,
+    complex128); dx/dy (float64) enable the current operators
+    J = (0, cur_scale) * hop * d (funcs.f90:12-33).
+    """
+
+    def __init__(self, n, indptr, indices, hop, dx=None, dy=None, cur_scale=1.0,
+                 precision="fp64", handle=None, device=0):
+        torch = _lib.torch_cuda()
+        self.handle = handle or _lib.default_handle(device)
+        dv = f"cuda:{self.handle.device}"
+
+        def to_dev(a, dt):
+            if a is None:
+                return None
+            if isinstance(a, torch.Tensor):
+                return a.to(device=dv, dtype=dt).contiguous()
+            return torch.from_numpy(np.ascontiguousarray(a)).to(device=dv, dtype=dt)
+
+        ip = to_dev(indptr, torch.int64)
+        ix = to_dev(indices, torch.int64)
+        hv = to_dev(hop, torch.complex128)
+        dxd = to_dev(dx, torch.float64)
+        dyd = to_dev(dy, torch.float64)
+        if ix.numel() == 0:
+            ix = torch.zeros(1, dtype=torch.int64, device=dv)
+            hv = torch.zeros(1, dtype=torch.complex128, device=dv)
+        self.n = int(n)
+        self.precision = precision_code(precision)
+        self.has_current = dx is not None
+        out = ctypes.c_void_p()
+        _lib.check(_lib.lib().tbg_ham_create(
+            self.handle.ptr, self.n, _lib.ptr(ip), _lib.ptr(ix), _lib.ptr(hv),
+            _lib.ptr(dxd) if dxd is not None else None,
+            _lib.ptr(dyd) if dyd is not None else None,
+            float(cur_scale), self.precision, ctypes.byref(out)))
+        self._h = out
+
+    @property
+    def ptr(self):
+        return self._h
+
+    def info(self):
+        n, nnz, pad = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        w = ctypes.c_int()
+        _lib.check(_lib.lib().tbg_ham_info(self._h, ctypes.byref(n), ctypes.byref(nnz),
+                                           ctypes.byref(pad), ctypes.byref(w)))
+        return {"n": n.value, "nnz": nnz.value, "padded": pad.value,
+                "max_width": w.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().tbg_ham_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bessel_series(time_scaled, precision=1e-14, bessel_max=250):
+    """Solver._get_bessel_series (solver.py:194-224), scipy.special.jv."""
+    import scipy.special as spec
+    series = []
+    for i in range(bessel_max):
+        v = spec.jv(i, time_scaled)
+        if abs(v) > precision:
+            series.append(float(v))
+        elif abs(spec.jv(i + 1, time_scaled)) > precision:
+            series.append(float(v))
+        else:
+            return series
+    raise ValueError("Bessel_max too low")
+
+
+def fermi_cheb_coef(nr_fermi, beta, mu, one_minus, eps):
+    """Chebyshev coefficients of the (one-minus) Fermi operator
+    (get_Fermi_cheb_coef, funcs.f90:60-123): FFT of f(cos(2 pi i / N)),
+    Jackson damping (kpm.f90:8-29), cut at the first two consecutive
+    coefficients below eps."""
+    nr = int(nr_fermi)
+    if nr < 4 or nr & (nr - 1):
+        raise ValueError("nr_Fermi_fft_steps must be a power of two >= 4")
+    r0 = 2 * math.pi / nr
+    energy = np.cos(np.arange(nr) * r0)
+    f = np.empty(nr)
+    hi = energy >= mu
+    x = np.exp(beta * (mu - energy[hi]))
+    f[hi] = x / (1 + x)
+    x = np.exp(beta * (energy[~hi] - mu))
+    f[~hi] = 1 / (1 + x)
+    if one_minus:
+        f = 1.0 - f
+    c = np.fft.fft(f.astype(np.complex128))
+    n = nr // 2
+    k = np.arange(1, n)
+    q = math.pi / n
+    cotq = math.cos(q) / math.sin(q)
+    g = np.empty(n)
+    g[0] = 0.5
+    g[1:] = ((n - k) * np.cos(q * k) + cotq * np.sin(q * k)) / n
+    coef = g * c.real[:n] / n
+    small = np.abs(coef) < eps
+    nxt = np.append(small[1:], True)
+    hit = np.nonzero(small & nxt)[0]
+    keep = int(hit[0]) + 1 if len(hit) else n
+    return coef[:keep].copy()
+
+
+def _states_dev(psi0, device):
+    psi0 = np.ascontiguousarray(np.atleast_2d(psi0), dtype=np.complex128)
+    return _lib.dev(psi0, np.complex128, device), psi0.shape
+
+
+def corr_dos(H, bes, psi0, n_timestep, chk_step=128, chk_thr=1e-9):
+    """Per-state DOS correlation (K, T+1) (tbpm_dos, tbpm.f90:6-93)."""
+    torch = _lib.torch_cuda()
+    dv = H.handle.device
+    p, (K, n) = _states_dev(psi0, dv)
+    if n != H.n:
+        raise ValueError("state length does not match the Hamiltonian")
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    out = torch.empty((K, n_timestep + 1), dtype=torch.complex128, device=f"cuda:{dv}")
+    fs = ctypes.c_int(0)
+    rc = _lib.lib().tbg_corr_dos(H.handle.ptr, H.ptr, bes.ctypes.data, len(bes), _lib.ptr(p),
+                                 K, int(n_timestep), int(chk_step), float(chk_thr),
+                                 _lib.ptr(out), ctypes.byref(fs))
+    _lib.check(rc, fs)
+    return out.cpu().numpy()
+
+
+def corr_dos_batch(hams, bes, psi0_list, n_timestep, chk_step=128, chk_thr=1e-9):
+    """DOS correlations of several Hamiltonians in one run per term."""
+    torch = _lib.torch_cuda()
+    handle = hams[0].handle
+    dv = handle.device
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    ps, outs = [], []
+    K = None
+    for H, psi in zip(hams, psi0_list):
+        p, (k, n) = _states_dev(psi, dv)
+        if n != H.n:
+            raise ValueError("state length does not match the Hamiltonian")
+        K = k if K is None else K
+        if k != K:
+            raise ValueError("every instance needs the same number of states")
+        ps.append(p)
+        outs.append(torch.empty((K, n_timestep + 1), dtype=torch.complex128, device=f"cuda:{dv}"))
+    ni = len(hams)
+    harr = (ctypes.c_void_p * ni)(*[H.ptr.value for H in hams])
+    parr = (ctypes.c_void_p * ni)(*[t.data_ptr() for t in ps])
+    oarr = (ctypes.c_void_p * ni)(*[t.data_ptr() for t in outs])
+    fi, fs = ctypes.c_int(-1), ctypes.c_int(0)
+    rc = _lib.lib().tbg_corr_dos_batch(handle.ptr, ni, harr, bes.ctypes.data, len(bes), parr, K,
+                                       int(n_timestep), int(chk_step), float(chk_thr), oarr,
+                                       ctypes.byref(fi), ctypes.byref(fs))
+    _lib.check(rc, fs)
+    return [o.cpu().numpy() for o in outs]
+
+
+def corr_ac(H, bes, coef_F, coef_omF, psi0, n_timestep, chk_step=128, chk_thr=1e-9):
+    """Per-state AC correlation (K, 4, T) (tbpm_accond, tbpm.f90:193-351)."""
+    torch = _lib.torch_cuda()
+    dv = H.handle.device
+    p, (K, n) = _states_dev(psi0, dv)
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    cF = np.ascontiguousarray(coef_F, dtype=np.float64)
+    cO = np.ascontiguousarray(coef_omF, dtype=np.float64)
+    out = torch.empty((K, 4, n_timestep), dtype=torch.complex128, device=f"cuda:{dv}")
+    fs = ctypes.c_int(0)
+    rc = _lib.lib().tbg_corr_ac(H.handle.ptr, H.ptr, bes.ctypes.data, len(bes), cF.ctypes.data,
+                                len(cF), cO.ctypes.data, len(cO), _lib.ptr(p), K,
+                                int(n_timestep), int(chk_step), float(chk_thr), _lib.ptr(out),
+                                ctypes.byref(fs))
+    _lib.check(rc, fs)
+    return out.cpu().numpy()
+
+
+def corr_dc(H, bes, psi0, n_timestep, t_step, energies, en_inds, chk_step=128,
+            chk_thr=1e-9):
+    """Per-state DC correlations (tbpm_dccond, tbpm.f90:497-718):
+    dos (K, T) and dc (K, 2, n_en, T)."""
+    torch = _lib.torch_cuda()
+    dv = H.handle.device
+    p, (K, n) = _states_dev(psi0, dv)
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    energies = np.ascontiguousarray(energies, dtype=np.float64)
+    en_inds = np.ascontiguousarray(en_inds, dtype=np.int64)
+    n_en = len(en_inds)
+    dos = torch.empty((K, n_timestep), dtype=torch.complex128, device=f"cuda:{dv}")
+    dc = torch.empty((K, 2, max(n_en, 1), n_timestep), dtype=torch.complex128,
+                     device=f"cuda:{dv}")
+    fs = ctypes.c_int(0)
+    rc = _lib.lib().tbg_corr_dc(H.handle.ptr, H.ptr, bes.ctypes.data, len(bes), _lib.ptr(p), K,
+                                int(n_timestep), float(t_step), energies.ctypes.data,
+                                en_inds.ctypes.data, n_en, int(chk_step), float(chk_thr),
+                                _lib.ptr(dos), _lib.ptr(dc), ctypes.byref(fs))
+    _lib.check(rc, fs)
+    return dos.cpu().numpy(), dc.cpu().numpy()[:, :, :n_en, :]
+
+
+def cheb_step(H, bes, psi, fwd=True):
+    """psi (K, n) -> exp(-+i H~ tau) psi (cheb_wf_timestep)."""
+    dv = H.handle.device
+    p, (K, n) = _states_dev(psi, dv)
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    _lib.check(_lib.lib().tbg_cheb_step(H.handle.ptr, H.ptr, bes.ctypes.data, len(bes),
+                                        _lib.ptr(p), K, 1 if fwd else 0))
+    return p.cpu().numpy()
+
+
+def cheb_poly(H, coef, psi):
+    """psi (K, n) -> sum_k c_k T_k(H~) psi (Fermi, propagation.f90:52-82)."""
+    dv = H.handle.device
+    p, (K, n) = _states_dev(psi, dv)
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    _lib.check(_lib.lib().tbg_cheb_poly(H.handle.ptr, H.ptr, coef.ctypes.data, len(coef),
+                                        _lib.ptr(p), K))
+    return p.cpu().numpy()
