@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Benchmark of the TBPM hot path on one MI355X.
+
+Workload (the metric's configuration): pristine graphene 2048 x 2048 with
+periodic boundaries (8,388,608 orbitals, 25,165,824 non-zeros), rescaled by
+9 eV, 8 random-phase states propagated together for 2048 Chebyshev time
+steps (21 Bessel terms at precision 1e-15, i.e. 20 SpMM terms per step) with
+the DOS correlation C(t) formed every step.  FP32 storage and arithmetic,
+FP64 correlation sums.
+
+A "step" is one time step of all 8 states; ``value`` is step*vectors/s over
+the timed steps with the states resident in HBM.  Printed as one JSON line,
+with
+  * ``roofline``: algorithmic bytes per Chebyshev term
+      nnz*(s_val + 4) + 4*(n + 1) + K*5*n*s_vec   (s = 8 B for complex64)
+    divided by the average time of one term over the timed region (HIP
+    events on the library's stream), against 8 TB/s;
+  * ``fp64_k1``: the plain FP64 single-state path on the same sample (the
+    speedup denominator) and ``speedup_vs_fp64_k1``;
+  * ``ct_err_fp32_vs_fp64``: max |C32(t) - C64(t)| over all timed steps for
+    state 0, both from the GPU on the identical initial state;
+  * ``cpu_baseline``: the reference's own Fortran tbpm_dos (compiled from its
+    sources into oracle/_ref) timed on the host cores on a bounded sample.
+
+``--gpus N`` with N > 1 exits: this build drives one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "TBPM step·vectors/s + % HBM roofline, 8.4M-orb graphene; C(t) err vs FP64"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2048)
+    p.add_argument("--warmup", type=int, default=16)
+    p.add_argument("--L", type=int, default=2048)
+    p.add_argument("--vectors", type=int, default=8)
+    p.add_argument("--fp64-steps", type=int, default=None,
+                   help="timed steps of the FP64 K=1 run (default: --steps)")
+    p.add_argument("--only-fp32", action="store_true",
+                   help="skip the FP64 and CPU legs (for profiling)")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-steps", type=int, default=16)
+    p.add_argument("--no-graph", action="store_true")
+    p.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
+    return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the reference Fortran, in a child process (no GPU there)
+# ---------------------------------------------------------------------------
+def cpu_child(args):
+    import numpy as np
+    import oracle
+    from tbplas_amd.sample import graphene_hops
+    L = args.L
+    pos, hi, hj, hv, dr = graphene_hops(L, L)
+    n = len(pos)
+    hv = hv * (1.0 / 9.0)
+    ip, ix, dt = oracle.build_ham_csr(np.zeros(n), hi, hj, hv)
+    bes = oracle.bessel_series(2 * math.pi / 18.0 * 9.0, 1e-15)
+    t0 = time.perf_counter()
+    oracle.ref_tbpm_dos(bes, ip, ix, dt, 1337, 1, 1, chk_step=0)
+    t1 = time.perf_counter()
+    oracle.ref_tbpm_dos(bes, ip, ix, dt, 1337, 1 + args.cpu_steps, 1, chk_step=0)
+    t2 = time.perf_counter()
+    per_step = ((t2 - t1) - (t1 - t0)) / args.cpu_steps
+    sys.stdout.flush()
+    print(json.dumps({"per_step_s": per_step, "t_short": t1 - t0, "t_long": t2 - t1}),
+          flush=True)
+
+
+def cpu_baseline(args):
+    import resource
+    if not os.path.exists(os.path.join(HERE, "oracle", "_ref", "libtbpm_ref.so")):
+        log("cpu_baseline: oracle/_ref/libtbpm_ref.so missing, skipped")
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+
+    def unlimited_stack():
+        # the reference keeps n-sized automatic arrays on the stack
+        resource.setrlimit(resource.RLIMIT_STACK,
+                           (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+    cmd = [sys.executable, os.path.join(HERE, "bench.py"), "--cpu-child", "--L", str(args.L),
+           "--cpu-steps", str(args.cpu_steps)]
+    log(f"cpu_baseline: reference tbpm_dos, {threads} threads, {args.cpu_steps} steps")
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900,
+                       preexec_fn=unlimited_stack)
+    if r.returncode != 0:
+        log("cpu_baseline failed:", r.stderr[-2000:])
+        return None
+    # the Fortran prints progress lines on the same stdout
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith('{"per_step_s"')]
+    if not lines:
+        log("cpu_baseline: no result line:", r.stdout[-1000:])
+        return None
+    res = json.loads(lines[-1])
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": 1.0 / res["per_step_s"], "unit": "step·vectors/s", "cores": threads,
+            "kind": "reference",
+            "sample": (f"reference Fortran tbpm_dos (oracle/_ref, amdflang -O3 -fopenmp) on "
+                       f"graphene {args.L}x{args.L}, FP64, 1 state: time of T={1 + args.cpu_steps}"
+                       f" minus T=1 over {args.cpu_steps} steps; host {cpu_model}"),
+            "per_step_s": res["per_step_s"]}
+
+
+# ---------------------------------------------------------------------------
+# GPU legs
+# ---------------------------------------------------------------------------
+def term_bytes(n, nnz, K, s):
+    """Algorithmic bytes of one Chebyshev term: H (values + int32 columns +
+    row pointers) and five vector streams per state."""
+    return nnz * (s + 4) + 4 * (n + 1) + K * 5 * n * s
+
+
+def gpu_run(H, bes, psi, warmup, steps):
+    from tbplas_amd import tbpm
+    run = tbpm.DosRun(H, bes, psi, warmup + steps)
+    if warmup:
+        run.advance(warmup)
+    ms = run.advance(steps)
+    corr = run.result(chk_step=0)
+    run.close()
+    return ms, corr
+
+
+def main():
+    args = parse()
+    if args.cpu_child:
+        cpu_child(args)
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py drives one GPU; --gpus {args.gpus} is not supported by this build "
+              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        return 2
+    cpu = None if (args.no_cpu or args.only_fp32) else cpu_baseline(args)
+
+    import numpy as np
+    import tbplas_amd as tb
+    from tbplas_amd import tbpm, _lib
+
+    t0 = time.perf_counter()
+    s = tb.make_graphene_sample(args.L)
+    s.rescale_ham(9.0)
+    indptr, indices, data = s.build_ham_csr_device()
+    n, nnz = s.num_orb, int(indices.numel())
+    ts = 2 * math.pi / s.energy_range
+    bes = tbpm.bessel_series(ts * s.rescale, 1e-15)
+    K = args.vectors
+    psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
+    log(f"setup {time.perf_counter() - t0:.1f}s: n={n} nnz={nnz} n_bes={len(bes)}")
+    h = _lib.default_handle(0)
+    h.set_graph(not args.no_graph)
+
+    H32 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp32", handle=h)
+    ms32, c32 = gpu_run(H32, bes, psi, args.warmup, args.steps)
+    H32.close()
+    t_step = ms32 / 1e3 / args.steps
+    value = K / t_step
+    nterm = len(bes) - 1
+    bt = term_bytes(n, nnz, K, 8)
+    achieved = bt / (t_step / nterm) / 1e9
+    log(f"fp32 K={K}: {ms32:.1f} ms for {args.steps} steps -> {value:.1f} step·vectors/s, "
+        f"{achieved:.0f} GB/s algorithmic")
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "step·vectors/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded random-phase states; pristine graphene lattice)",
+        "config": {"workload": (f"graphene {args.L}x{args.L} PBC DOS correlation, {K} states "
+                                f"together, {args.warmup + args.steps} Chebyshev steps, "
+                                "complex64 storage/FP64 correlation"),
+                   "n_orbitals": n, "nnz": nnz, "states": K, "bessel_terms": len(bes),
+                   "terms_per_step": nterm, "hipgraph": not args.no_graph},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_term": bt,
+                     "term_avg_us": round(t_step / nterm * 1e6, 2)},
+    }
+    traffic_file = os.path.join(HERE, "profiles", "traffic_term_fp32_k8.json")
+    if os.path.exists(traffic_file) and args.L == 2048 and K == 8:
+        with open(traffic_file) as f:
+            out["roofline"]["traffic"] = json.load(f).get("bytes_per_term")
+
+    if not args.only_fp32:
+        steps64 = args.fp64_steps or args.steps
+        H64 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp64", handle=h)
+        ms64, c64 = gpu_run(H64, bes, psi[:1], args.warmup, steps64)
+        H64.close()
+        t64 = ms64 / 1e3 / steps64
+        b64 = term_bytes(n, nnz, 1, 16)
+        tmax = min(args.warmup + args.steps, args.warmup + steps64)
+        err = float(np.max(np.abs(c32[0, :tmax + 1] - c64[0, :tmax + 1])))
+        out["fp64_k1"] = {"value": round(1.0 / t64, 3), "unit": "step·vectors/s",
+                          "ms_per_step": round(t64 * 1e3, 4),
+                          "roofline_frac": round(b64 / (t64 / nterm) / 1e9 / HBM_PEAK_GBS, 4)}
+        out["speedup_vs_fp64_k1"] = round(value * t64, 3)
+        out["ct_err_fp32_vs_fp64"] = err
+        out["ct_err_steps"] = tmax
+        log(f"fp64 K=1: {1.0 / t64:.1f} step/s; fp32 vs fp64 max |dC| = {err:.3e}")
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

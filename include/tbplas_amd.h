@@ -174,6 +174,23 @@ int tbg_corr_dos_batch(tbg_handle *h, int n_inst, const tbg_ham *const *hams,
                        double chk_thr, double *const *corr, int *fail_inst,
                        int *fail_step);
 
+/* ---- DOS run in pieces ----------------------------------------------------
+ * tbg_corr_dos / tbg_corr_dos_batch split into set-up, propagation and
+ * read-out, so a caller can time the propagation alone or stop early:
+ * begin packs the states and forms C(0); advance runs nsteps more steps
+ * (at most T in total) and reports their device time in milliseconds (HIP
+ * events on the handle's stream); end writes corr for the steps run so far
+ * (later steps stay zero) and applies the norm check when chk_step was
+ * reached.  The run must be destroyed with tbg_run_destroy. */
+typedef struct tbg_run tbg_run;
+int tbg_dos_begin(tbg_handle *h, int n_inst, const tbg_ham *const *hams,
+                  const double *bes, int n_bes, const double *const *psi0,
+                  int K, int T, tbg_run **out);
+int tbg_run_advance(tbg_run *r, int nsteps, double *elapsed_ms);
+int tbg_dos_end(tbg_run *r, int chk_step, double chk_thr,
+                double *const *corr, int *fail_inst, int *fail_step);
+int tbg_run_destroy(tbg_run *r);
+
 #ifdef __cplusplus
 }
 #endif

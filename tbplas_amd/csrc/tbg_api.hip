@@ -51,84 +51,137 @@ static bool norm_broken(double n, double ref, double thr) {
     return std::isnan(d) || !std::isfinite(d) || d > thr;
 }
 
+}  // namespace tbg
+
+// A DOS correlation run in progress: set up once, advanced in pieces, read
+// out at the end (tbg_dos_begin / tbg_run_advance / tbg_dos_end).
+struct tbg_run {
+    virtual ~tbg_run() = default;
+    virtual void advance(int nsteps) = 0;
+    virtual void end(int chk_step, double chk_thr, double* const* corr, int* fail_inst,
+                     int* fail_step) = 0;
+    tbg_handle* h = nullptr;
+    int T = 0, done = 0;
+};
+
+namespace tbg {
+
 template <typename R>
-static int corr_dos_impl(tbg_handle* h, int n_inst, const tbg_ham* const* hams, const double* bes,
-                         int nb, const double* const* psi0, int K, int T, int chk_step,
-                         double chk_thr, double* const* corr, int* fail_inst, int* fail_step) {
-    // Concatenate instances (block diagonal) when more than one.
-    std::vector<int64_t> inst_slices(1, 0);
-    for (int i = 0; i < n_inst; ++i) inst_slices.push_back(inst_slices.back() + hams[i]->nslices);
+struct DosRun : tbg_run {
+    int n_inst = 0, K = 0, nv = 0;
+    int64_t stride = 0;
+    std::vector<int64_t> inst_slices;
     Concat cat;
-    MatView mv;
-    if (n_inst == 1) {
-        mv = view_of(hams[0]);
-    } else {
-        concat_hams(h, n_inst, hams, cat);
-        mv = cat.view();
-    }
     Engine<R> e;
-    e.init(h, mv, inst_slices, K, chunk_target(h, mv.nslices));
-    const int Kp = e.Kp;
-    const int nv = 3 * Kp;
-    DevBuf psi((size_t)e.vbytes);
-    launch_fill_zero(psi.p, e.vbytes, h->stream);
-    for (int i = 0; i < n_inst; ++i) {
-        char* base = reinterpret_cast<char*>(psi.p) +
-                     (size_t)inst_slices[i] * 64 * Kp * sizeof(typename CplxT<R>::T);
-        launch_pack<R>(psi0[i], K, hams[i]->n, base, Kp, 0, 0, h->stream);
+    DevBuf psi, cbuf, counter;
+    StepPoly sp;
+
+    DosRun(tbg_handle* hh, int ni, const tbg_ham* const* hams, const double* bes, int nb,
+           const double* const* psi0, int k, int t) {
+        h = hh;
+        n_inst = ni;
+        K = k;
+        T = t;
+        inst_slices.assign(1, 0);
+        for (int i = 0; i < ni; ++i) inst_slices.push_back(inst_slices.back() + hams[i]->nslices);
+        MatView mv;
+        if (ni == 1) {
+            mv = view_of(hams[0]);
+        } else {
+            concat_hams(h, ni, hams, cat);  // block-diagonal batch
+            mv = cat.view();
+        }
+        e.init(h, mv, inst_slices, K, chunk_target(h, mv.nslices));
+        const int Kp = e.Kp;
+        nv = 3 * Kp;
+        psi.alloc(e.vbytes);
+        launch_fill_zero(psi.p, e.vbytes, h->stream);
+        for (int i = 0; i < ni; ++i) {
+            char* base = reinterpret_cast<char*>(psi.p) +
+                         (size_t)inst_slices[i] * 64 * Kp * sizeof(typename CplxT<R>::T);
+            launch_pack<R>(psi0[i], K, hams[i]->n, base, Kp, 0, 0, h->stream);
+        }
+        TBG_HIP(hipMemcpyAsync(e.X.p, psi.p, e.vbytes, hipMemcpyDeviceToDevice, h->stream));
+        stride = (int64_t)(T + 1) * nv;
+        cbuf.alloc((size_t)ni * stride * sizeof(double));
+        TBG_HIP(hipMemsetAsync(cbuf.p, 0, cbuf.bytes, h->stream));
+        counter.alloc(sizeof(int));
+        TBG_HIP(hipMemsetAsync(counter.p, 0, sizeof(int), h->stream));
+        // t = 0: <psi0|psi0> (tbpm.f90:58-62)
+        launch_dotnorm<R>(e.plan, psi.p, psi.p, Kp, Kp, e.part.template as<double>(), h->stream);
+        reduce();
+        sp = time_poly(bes, nb);
+        TBG_HIP(hipStreamSynchronize(h->stream));
     }
-    TBG_HIP(hipMemcpyAsync(e.X.p, psi.p, e.vbytes, hipMemcpyDeviceToDevice, h->stream));
-    const int64_t stride = (int64_t)(T + 1) * nv;
-    DevBuf cbuf((size_t)n_inst * stride * sizeof(double));
-    DevBuf counter(sizeof(int));
-    TBG_HIP(hipMemsetAsync(counter.p, 0, sizeof(int), h->stream));
-    launch_dotnorm<R>(e.plan, psi.p, psi.p, Kp, Kp, e.part.template as<double>(), h->stream);
-    launch_reduce(e.part.template as<double>(), nv, e.plan.inst_chunk_dev.template as<int>(), n_inst,
-                  cbuf.as<double>(), stride, counter.as<int>(), 0, h->stream);
-    const StepPoly sp = time_poly(bes, nb);
-    auto after = [&]() {
+
+    void reduce() {
         launch_reduce(e.part.template as<double>(), nv, e.plan.inst_chunk_dev.template as<int>(),
                       n_inst, cbuf.as<double>(), stride, counter.as<int>(), 0, h->stream);
-    };
-    e.run(T, sp, 0, 0, F_DOT | F_NORM, psi.p, counter.as<int>(), after);
-    std::vector<double> hb((size_t)n_inst * stride);
-    TBG_HIP(hipMemcpyAsync(hb.data(), cbuf.p, hb.size() * sizeof(double), hipMemcpyDeviceToHost,
-                           h->stream));
-    TBG_HIP(hipStreamSynchronize(h->stream));
-    int rc = TBG_OK;
-    for (int i = 0; i < n_inst; ++i) {
-        std::vector<double> out((size_t)K * (T + 1) * 2);
-        const double* b = hb.data() + (size_t)i * stride;
-        for (int k = 0; k < K; ++k)
-            for (int t = 0; t <= T; ++t) {
-                out[((size_t)k * (T + 1) + t) * 2 + 0] = b[(size_t)t * nv + 3 * k + 0];
-                out[((size_t)k * (T + 1) + t) * 2 + 1] = b[(size_t)t * nv + 3 * k + 1];
-            }
-        TBG_HIP(hipMemcpyAsync(corr[i], out.data(), out.size() * sizeof(double),
-                               hipMemcpyHostToDevice, h->stream));
+    }
+
+    void advance(int nsteps) override {
+        TBG_REQUIRE(nsteps >= 0 && done + nsteps <= T, "advancing past the last time step");
+        e.run(nsteps, sp, 0, 0, F_DOT | F_NORM, psi.p, counter.as<int>(), [&]() { reduce(); });
+        done += nsteps;
+    }
+
+    void end(int chk_step, double chk_thr, double* const* corr, int* fail_inst,
+             int* fail_step) override {
+        std::vector<double> hb((size_t)n_inst * stride);
+        TBG_HIP(hipMemcpyAsync(hb.data(), cbuf.p, hb.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, h->stream));
         TBG_HIP(hipStreamSynchronize(h->stream));
-        // tbpm.f90:73-78: norm_ref after step 1, checked on the state of
-        // step chk_step-1 at loop index chk_step (2..T).
-        if (rc == TBG_OK && chk_step >= 2 && chk_step <= T) {
-            for (int k = 0; k < K; ++k) {
-                const double ref = b[(size_t)1 * nv + 3 * k + 2];
-                const double nn = b[(size_t)(chk_step - 1) * nv + 3 * k + 2];
-                if (norm_broken(nn, ref, chk_thr)) {
-                    rc = TBG_ENORM;
-                    if (fail_inst) *fail_inst = i;
-                    if (fail_step) *fail_step = chk_step;
-                    set_error("wave function norm broken at step " + std::to_string(chk_step) +
+        int rc = TBG_OK;
+        std::string msg;
+        for (int i = 0; i < n_inst; ++i) {
+            std::vector<double> out((size_t)K * (T + 1) * 2);
+            const double* b = hb.data() + (size_t)i * stride;
+            for (int k = 0; k < K; ++k)
+                for (int t = 0; t <= T; ++t) {
+                    out[((size_t)k * (T + 1) + t) * 2 + 0] = b[(size_t)t * nv + 3 * k + 0];
+                    out[((size_t)k * (T + 1) + t) * 2 + 1] = b[(size_t)t * nv + 3 * k + 1];
+                }
+            TBG_HIP(hipMemcpyAsync(corr[i], out.data(), out.size() * sizeof(double),
+                                   hipMemcpyHostToDevice, h->stream));
+            TBG_HIP(hipStreamSynchronize(h->stream));
+            // tbpm.f90:73-78: norm_ref after step 1, checked on the state of
+            // step chk_step-1 at loop index chk_step (2..T).
+            if (rc == TBG_OK && chk_step >= 2 && chk_step <= done) {
+                for (int k = 0; k < K; ++k) {
+                    const double ref = b[(size_t)1 * nv + 3 * k + 2];
+                    const double nn = b[(size_t)(chk_step - 1) * nv + 3 * k + 2];
+                    if (norm_broken(nn, ref, chk_thr)) {
+                        rc = TBG_ENORM;
+                        if (fail_inst) *fail_inst = i;
+                        if (fail_step) *fail_step = chk_step;
+                        msg = "wave function norm broken at step " + std::to_string(chk_step) +
                               " (state " + std::to_string(k) + ", instance " +
                               std::to_string(i) + "): |" + std::to_string(nn) + " - " +
                               std::to_string(ref) + "| > threshold; check the Hermiticity of "
-                              "the Hamiltonian or increase the rescale factor");
-                    break;
+                              "the Hamiltonian or increase the rescale factor";
+                        break;
+                    }
                 }
             }
         }
+        if (rc != TBG_OK) throw Error{rc, msg};
     }
-    if (rc != TBG_OK) throw Error{rc, g_err};
-    return rc;
+};
+
+static tbg_run* dos_begin(tbg_handle* h, int n_inst, const tbg_ham* const* hams,
+                          const double* bes, int nb, const double* const* psi0, int K, int T) {
+    if (hams[0]->precision == TBG_FP32)
+        return new DosRun<float>(h, n_inst, hams, bes, nb, psi0, K, T);
+    return new DosRun<double>(h, n_inst, hams, bes, nb, psi0, K, T);
+}
+
+static void corr_dos_impl(tbg_handle* h, int n_inst, const tbg_ham* const* hams,
+                          const double* bes, int nb, const double* const* psi0, int K, int T,
+                          int chk_step, double chk_thr, double* const* corr, int* fail_inst,
+                          int* fail_step) {
+    std::unique_ptr<tbg_run> r(dos_begin(h, n_inst, hams, bes, nb, psi0, K, T));
+    r->advance(T);
+    r->end(chk_step, chk_thr, corr, fail_inst, fail_step);
 }
 
 template <typename R>
@@ -326,22 +379,28 @@ int tbg_ham_info(const tbg_ham* H, int64_t* n, int64_t* nnz, int64_t* padded, in
     });
 }
 
+static void check_batch(tbg_handle* h, int n_inst, const tbg_ham* const* hams,
+                        const double* bes, int n_bes, const double* const* psi0, int K, int T) {
+    check_handle(h);
+    TBG_REQUIRE(n_inst >= 1 && hams && psi0, "null argument");
+    TBG_REQUIRE(bes && n_bes >= 3, "need >= 3 Bessel terms");
+    for (int i = 0; i < n_inst; ++i) {
+        check_psi_args(hams[i], psi0[i], K, T);
+        TBG_REQUIRE(hams[i]->precision == hams[0]->precision,
+                    "all instances must share one precision");
+    }
+}
+
 int tbg_corr_dos(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
                  const double* psi0, int K, int T, int chk_step, double chk_thr, double* corr,
                  int* fail_step) {
     return guard([&] {
-        check_handle(h);
-        check_psi_args(H, psi0, K, T);
-        TBG_REQUIRE(bes && n_bes >= 3 && corr, "need >= 3 Bessel terms and an output");
         const tbg_ham* hs[1] = {H};
         const double* ps[1] = {psi0};
         double* cs[1] = {corr};
-        if (H->precision == TBG_FP32)
-            corr_dos_impl<float>(h, 1, hs, bes, n_bes, ps, K, T, chk_step, chk_thr, cs, nullptr,
-                                 fail_step);
-        else
-            corr_dos_impl<double>(h, 1, hs, bes, n_bes, ps, K, T, chk_step, chk_thr, cs, nullptr,
-                                  fail_step);
+        check_batch(h, 1, hs, bes, n_bes, ps, K, T);
+        TBG_REQUIRE(corr != nullptr, "null output");
+        corr_dos_impl(h, 1, hs, bes, n_bes, ps, K, T, chk_step, chk_thr, cs, nullptr, fail_step);
     });
 }
 
@@ -349,21 +408,58 @@ int tbg_corr_dos_batch(tbg_handle* h, int n_inst, const tbg_ham* const* hams, co
                        int n_bes, const double* const* psi0, int K, int T, int chk_step,
                        double chk_thr, double* const* corr, int* fail_inst, int* fail_step) {
     return guard([&] {
-        check_handle(h);
-        TBG_REQUIRE(n_inst >= 1 && hams && psi0 && corr, "null argument");
-        TBG_REQUIRE(bes && n_bes >= 3, "need >= 3 Bessel terms");
-        for (int i = 0; i < n_inst; ++i) {
-            check_psi_args(hams[i], psi0[i], K, T);
-            TBG_REQUIRE(corr[i] != nullptr, "null output");
-            TBG_REQUIRE(hams[i]->precision == hams[0]->precision,
-                        "all instances must share one precision");
-        }
-        if (hams[0]->precision == TBG_FP32)
-            corr_dos_impl<float>(h, n_inst, hams, bes, n_bes, psi0, K, T, chk_step, chk_thr, corr,
-                                 fail_inst, fail_step);
-        else
-            corr_dos_impl<double>(h, n_inst, hams, bes, n_bes, psi0, K, T, chk_step, chk_thr,
-                                  corr, fail_inst, fail_step);
+        check_batch(h, n_inst, hams, bes, n_bes, psi0, K, T);
+        TBG_REQUIRE(corr != nullptr, "null output");
+        for (int i = 0; i < n_inst; ++i) TBG_REQUIRE(corr[i] != nullptr, "null output");
+        corr_dos_impl(h, n_inst, hams, bes, n_bes, psi0, K, T, chk_step, chk_thr, corr,
+                      fail_inst, fail_step);
+    });
+}
+
+int tbg_dos_begin(tbg_handle* h, int n_inst, const tbg_ham* const* hams, const double* bes,
+                  int n_bes, const double* const* psi0, int K, int T, tbg_run** out) {
+    return guard([&] {
+        check_batch(h, n_inst, hams, bes, n_bes, psi0, K, T);
+        TBG_REQUIRE(out != nullptr, "null output pointer");
+        *out = dos_begin(h, n_inst, hams, bes, n_bes, psi0, K, T);
+    });
+}
+
+int tbg_run_advance(tbg_run* r, int nsteps, double* elapsed_ms) {
+    return guard([&] {
+        TBG_REQUIRE(r != nullptr, "null run");
+        check_handle(r->h);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        TBG_HIP(hipEventCreate(&e0));
+        TBG_HIP(hipEventCreate(&e1));
+        struct Ev {
+            hipEvent_t a, b;
+            ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+        } ev{e0, e1};
+        TBG_HIP(hipEventRecord(e0, r->h->stream));
+        r->advance(nsteps);
+        TBG_HIP(hipEventRecord(e1, r->h->stream));
+        TBG_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        TBG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (elapsed_ms) *elapsed_ms = ms;
+    });
+}
+
+int tbg_dos_end(tbg_run* r, int chk_step, double chk_thr, double* const* corr, int* fail_inst,
+                int* fail_step) {
+    return guard([&] {
+        TBG_REQUIRE(r != nullptr && corr != nullptr, "null argument");
+        check_handle(r->h);
+        r->end(chk_step, chk_thr, corr, fail_inst, fail_step);
+    });
+}
+
+int tbg_run_destroy(tbg_run* r) {
+    return guard([&] {
+        if (!r) return;
+        (void)hipSetDevice(r->h->device);
+        delete r;
     });
 }
 

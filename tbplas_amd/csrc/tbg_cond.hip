@@ -510,6 +510,7 @@ void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
     }
     qe.release();
     // Phase 2: J_x psi0, J_y psi0 propagated forward (tbpm.f90:655-697).
+    e.drop_graph();
     launch_fill_zero(e.X.p, e.vbytes, st);
     launch_spmv<R>(H, H->jx.p, psi.p, Kp, 0, e.X.p, Kp, 0, K, st);
     launch_spmv<R>(H, H->jy.p, psi.p, Kp, 0, e.X.p, Kp, K, K, st);

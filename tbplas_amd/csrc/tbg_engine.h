@@ -169,45 +169,75 @@ struct Engine {
             launch_term<R>(fl, a, plan.nchunks, h->stream);
         }
         std::swap(X, Y);
+        parity ^= 1;
     }
 
-    // Steps with an optional per-step callback (after each step), replayed
-    // from captured graphs of an even number of steps when enabled.
+    // Number of X/Y swaps so far, mod 2: a captured graph is only valid at
+    // the parity it was captured at.
+    int parity = 0;
+
+    // Captured graph of `graph_steps` (even) consecutive steps; replaying it
+    // leaves the X/Y roles unchanged.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_steps = 0;
+    int graph_parity = 0;
+
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_steps = 0;
+    }
+    ~Engine() { drop_graph(); }
+
+    template <typename F>
+    void capture(int per, const StepPoly& sp, uint64_t bwd, uint64_t cls, int epi,
+                 const void* psi0, int* counter, F&& after) {
+        drop_graph();
+        graph_parity = parity;
+        TBG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int s = 0; s < per; ++s) {
+                step(sp, bwd, cls, epi, psi0, counter);
+                after();
+            }
+        } catch (...) {
+            hipGraph_t dummy = nullptr;
+            (void)hipStreamEndCapture(h->stream, &dummy);
+            if (dummy) (void)hipGraphDestroy(dummy);
+            throw;
+        }
+        TBG_HIP(hipStreamEndCapture(h->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            drop_graph();
+            TBG_HIP(e);
+        }
+        graph_steps = per;
+    }
+
+    // Enqueue nsteps steps, each followed by `after` (reductions etc.),
+    // replaying the captured graph where possible.
     template <typename F>
     void run(int nsteps, const StepPoly& sp, uint64_t bwd, uint64_t cls, int epi,
              const void* psi0, int* counter, F&& after) {
         int done = 0;
-        const int per = std::min(16, nsteps & ~1);
-        if (h->use_graph && per >= 2 && nsteps / per >= 1) {
-            hipGraph_t g = nullptr;
-            hipGraphExec_t ge = nullptr;
-            TBG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            try {
-                for (int s = 0; s < per; ++s) {
-                    step(sp, bwd, cls, epi, psi0, counter);
-                    after();
-                }
-            } catch (...) {
-                hipGraph_t dummy = nullptr;
-                (void)hipStreamEndCapture(h->stream, &dummy);
-                if (dummy) (void)hipGraphDestroy(dummy);
-                throw;
+        if (h->use_graph) {
+            const int per = std::min(16, nsteps & ~1);
+            if (graph_steps == 0 && per >= 2) capture(per, sp, bwd, cls, epi, psi0, counter, after);
+            if (graph_steps > 0 && parity != graph_parity && nsteps > 0) {
+                // one launched step restores the buffer roles of the graph
+                step(sp, bwd, cls, epi, psi0, counter);
+                after();
+                ++done;
             }
-            TBG_HIP(hipStreamEndCapture(h->stream, &g));
-            hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
-                (void)hipGraphDestroy(g);
-                TBG_HIP(e);
+            if (graph_steps > 0) {
+                const int reps = (nsteps - done) / graph_steps;
+                for (int r = 0; r < reps; ++r) TBG_HIP(hipGraphLaunch(graph_exec, h->stream));
+                done += reps * graph_steps;
             }
-            const int reps = nsteps / per;
-            for (int r = 0; r < reps; ++r) {
-                e = hipGraphLaunch(ge, h->stream);
-                if (e != hipSuccess) break;
-            }
-            (void)hipGraphExecDestroy(ge);
-            (void)hipGraphDestroy(g);
-            TBG_HIP(e);
-            done = reps * per;
         }
         for (; done < nsteps; ++done) {
             step(sp, bwd, cls, epi, psi0, counter);

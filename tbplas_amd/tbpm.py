@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["Hamiltonian", "bessel_series", "fermi_cheb_coef", "corr_dos",
+__all__ = ["Hamiltonian", "DosRun", "bessel_series", "fermi_cheb_coef", "corr_dos",
            "corr_dos_batch", "corr_ac", "corr_dc", "cheb_step", "cheb_poly",
            "precision_code"]
 
@@ -186,6 +186,71 @@ def corr_dos_batch(hams, bes, psi0_list, n_timestep, chk_step=128, chk_thr=1e-9)
                                        ctypes.byref(fi), ctypes.byref(fs))
     _lib.check(rc, fs)
     return [o.cpu().numpy() for o in outs]
+
+
+class DosRun:
+    """A DOS correlation run advanced in pieces (tbg_dos_begin/advance/end).
+
+    ``advance`` returns the device time of the steps in milliseconds,
+    measured with HIP events on the library's stream.
+    """
+
+    def __init__(self, hams, bes, psi0_list, n_timestep):
+        torch = _lib.torch_cuda()
+        if isinstance(hams, Hamiltonian):
+            hams, psi0_list = [hams], [psi0_list]
+        self.hams = list(hams)
+        self.handle = self.hams[0].handle
+        dv = self.handle.device
+        self.bes = np.ascontiguousarray(bes, dtype=np.float64)
+        self.ps = []
+        for H, psi in zip(self.hams, psi0_list):
+            p, (k, n) = _states_dev(psi, dv)
+            if n != H.n:
+                raise ValueError("state length does not match the Hamiltonian")
+            self.ps.append(p)
+        self.K = self.ps[0].shape[0]
+        self.T = int(n_timestep)
+        self.done = 0
+        ni = len(self.hams)
+        harr = (ctypes.c_void_p * ni)(*[H.ptr.value for H in self.hams])
+        parr = (ctypes.c_void_p * ni)(*[t.data_ptr() for t in self.ps])
+        out = ctypes.c_void_p()
+        _lib.check(_lib.lib().tbg_dos_begin(self.handle.ptr, ni, harr, self.bes.ctypes.data,
+                                            len(self.bes), parr, self.K, self.T,
+                                            ctypes.byref(out)))
+        self._r = out
+        self._torch = torch
+
+    def advance(self, nsteps):
+        ms = ctypes.c_double(0.0)
+        _lib.check(_lib.lib().tbg_run_advance(self._r, int(nsteps), ctypes.byref(ms)))
+        self.done += int(nsteps)
+        return ms.value
+
+    def result(self, chk_step=128, chk_thr=1e-9):
+        torch = self._torch
+        dv = self.handle.device
+        outs = [torch.zeros((self.K, self.T + 1), dtype=torch.complex128, device=f"cuda:{dv}")
+                for _ in self.hams]
+        oarr = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
+        fi, fs = ctypes.c_int(-1), ctypes.c_int(0)
+        rc = _lib.lib().tbg_dos_end(self._r, int(chk_step), float(chk_thr), oarr,
+                                    ctypes.byref(fi), ctypes.byref(fs))
+        _lib.check(rc, fs)
+        res = [o.cpu().numpy() for o in outs]
+        return res if len(res) > 1 else res[0]
+
+    def close(self):
+        if getattr(self, "_r", None):
+            _lib.lib().tbg_run_destroy(self._r)
+            self._r = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def corr_ac(H, bes, coef_F, coef_omF, psi0, n_timestep, chk_step=128, chk_thr=1e-9):

@@ -1,0 +1,311 @@
+"""GPU TBPM correlation kernels against the oracle.
+
+Tolerances: FP64 path 1e-12 on small samples and 1e-10 on the 131,072-orbital
+1024-step case; FP32 path (complex64 storage, FP64 correlation sums)
+2e-5 max |dC| over 64 steps and 2e-4 over 1024 steps.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import tbplas_amd as tb
+from tbplas_amd import tbpm
+from conftest import golden, graphene_sample, tau_of
+
+pytestmark = pytest.mark.gpu
+
+
+def _csr(s):
+    return oracle.build_ham_csr(s.orb_eng, s.hop_i, s.hop_j, s.hop_v)
+
+
+def _states(n, K, seed=1337):
+    return tb.states.random_states(n, tb.states.sample_seeds(seed, K))
+
+
+def test_dos_reference_fixture_fp64(gpu):
+    """Identical initial state to the reference Fortran run (its own RNG)."""
+    g = golden("ref_dos.npz")
+    n = len(g["indptr"]) - 1
+    H = tbpm.Hamiltonian(n, g["indptr"], g["indices"], g["data"], precision="fp64")
+    c = tbpm.corr_dos(H, g["bes"], g["psi0"][None], int(g["T"]))[0]
+    assert np.max(np.abs(c - g["corr"])) < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+@pytest.mark.parametrize("K", [1, 2, 3, 8])
+def test_dos_vs_oracle(gpu, prec, tol, K):
+    s = graphene_sample(20)
+    ip, ix, dt = _csr(s)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, K)
+    T = 64
+    ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, T)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec)
+    c = tbpm.corr_dos(H, bes, psi, T)
+    assert c.shape == (K, T + 1)
+    assert np.max(np.abs(c - ref)) < tol
+
+
+@pytest.mark.parametrize("nb_prec", [1e-14, 1e-15, 1e-6])
+def test_dos_disordered_complex_hops(gpu, nb_prec):
+    """Vacancies, on-site disorder (4-entry rows), Peierls phases; different
+    series lengths exercise every accumulation schedule of the fused kernel."""
+    s = tb.make_graphene_sample(18, vacancies=tb.random_vacancies(2 * 18 * 18, 0.02, 3))
+    tb.add_anderson_disorder(s, 0.8, 4)
+    s.set_magnetic_field(40.0, 0)
+    s.rescale_ham(9.0)
+    ip, ix, dt = _csr(s)
+    bes = tbpm.bessel_series(tau_of(s), nb_prec)
+    psi = _states(s.num_orb, 2, seed=5)
+    T = 40
+    ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, T)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+    c = tbpm.corr_dos(H, bes, psi, T)
+    assert np.max(np.abs(c - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("nterms", [3, 4, 5, 6, 7, 8])
+def test_short_series_schedules(gpu, nterms):
+    s = graphene_sample(8)
+    ip, ix, dt = _csr(s)
+    bes = np.asarray(tbpm.bessel_series(tau_of(s), 1e-15)[:nterms])
+    psi = _states(s.num_orb, 1)
+    ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 6, chk_step=0)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+    c = tbpm.corr_dos(H, bes, psi, 6, chk_step=0)
+    assert np.max(np.abs(c - ref)) < 1e-13
+
+
+def test_multivector_equals_single_runs(gpu):
+    s = graphene_sample(16)
+    ip, ix, dt = _csr(s)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, 8)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+    many = tbpm.corr_dos(H, bes, psi, 32)
+    for k in range(8):
+        one = tbpm.corr_dos(H, bes, psi[k:k + 1], 32)[0]
+        assert np.max(np.abs(one - many[k])) < 1e-14
+
+
+def test_graph_replay_equals_launches(gpu):
+    s = graphene_sample(16)
+    ip, ix, dt = _csr(s)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, 4)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp32")
+    H.handle.set_graph(True)
+    a = tbpm.corr_dos(H, bes, psi, 70)      # 4 replays of 16 steps + 6 launched
+    b = tbpm.corr_dos(H, bes, psi, 70)
+    H.handle.set_graph(False)
+    c = tbpm.corr_dos(H, bes, psi, 70)
+    H.handle.set_graph(True)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(a, c)
+
+
+def test_dos_run_pieces_equal_one_call(gpu):
+    s = graphene_sample(12)
+    ip, ix, dt = _csr(s)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, 2)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+    full = tbpm.corr_dos(H, bes, psi, 50)
+    run = tbpm.DosRun(H, bes, psi, 50)
+    ms = run.advance(17) + run.advance(33)
+    assert ms > 0
+    np.testing.assert_array_equal(run.result(), full)
+    run.close()
+
+
+def test_cheb_step_and_poly(gpu):
+    s = tb.make_graphene_sample(10)
+    tb.add_anderson_disorder(s, 0.5, 9)
+    s.set_magnetic_field(15.0, 1)
+    s.rescale_ham(9.0)
+    ip, ix, dt = _csr(s)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, 3)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+    for fwd in (True, False):
+        g = tbpm.cheb_step(H, bes, psi, fwd)
+        for k in range(3):
+            r = oracle.orc_step(ip, ix, dt, bes, psi[k], fwd)
+            assert np.max(np.abs(g[k] - r)) < 1e-14
+    coef = golden("fermi.npz")["coef_F"][:400]
+    g = tbpm.cheb_poly(H, coef, psi)
+    # dense evaluation of sum_k c_k T_k(H) psi
+    n = s.num_orb
+    Hd = np.zeros((n, n), complex)
+    for r in range(n):
+        Hd[r, ix[ip[r]:ip[r + 1]]] += dt[ip[r]:ip[r + 1]]
+    w, v = np.linalg.eigh(Hd)
+    f = np.polynomial.chebyshev.chebval(w, coef)
+    dense = (v * f) @ (v.conj().T @ psi.T)
+    assert np.max(np.abs(g - dense.T)) < 1e-12
+
+
+def test_cfg1_full_size_fp64_vs_oracle_and_kspace(gpu):
+    """The reference CPU case at full size: 256x256 graphene (131,072
+    orbitals), 1024 steps, Bessel terms to 1e-15."""
+    L = 256
+    s = graphene_sample(L)
+    ip, ix, dt = [t.cpu().numpy() for t in s.build_ham_csr_device()]
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    assert len(bes) == 21
+    psi = _states(s.num_orb, 1)
+    T = 1024
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+    c = tbpm.corr_dos(H, bes, psi, T)[0]
+    ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, T)[0]
+    assert np.max(np.abs(c - ref)) < 1e-10
+    pts = [0, 1, 2, 10, 100, 511, 1024]
+    k = oracle.kspace_graphene_corr(L, L, s.hop_v[0], psi[0], tau_of(s), pts)
+    assert np.max(np.abs(c[pts] - k)) < 1e-10
+    H32 = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp32")
+    c32 = tbpm.corr_dos(H32, bes, psi, T, chk_thr=1e-4)[0]
+    assert np.max(np.abs(c32 - ref)) < 2e-4
+
+
+def test_large_kspace_spot_check(gpu):
+    """2048x2048 (8.4M orbitals, the benchmark sample), 8 states, 64 steps,
+    FP64 and FP32 against exact k-space propagation at sampled steps."""
+    L = 2048
+    s = graphene_sample(L)
+    ip, ix, dt = s.build_ham_csr_device()
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, 8)
+    T = 64
+    pts = [0, 1, 7, 33, 64]
+    ex = np.array([oracle.kspace_graphene_corr(L, L, s.hop_v[0], psi[k], tau_of(s), pts)
+                   for k in range(8)])
+    for prec, tol in (("fp64", 1e-11), ("fp32", 2e-5)):
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec)
+        c = tbpm.corr_dos(H, bes, psi, T, chk_thr=1e-4)
+        H.close()
+        assert np.max(np.abs(c[:, pts] - ex)) < tol, prec
+
+
+def test_norm_check_raises(gpu):
+    """rescale below the spectral radius makes the expansion diverge; the
+    reference STOPs (funcs.f90:175-192), here NormError is raised."""
+    s = graphene_sample(8, rescale=2.0)
+    ip, ix, dt = _csr(s)
+    bes = tbpm.bessel_series(tau_of(s), 1e-14)
+    psi = _states(s.num_orb, 1)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+    from tbplas_amd._lib import NormError
+    with pytest.raises(NormError) as ei:
+        tbpm.corr_dos(H, bes, psi, 40, chk_step=16)
+    assert ei.value.step == 16
+
+
+def test_batch_equals_individual(gpu):
+    rng = np.random.default_rng(0)
+    hams, psis, refs = [], [], []
+    bes = None
+    for i, L in enumerate((8, 13, 5, 16)):
+        n0 = 2 * L * L
+        s = tb.make_graphene_sample(L, vacancies=tb.random_vacancies(n0, 0.03 * (i % 2), i))
+        tb.add_anderson_disorder(s, 0.5 * i, 10 + i)
+        s.rescale_ham(9.0)
+        ip, ix, dt = _csr(s)
+        bes = tbpm.bessel_series(tau_of(s), 1e-15)
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+        psi = _states(s.num_orb, 2, seed=100 + i)
+        hams.append(H)
+        psis.append(psi)
+        refs.append(oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 24))
+    out = tbpm.corr_dos_batch(hams, bes, psis, 24)
+    for o, r in zip(out, refs):
+        assert np.max(np.abs(o - r)) < 1e-12
+
+
+def test_ac_reference_fixture(gpu):
+    g = golden("ref_ac.npz")
+    n = len(g["indptr"]) - 1
+    H = tbpm.Hamiltonian(n, g["indptr"], g["indices"], g["hop"], g["dx"], g["dy"],
+                         cur_scale=float(g["rescale"]), precision="fp64")
+    cF = tbpm.fermi_cheb_coef(2 ** 15, float(g["beta_re"]), float(g["mu_re"]), False, 1e-12)
+    cO = tbpm.fermi_cheb_coef(2 ** 15, float(g["beta_re"]), float(g["mu_re"]), True, 1e-12)
+    c = tbpm.corr_ac(H, g["bes"], cF, cO, g["psi0"], int(g["T"]))
+    S = c.shape[0]
+    avg = np.zeros_like(c[0])
+    for s in range(S):
+        avg = avg + c[s] / S
+    ref = g["corr"]
+    assert np.all(avg[:, 0] == 0)
+    assert np.max(np.abs(avg - ref)) < 1e-11 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_ac_fp32_vs_oracle(gpu):
+    s = tb.make_graphene_sample(10)
+    s.set_magnetic_field(10.0, 0)
+    s.rescale_ham(9.0)
+    ip, ix, h, dx, dy = oracle.build_ham_dxy_loop(s.orb_eng, s.hop_i, s.hop_j, s.hop_v, s.dr)
+    bes = tbpm.bessel_series(tau_of(s, 0.5), 1e-14)
+    beta_re = 1 / (8.617333262e-5 * 300) * 9.0
+    cF = tbpm.fermi_cheb_coef(2 ** 15, beta_re, 0.02, False, 1e-12)
+    cO = tbpm.fermi_cheb_coef(2 ** 15, beta_re, 0.02, True, 1e-12)
+    psi = _states(s.num_orb, 3)
+    T = 12
+    ref = oracle.orc_tbpm_accond(ip, ix, h, dx, dy, 9.0, beta_re, 0.02, bes, psi, T)
+    scale = np.max(np.abs(ref))
+    for prec, tol in (("fp64", 1e-11), ("fp32", 1e-4)):
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, h, dx, dy, cur_scale=9.0, precision=prec)
+        c = tbpm.corr_ac(H, bes, cF, cO, psi, T, chk_thr=1e-4)
+        assert np.max(np.abs(c - ref)) < tol * scale, prec
+
+
+def test_fermi_coef_product_vs_reference_fixture(gpu):
+    g = golden("fermi.npz")
+    a = tbpm.fermi_cheb_coef(int(g["nr_fermi"]), float(g["beta_re"]), 0.0, False, 1e-12)
+    b = tbpm.fermi_cheb_coef(int(g["nr_fermi"]), float(g["beta_re"]), 0.0, True, 1e-12)
+    assert len(a) == len(g["coef_F"]) and len(b) == len(g["coef_omF"])
+    assert np.max(np.abs(a - g["coef_F"])) < 1e-14
+    assert np.max(np.abs(b - g["coef_omF"])) < 1e-14
+    c = tbpm.fermi_cheb_coef(int(g["nr_fermi"]), float(g["beta_re_cold"]),
+                             float(g["mu_re_cold"]), False, 1e-12)
+    assert len(c) == len(g["coef_F_cold"])
+    assert np.max(np.abs(c - g["coef_F_cold"])) < 1e-14
+
+
+def test_dc_reference_fixture(gpu):
+    g = golden("ref_dc.npz")
+    n = len(g["indptr"]) - 1
+    H = tbpm.Hamiltonian(n, g["indptr"], g["indices"], g["hop"], g["dx"], g["dy"],
+                         cur_scale=float(g["rescale"]), precision="fp64")
+    dos, dc = tbpm.corr_dc(H, g["bes"], g["psi0"], int(g["T"]), float(g["t_step"]),
+                           g["energies"], g["en_inds"])
+    S = dos.shape[0]
+    ne = len(g["en_inds"])
+    adc = np.zeros_like(dos[0])
+    acc = np.zeros_like(g["dc"])
+    for s in range(S):
+        adc = adc + dos[s] / S
+        acc[:, :ne, :] = acc[:, :ne, :] + dc[s] / S
+    assert np.max(np.abs(adc - g["dos"])) < 1e-12
+    assert np.max(np.abs(acc - g["dc"])) < 1e-10 * max(1.0, np.max(np.abs(g["dc"])))
+
+
+def test_dc_fp32_vs_oracle(gpu):
+    s = tb.make_graphene_sample(10, vacancies=tb.random_vacancies(200, 0.02, 8))
+    tb.add_anderson_disorder(s, 0.6, 8)
+    s.rescale_ham(9.0)
+    ip, ix, h, dx, dy = oracle.build_ham_dxy_loop(s.orb_eng, s.hop_i, s.hop_j, s.hop_v, s.dr)
+    tau = tau_of(s)
+    bes = tbpm.bessel_series(tau, 1e-14)
+    T = 12
+    en = np.array([0.5 * i * 18.0 / T - 9.0 for i in range(2 * T)])
+    inds = np.array([5, 11, 12, 13, 20])
+    psi = _states(s.num_orb, 4)
+    t_step = 2 * math.pi / 18.0
+    rd, rc = oracle.orc_tbpm_dccond(ip, ix, h, dx, dy, 9.0, bes, psi, T, t_step, en, inds)
+    for prec, tol in (("fp64", 1e-10), ("fp32", 2e-4)):
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, h, dx, dy, cur_scale=9.0, precision=prec)
+        d, c = tbpm.corr_dc(H, bes, psi, T, t_step, en, inds, chk_thr=1e-4)
+        assert np.max(np.abs(d - rd)) < tol, prec
+        assert np.max(np.abs(c - rc)) < tol * max(1.0, np.max(np.abs(rc))), prec

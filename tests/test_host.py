@@ -1,0 +1,121 @@
+"""Host-side logic without a GPU: states, Config, Analyzer, bench contract."""
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import tbplas_amd as tb
+from conftest import ROOT
+
+
+def test_states_deterministic_normalised():
+    a = tb.states.random_states(1000, [1337, 2674])
+    b = tb.states.random_states(1000, [1337, 2674])
+    np.testing.assert_array_equal(a, b)
+    assert np.allclose(np.sum(np.abs(a) ** 2, axis=1), 1.0, atol=1e-14)
+    assert np.allclose(np.abs(a[0]), 1 / math.sqrt(1000), atol=1e-15)
+    assert not np.allclose(a[0], a[1])
+    u = tb.states.uniform01(tb.states._key(1), 200000)
+    assert 0.0 <= u.min() and u.max() < 1.0 and abs(u.mean() - 0.5) < 0.01
+    e = tb.states.random_state(200000, 3, kind="exp")
+    assert abs(np.sum(np.abs(e) ** 2) - 1) < 1e-12
+    assert tb.states.sample_seeds(1337, 3) == [1337, 2674, 4011]
+
+
+def test_splitmix64_known_answers():
+    # first outputs of splitmix64 seeded with 0 (Vigna's reference values)
+    out = tb.states.splitmix64(0, 3)
+    assert [int(x) for x in out] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4,
+                                     0x06C45D188009454F]
+
+
+def test_config_defaults_and_checks(tmp_path):
+    c = tb.Config()
+    assert c.generic["Bessel_max"] == 250 and c.generic["nr_time_steps"] == 1024
+    assert c.generic["beta"] == 1.0 / (8.617333262e-5 * 300)
+    assert c.DC_conductivity["energy_limits"] == (-0.5, 0.5)
+    c.set_temperature(100.0)
+    assert c.generic["beta"] == pytest.approx(1 / (8.617333262e-5 * 100))
+    c.generic["bogus"] = 1
+    with pytest.raises(ValueError):
+        c.check_params()
+    del c.generic["bogus"]
+    c.gpu["precision"] = "fp16"
+    with pytest.raises(ValueError):
+        c.check_params()
+
+
+class _S:
+    energy_range = 18.0
+    nr_orbitals = 2
+    area_unit_cell = 0.0524
+    volume_unit_cell = 0.0524
+    extended = 1
+
+
+def test_analyzer_dos_matches_loop_restatement():
+    cfg = tb.Config()
+    T = 128
+    cfg.generic["nr_time_steps"] = T
+    rng = np.random.default_rng(0)
+    corr = np.exp(-1j * rng.uniform(0, 3, T + 1).cumsum() / 10) * np.exp(-np.arange(T + 1) / 50)
+    corr[0] = 1.0
+    a = tb.Analyzer(_S(), cfg, dimension=2)
+    e1, d1 = a.calc_dos(corr)
+    e2, d2 = oracle.analyzer_calc_dos(corr, T, 18.0)
+    np.testing.assert_array_equal(e1, e2)
+    assert np.max(np.abs(d1 - d2)) < 1e-12 * np.max(d2)
+
+
+def test_analyzer_ac_dc_match_loop_restatement():
+    cfg = tb.Config()
+    T = 64
+    cfg.generic["nr_time_steps"] = T
+    rng = np.random.default_rng(1)
+    corr_ac = rng.normal(size=(4, T)) + 1j * rng.normal(size=(4, T))
+    corr_ac[:, 0] = 0
+    a = tb.Analyzer(_S(), cfg, dimension=2)
+    om1, ac1 = a.calc_ac_cond(corr_ac)
+    pref = 2 / (0.0524 * 1)
+    om2, ac2 = oracle.analyzer_calc_ac_cond(corr_ac, T, 18.0, cfg.generic["beta"], pref)
+    np.testing.assert_array_equal(om1, om2)
+    assert np.max(np.abs(ac1 - ac2)) < 1e-9 * np.max(np.abs(ac2))
+    corr_dos = np.exp(-np.arange(T) / 30.0) + 0j
+    lim = (-0.5, 0.5)
+    cfg.DC_conductivity["energy_limits"] = lim
+    ne = int(np.sum((np.array([0.5 * i * 18 / T - 9 for i in range(2 * T)]) >= lim[0]) &
+                    (np.array([0.5 * i * 18 / T - 9 for i in range(2 * T)]) <= lim[1])))
+    corr_dc = rng.normal(size=(2, 2 * T, T)) + 1j * rng.normal(size=(2, 2 * T, T))
+    e1, dc1 = a.calc_dc_cond(corr_dos, corr_dc)
+    e2, dc2 = oracle.analyzer_calc_dc_cond(corr_dos, corr_dc, T, 18.0, lim, pref)
+    assert len(e1) == ne
+    np.testing.assert_array_equal(e1, e2)
+    assert np.max(np.abs(dc1 - dc2)) < 1e-10 * np.max(np.abs(dc2))
+
+
+def test_fermi_coef_host_vs_oracle():
+    from tbplas_amd.tbpm import fermi_cheb_coef
+    for beta, mu, om in ((350.0, 0.0, False), (350.0, 0.0, True), (1200.0, -0.05, False)):
+        a = fermi_cheb_coef(2 ** 14, beta, mu, om, 1e-12)
+        b = oracle.orc_fermi_coef(2 ** 14, beta, mu, om, 1e-12)
+        assert len(a) == len(b) and np.max(np.abs(a - b)) < 1e-14
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_product_has_no_oracle_import():
+    """The product path never imports the oracle (no CPU fallback)."""
+    pkg = os.path.join(ROOT, "tbplas_amd")
+    for dp, _, fs in os.walk(pkg):
+        for f in fs:
+            if f.endswith((".py", ".hip", ".h", ".cpp")):
+                txt = open(os.path.join(dp, f)).read()
+                assert "import oracle" not in txt and "from oracle" not in txt, f
