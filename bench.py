@@ -53,8 +53,32 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=16)
     p.add_argument("--no-graph", action="store_true")
+    p.add_argument("--no-tune", action="store_true",
+                   help="use the default launch shape instead of the autotuner")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="time a plain device copy of known size (counter calibration)")
     p.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
+
+
+def copy_probe(nbytes=1 << 30, reps=20):
+    """Plain device-to-device copy (torch's copy kernel) of nbytes, timed
+    with events: the calibration kernel for FETCH_SIZE / WRITE_SIZE and the
+    practical streaming bandwidth of the device."""
+    import torch
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = 2 * nbytes / (ms / 1e3) / 1e9
+    print(json.dumps({"copy_bytes": nbytes, "copy_ms": ms, "copy_rw_GBs": gbs}), flush=True)
 
 
 def log(*a):
@@ -155,6 +179,9 @@ def main():
     if args.cpu_child:
         cpu_child(args)
         return 0
+    if args.copy_probe:
+        copy_probe()
+        return 0
     if args.gpus != 1:
         print(f"bench.py drives one GPU; --gpus {args.gpus} is not supported by this build "
               "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
@@ -178,7 +205,9 @@ def main():
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
 
+    from tbplas_amd import tune
     H32 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp32", handle=h)
+    shape32 = tune.apply(H32, K, bes, not args.no_tune)
     ms32, c32 = gpu_run(H32, bes, psi, args.warmup, args.steps)
     H32.close()
     t_step = ms32 / 1e3 / args.steps
@@ -198,7 +227,8 @@ def main():
                                 f"together, {args.warmup + args.steps} Chebyshev steps, "
                                 "complex64 storage/FP64 correlation"),
                    "n_orbitals": n, "nnz": nnz, "states": K, "bessel_terms": len(bes),
-                   "terms_per_step": nterm, "hipgraph": not args.no_graph},
+                   "terms_per_step": nterm, "hipgraph": not args.no_graph,
+                   "launch_shape": {"blocks_per_cu": shape32[0], "lds_stage": shape32[1]}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_term": bt,
@@ -212,6 +242,7 @@ def main():
     if not args.only_fp32:
         steps64 = args.fp64_steps or args.steps
         H64 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp64", handle=h)
+        shape64 = tune.apply(H64, 1, bes, not args.no_tune)
         ms64, c64 = gpu_run(H64, bes, psi[:1], args.warmup, steps64)
         H64.close()
         t64 = ms64 / 1e3 / steps64
@@ -220,7 +251,8 @@ def main():
         err = float(np.max(np.abs(c32[0, :tmax + 1] - c64[0, :tmax + 1])))
         out["fp64_k1"] = {"value": round(1.0 / t64, 3), "unit": "step·vectors/s",
                           "ms_per_step": round(t64 * 1e3, 4),
-                          "roofline_frac": round(b64 / (t64 / nterm) / 1e9 / HBM_PEAK_GBS, 4)}
+                          "roofline_frac": round(b64 / (t64 / nterm) / 1e9 / HBM_PEAK_GBS, 4),
+                          "launch_shape": {"blocks_per_cu": shape64[0], "lds_stage": shape64[1]}}
         out["speedup_vs_fp64_k1"] = round(value * t64, 3)
         out["ct_err_fp32_vs_fp64"] = err
         out["ct_err_steps"] = tmax

@@ -45,6 +45,10 @@ int tbg_create(tbg_handle **h, int device);
 int tbg_destroy(tbg_handle *h);
 /* 1: replay each time step from a captured hipGraph (default); 0: launch. */
 int tbg_set_graph(tbg_handle *h, int enable);
+/* Launch shape of the propagation kernels: persistent blocks per compute
+ * unit (1..64, default 16) and LDS staging of each slice's own rows (default
+ * 1).  No reference counterpart; set by the tuner (tbplas_amd/tune.py). */
+int tbg_set_tuning(tbg_handle *h, int blocks_per_cu, int stage);
 
 /* ---- CSR assembly --------------------------------------------------------
  * Replaces the Cython build_ham_dxy_fast (tbplas/cython/sample.pyx:148-257,

@@ -28,6 +28,7 @@ _SIGS = {
     "tbg_create": [ctypes.POINTER(_P), _I],
     "tbg_destroy": [_P],
     "tbg_set_graph": [_P, _I],
+    "tbg_set_tuning": [_P, _I, _I],
     "tbg_ham_dxy_count": [_P, _I64, _I64, _P, _P, _P, _D, _P, ctypes.POINTER(_I64)],
     "tbg_ham_dxy_fill": [_P, _I64, _I64, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P, _P],
     "tbg_ham_csr_count": [_P, _I64, _I64, _P, _P, _P, _P, _P, ctypes.POINTER(_I64)],
@@ -131,6 +132,10 @@ class Handle:
 
     def set_graph(self, enable):
         check(lib().tbg_set_graph(self._h, 1 if enable else 0))
+
+    def set_tuning(self, blocks_per_cu=16, stage=True):
+        check(lib().tbg_set_tuning(self._h, int(blocks_per_cu), 1 if stage else 0))
+        self.tuning = (int(blocks_per_cu), bool(stage))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -9,6 +9,8 @@ look at:
                        arithmetic, FP64 correlation sums)     default 'fp64'
     gpu['device']      GPU index                              default 0
     gpu['use_graph']   replay time steps from captured hipGraphs  default True
+    gpu['autotune']    time the candidate launch shapes once per problem
+                       shape and keep the fastest (tbplas_amd.tune) default True
     gpu['max_vectors'] states propagated together per launch  default 16
     gpu['state_kind']  'phase' (unit amplitude, random phase) or 'exp'
                        (the reference's Exp(1) amplitudes)    default 'phase'
@@ -54,6 +56,7 @@ class Config:
         self.gpu = {'precision': 'fp64',
                     'device': 0,
                     'use_graph': True,
+                    'autotune': True,
                     'max_vectors': 16,
                     'state_kind': 'phase',
                     'fp32_norm_thr': 1.0e-4}

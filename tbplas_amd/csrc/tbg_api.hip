@@ -38,13 +38,6 @@ static void check_handle(tbg_handle* h) {
     TBG_HIP(hipSetDevice(h->device));
 }
 
-// Target number of persistent blocks: enough waves to cover latency on all
-// CUs without making chunks smaller than one 4-slice tile.
-static int chunk_target(tbg_handle* h, int64_t nslices) {
-    const int64_t by_cu = (int64_t)h->num_cus * 8;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(by_cu, (nslices + 3) / 4));
-}
-
 // Norm check of the reference (funcs.f90:175-192): |n - n_ref| > thr or NaN.
 static bool norm_broken(double n, double ref, double thr) {
     const double d = std::fabs(n - ref);
@@ -245,6 +238,15 @@ int tbg_set_graph(tbg_handle* h, int enable) {
     return guard([&] {
         check_handle(h);
         h->use_graph = enable ? 1 : 0;
+    });
+}
+
+int tbg_set_tuning(tbg_handle* h, int blocks_per_cu, int stage) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(blocks_per_cu >= 1 && blocks_per_cu <= 64, "blocks_per_cu must be in [1, 64]");
+        h->blocks_per_cu = blocks_per_cu;
+        h->stage = stage ? 1 : 0;
     });
 }
 

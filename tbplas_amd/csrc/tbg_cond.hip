@@ -328,10 +328,7 @@ static bool broken(double n, double ref, double thr) {
     return std::isnan(d) || !std::isfinite(d) || d > thr;
 }
 
-static int chunks_for(tbg_handle* h, int64_t nslices) {
-    const int64_t by_cu = (int64_t)h->num_cus * 8;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(by_cu, (nslices + 3) / 4));
-}
+static int chunks_for(tbg_handle* h, int64_t nslices) { return chunk_target(h, nslices); }
 
 template <typename R>
 void corr_ac_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, const double* cF,

@@ -28,6 +28,13 @@ inline MatView view_of(const tbg_ham* H) {
     return m;
 }
 
+// Persistent blocks for the term kernels: blocks_per_cu per CU, but no
+// chunk smaller than one 4-slice tile.
+inline int chunk_target(const tbg_handle* h, int64_t nslices) {
+    const int64_t by_cu = (int64_t)h->num_cus * h->blocks_per_cu;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(by_cu, (nslices + 3) / 4));
+}
+
 // Vector slots: a power of two >= the vectors needed (lanes per row must
 // divide the wavefront).
 inline int slots_for(int k_needed) {
@@ -112,6 +119,7 @@ struct Engine {
         a.bwd_mask = bwd;
         a.cls_mask = cls;
         a.nv = 3 * Kp;
+        a.stage = h->stage;
         return a;
     }
 

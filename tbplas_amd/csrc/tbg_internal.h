@@ -86,6 +86,10 @@ struct tbg_handle {
     hipStream_t stream = nullptr;
     int num_cus = 256;
     int use_graph = 1;
+    // Launch shape of the term kernels (set by tbg_set_tuning / the tuner):
+    // persistent blocks per CU and LDS staging of the slice's own rows.
+    int blocks_per_cu = 16;
+    int stage = 1;
     tbg::DevBuf scratch;      // reduction partials and small tables
     tbg::DevBuf counter;      // step counter for graph-replayed reductions
 };

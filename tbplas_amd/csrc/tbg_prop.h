@@ -40,6 +40,7 @@ struct TermArgs {
     double* part;            // [nchunks][nv] partial sums
     int nv;                  // values per partial row (3K for dot+norm)
     int* counter;            // incremented once by block 0 when non-null
+    int stage;               // stage the slice's own rows of pin in LDS
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).

@@ -74,9 +74,15 @@ __device__ __forceinline__ void vst(void* base, int64_t idx, const Vec<R, V>& o)
     }
 }
 
-template <typename R, int V, int FL>
+// STAGE: the slice's own 64 rows of v_{m-1} are first read once, coalesced,
+// into LDS; gathers whose column falls inside the slice (for lattice models
+// most of them: the +-1 neighbours) and the v_{m-1} term of the
+// accumulation are then served from LDS, and only the remaining gathers go
+// through L2.  Each wave owns its own 64*K-element LDS area.
+template <typename R, int V, int FL, bool STAGE>
 __global__ void __launch_bounds__(256) k_term(TermArgs a) {
     using C = typename CplxT<R>::T;
+    extern __shared__ __align__(16) unsigned char tbg_smem[];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int c = remap_chunk(blockIdx.x, gridDim.x);
@@ -107,24 +113,44 @@ __global__ void __launch_bounds__(256) k_term(TermArgs a) {
 #pragma unroll
     for (int v = 0; v < V; ++v) { dre[v] = 0.0; dim_[v] = 0.0; nrm[v] = 0.0; }
 
+    unsigned char* stage = tbg_smem + (size_t)wv * 64 * K * sizeof(C);
     const int64_t s0 = a.chunk[c], s1 = a.chunk[c + 1];
     for (int64_t sl = s0 + wv; sl < s1; sl += 4) {
         const int64_t off = a.soff[sl];
         const int w = a.sw[sl];
         const int* colp = a.cols + off;
         const C* valp = reinterpret_cast<const C*>(a.vals) + off;
+        const int64_t row0 = sl << 6;
+        if constexpr (STAGE) {
+            for (int p = 0; p < G; ++p) {
+                const int lr = p * rpp + rsub;
+                vst<R, V>(stage, (int64_t)lr * K + k0,
+                          vld<R, V>(a.pin, (row0 + lr) * (int64_t)K + k0));
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
         for (int p = 0; p < G; ++p) {
             const int lr = p * rpp + rsub;
-            const int64_t vidx = ((sl << 6) + lr) * (int64_t)K + k0;
+            const int64_t vidx = (row0 + lr) * (int64_t)K + k0;
             Vec<R, V> prev, mid, accv;
             if constexpr ((FL & F_PREV) != 0) prev = vld<R, V>(a.pprev, vidx);
-            if constexpr ((FL & F_MID) != 0) mid = vld<R, V>(a.pin, vidx);
+            if constexpr ((FL & F_MID) != 0) {
+                if constexpr (STAGE) mid = vld<R, V>(stage, (int64_t)lr * K + k0);
+                else mid = vld<R, V>(a.pin, vidx);
+            }
             if constexpr ((FL & F_ACCRMW) != 0) accv = vld<R, V>(a.acc, vidx);
             Vec<R, V> s = vzero<R, V>();
             for (int j = 0; j < w; ++j) {
                 const int col = colp[j * 64 + lr];
                 const C hv = valp[j * 64 + lr];
-                const Vec<R, V> x = vld<R, V>(a.pin, (int64_t)col * K + k0);
+                Vec<R, V> x;
+                if constexpr (STAGE) {
+                    const uint32_t d = (uint32_t)((int64_t)col - row0);
+                    if (d < 64u) x = vld<R, V>(stage, (int64_t)d * K + k0);
+                    else x = vld<R, V>(a.pin, (int64_t)col * K + k0);
+                } else {
+                    x = vld<R, V>(a.pin, (int64_t)col * K + k0);
+                }
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     s.re[v] = fma(hv.x, x.re[v], s.re[v]);
@@ -180,6 +206,7 @@ __global__ void __launch_bounds__(256) k_term(TermArgs a) {
                 }
             }
         }
+        if constexpr (STAGE) __builtin_amdgcn_wave_barrier();
     }
 
     if constexpr ((FL & (F_DOT | F_NORM)) != 0) {
@@ -208,30 +235,25 @@ __global__ void __launch_bounds__(256) k_term(TermArgs a) {
     }
 }
 
-// Fixed-order reduction of partial rows, one block per instance.
+// Fixed-order reduction of partial rows: block (q, inst) sums value q over
+// the instance's chunks (strided per thread, then an LDS tree).
 __global__ void __launch_bounds__(256)
 k_reduce(const double* part, int nv, const int* inst_chunk, double* out,
          int64_t inst_stride, const int* counter, int toff) {
     __shared__ double sh[256];
-    const int inst = blockIdx.x;
+    const int q = blockIdx.x, inst = blockIdx.y;
     const int c0 = inst_chunk[inst], c1 = inst_chunk[inst + 1];
-    const int t = (counter ? *counter : 0) + toff;
-    double* o = out + inst * inst_stride + (int64_t)t * nv;
-    for (int qb = 0; qb < nv; qb += 256) {
-        const int nq = min(256, nv - qb);
-        const int RG = 256 / nq;
-        const int q = threadIdx.x % nq, rg = threadIdx.x / nq;
-        double acc = 0.0;
-        if (rg < RG)
-            for (int c = c0 + rg; c < c1; c += RG) acc += part[(int64_t)c * nv + qb + q];
-        sh[threadIdx.x] = acc;
+    double acc = 0.0;
+    for (int c = c0 + threadIdx.x; c < c1; c += 256) acc += part[(int64_t)c * nv + q];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
         __syncthreads();
-        if (threadIdx.x < nq) {
-            double s = 0.0;
-            for (int r = 0; r < RG; ++r) s += sh[r * nq + threadIdx.x];
-            o[qb + threadIdx.x] = s;
-        }
-        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int t = (counter ? *counter : 0) + toff;
+        out[inst * inst_stride + (int64_t)t * nv + q] = sh[0];
     }
 }
 
@@ -347,12 +369,13 @@ __global__ void k_fill_zero(uint4* p, int64_t n16) {
     X(F_PREV | F_ACC | F_ACCRMW | F_MID | F_NORM)                              \
     X(F_PREV | F_ACC | F_ACCRMW | F_MID | F_NORM | F_DOT)
 
-template <typename R, int V>
+template <typename R, int V, bool STAGE>
 static void launch_term_v(int flags, const TermArgs& a, int nchunks, hipStream_t s) {
+    const size_t lds = STAGE ? (size_t)4 * 64 * a.K * sizeof(typename CplxT<R>::T) : 0;
     switch (flags) {
 #define TBG_CASE(F)                                                            \
     case (F):                                                                  \
-        hipLaunchKernelGGL((k_term<R, V, (F)>), dim3(nchunks), dim3(256), 0, s, a); \
+        hipLaunchKernelGGL((k_term<R, V, (F), STAGE>), dim3(nchunks), dim3(256), lds, s, a); \
         break;
         TBG_TERM_FLAG_LIST(TBG_CASE)
 #undef TBG_CASE
@@ -367,15 +390,25 @@ void launch_term(int flags, const TermArgs& a, int nchunks, hipStream_t s) {
     const int V = (std::is_same<R, float>::value && a.K >= 2) ? 2 : 1;
     if (a.K > 64 || (a.K & (a.K - 1)) || (1 << a.lgG) * V != a.K)
         throw Error{TBG_EINVAL, "internal: bad vector slot count"};
-    if (V == 2) launch_term_v<R, 2>(flags, a, nchunks, s);
-    else launch_term_v<R, 1>(flags, a, nchunks, s);
+    // Stage the slice in LDS while one wave's area stays within 4 KB
+    // (complex64 K <= 8, complex128 K <= 4): 16 KB per block.
+    const bool stage = a.stage && 64 * a.K * sizeof(typename CplxT<R>::T) <= 4096;
+    if constexpr (std::is_same<R, float>::value) {
+        if (V == 2) {
+            if (stage) launch_term_v<R, 2, true>(flags, a, nchunks, s);
+            else launch_term_v<R, 2, false>(flags, a, nchunks, s);
+            return;
+        }
+    }
+    if (stage) launch_term_v<R, 1, true>(flags, a, nchunks, s);
+    else launch_term_v<R, 1, false>(flags, a, nchunks, s);
 }
 template void launch_term<float>(int, const TermArgs&, int, hipStream_t);
 template void launch_term<double>(int, const TermArgs&, int, hipStream_t);
 
 void launch_reduce(const double* part, int nv, const int* inst_chunk, int ninst, double* out,
                    int64_t inst_stride, const int* counter, int toff, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce, dim3(ninst), dim3(256), 0, s, part, nv, inst_chunk, out,
+    hipLaunchKernelGGL(k_reduce, dim3(nv, ninst), dim3(256), 0, s, part, nv, inst_chunk, out,
                        inst_stride, counter, toff);
     TBG_HIP(hipGetLastError());
 }

@@ -16,7 +16,7 @@ import time
 
 import numpy as np
 
-from . import _lib, states
+from . import _lib, states, tune
 from .config import Config
 from .sample import Sample
 from .tbpm import (Hamiltonian, bessel_series, corr_ac, corr_dc, corr_dos,
@@ -150,6 +150,7 @@ class Solver:
         chk, thr = self._chk()
         T = self._config.generic["nr_time_steps"]
         per = max(1, min(64, int(gpu["max_vectors"])))
+        tune.apply(H, min(per, num_sample), bes, gpu["autotune"])
         corr = np.zeros(T + 1, dtype=np.complex128)
         for a, b in self._batches(num_sample, per):
             c = corr_dos(H, bes, psi0[a:b], T, chk, thr)
@@ -182,6 +183,7 @@ class Solver:
         chk, thr = self._chk()
         T = g["nr_time_steps"]
         per = max(1, min(21, int(gpu["max_vectors"])))
+        tune.apply(H, 3 * min(per, num_sample), bes, gpu["autotune"])
         corr = np.zeros((4, T), dtype=np.complex128)
         for a, b in self._batches(num_sample, per):
             c = corr_ac(H, bes, coef_F, coef_omF, psi0[a:b], T, chk, thr)
@@ -215,6 +217,7 @@ class Solver:
         psi0 = self._states(num_sample)
         chk, thr = self._chk()
         per = max(1, min(32, int(gpu["max_vectors"])))
+        tune.apply(H, 2 * min(per, num_sample), bes, gpu["autotune"])
         corr_dos_ = np.zeros(tnr, dtype=np.complex128)
         corr_dc_ = np.zeros((2, len(energies_dos), tnr), dtype=np.complex128)
         n_en = len(qe_indices)

@@ -1,0 +1,102 @@
+"""Autotuner for the launch shape of the propagation kernels.
+
+For a prepared Hamiltonian, a state count and a precision it times a few
+Chebyshev steps (HIP events on the library stream, after warm-up steps) for
+every candidate (persistent blocks per CU, LDS staging of the slice rows)
+and keeps the fastest.  Results are cached per (device name, rows, stored
+elements, states, precision) in memory and in a JSON file
+(``$TBPLAS_AMD_TUNE_CACHE``, default ``~/.cache/tbplas_amd/tune.json``).
+"""
+import json
+import os
+import threading
+
+import numpy as np
+
+from . import _lib
+
+__all__ = ["autotune", "apply", "CANDIDATES"]
+
+CANDIDATES = [(bpc, stage) for bpc in (4, 8, 12, 16) for stage in (True, False)]
+DEFAULT = (16, True)
+
+_lock = threading.Lock()
+_mem = {}
+
+
+def _cache_path():
+    return os.environ.get("TBPLAS_AMD_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "tbplas_amd",
+                                       "tune.json"))
+
+
+def _load_disk():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save_disk(d):
+    p = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        tmp = p + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        os.replace(tmp, p)
+    except OSError:
+        pass
+
+
+def _key(H, K):
+    import torch
+    info = H.info()
+    dev = torch.cuda.get_device_name(H.handle.device)
+    return f"{dev}|n={info['n']}|pad={info['padded']}|w={info['max_width']}|K={K}|p={H.precision}"
+
+
+def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
+    """Pick (blocks_per_cu, stage) for Hamiltonian H and K states; returns
+    (best, timings) and leaves the handle set to the best shape."""
+    from .tbpm import DosRun
+    key = _key(H, K)
+    with _lock:
+        if use_cache:
+            if key in _mem:
+                best = tuple(_mem[key])
+                H.handle.set_tuning(*best)
+                return best, None
+            disk = _load_disk()
+            if key in disk:
+                best = (int(disk[key][0]), bool(disk[key][1]))
+                _mem[key] = best
+                H.handle.set_tuning(*best)
+                return best, None
+        rng = np.random.default_rng(0)
+        psi = np.exp(2j * np.pi * rng.random((K, H.n)))
+        psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+        timings = {}
+        for cand in candidates or CANDIDATES:
+            H.handle.set_tuning(*cand)
+            run = DosRun(H, bes, psi, warmup + steps)
+            run.advance(warmup)
+            timings[cand] = run.advance(steps) / steps
+            run.close()
+        best = min(timings, key=timings.get)
+        H.handle.set_tuning(*best)
+        _mem[key] = best
+        if use_cache:
+            disk = _load_disk()
+            disk[key] = [best[0], best[1]]
+            _save_disk(disk)
+        return best, timings
+
+
+def apply(H, K, bes, enabled=True):
+    """Tune when enabled, otherwise use the default shape."""
+    if enabled:
+        return autotune(H, K, bes)[0]
+    H.handle.set_tuning(*DEFAULT)
+    return DEFAULT

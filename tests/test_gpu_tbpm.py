@@ -107,6 +107,32 @@ def test_graph_replay_equals_launches(gpu):
     np.testing.assert_array_equal(a, c)
 
 
+@pytest.mark.parametrize("prec,K", [("fp32", 8), ("fp32", 1), ("fp64", 1), ("fp64", 4)])
+def test_launch_shapes_agree(gpu, prec, K):
+    """LDS staging on/off gives identical states; other block counts only
+    reorder the FP64 correlation sums."""
+    s = tb.make_graphene_sample(24, vacancies=tb.random_vacancies(2 * 24 * 24, 0.02, 1))
+    tb.add_anderson_disorder(s, 0.5, 2)
+    s.rescale_ham(9.0)
+    ip, ix, dt = _csr(s)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, K)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec)
+    h = H.handle
+    try:
+        h.set_tuning(8, True)
+        a = tbpm.corr_dos(H, bes, psi, 20, chk_thr=1e-4)
+        h.set_tuning(8, False)
+        b = tbpm.corr_dos(H, bes, psi, 20, chk_thr=1e-4)
+        np.testing.assert_array_equal(a, b)
+        for bpc in (1, 3, 16):
+            h.set_tuning(bpc, True)
+            c = tbpm.corr_dos(H, bes, psi, 20, chk_thr=1e-4)
+            assert np.max(np.abs(c - a)) < 1e-14
+    finally:
+        h.set_tuning(8, True)
+
+
 def test_dos_run_pieces_equal_one_call(gpu):
     s = graphene_sample(12)
     ip, ix, dt = _csr(s)
