@@ -195,6 +195,33 @@ int tbg_dos_end(tbg_run *r, int chk_step, double chk_thr,
                 double *const *corr, int *fail_inst, int *fail_step);
 int tbg_run_destroy(tbg_run *r);
 
+/* ---- learned DOS surrogate -------------------------------------------------
+ * No reference counterpart.  A dense network x -> GELU(x W_0 + b_0) -> ... ->
+ * x W_{L-1} + b_{L-1} on the matrix cores: BF16 operands, FP32 accumulation,
+ * FP32 master weights, Adam.  widths: n_layers+1 entries (input, hidden...,
+ * output).  Parameters are packed per layer as W_l [in][out] row-major, then
+ * b_l [out]; set/get use HOST FP32 arrays.  x, target, y are DEVICE FP32
+ * row-major [B][features].  The loss is the mean squared error. */
+typedef struct tbg_mlp tbg_mlp;
+int tbg_mlp_create(tbg_handle *h, int n_layers, const int *widths,
+                   tbg_mlp **out);
+int tbg_mlp_destroy(tbg_mlp *m);
+int tbg_mlp_num_params(const tbg_mlp *m, int64_t *n);
+int tbg_mlp_set_params(tbg_mlp *m, const float *params, int64_t n);
+int tbg_mlp_get_params(const tbg_mlp *m, float *params, int64_t n);
+int tbg_mlp_forward(tbg_mlp *m, const float *x, int B, float *y);
+/* loss and host gradients (packed like the parameters), no update */
+int tbg_mlp_gradients(tbg_mlp *m, const float *x, const float *target, int B,
+                      float *grads, double *loss);
+/* `steps` Adam steps on one batch; *loss = loss before the last step */
+int tbg_mlp_train(tbg_mlp *m, const float *x, const float *target, int B,
+                  int steps, float lr, float beta1, float beta2, float eps,
+                  double *loss);
+/* C [M][N] (FP32) = A [M][K] B [K][N], row-major BF16 bit patterns, on the
+ * same MFMA kernel (device pointers) */
+int tbg_gemm_bf16(tbg_handle *h, int M, int N, int K, const uint16_t *A,
+                  const uint16_t *B, float *C);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,16 @@ _SIGS = {
     "tbg_run_advance": [_P, _I, ctypes.POINTER(_D)],
     "tbg_dos_end": [_P, _I, _D, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "tbg_run_destroy": [_P],
+    "tbg_mlp_create": [_P, _I, _P, ctypes.POINTER(_P)],
+    "tbg_mlp_destroy": [_P],
+    "tbg_mlp_num_params": [_P, ctypes.POINTER(_I64)],
+    "tbg_mlp_set_params": [_P, _P, _I64],
+    "tbg_mlp_get_params": [_P, _P, _I64],
+    "tbg_mlp_forward": [_P, _P, _I, _P],
+    "tbg_mlp_gradients": [_P, _P, _P, _I, _P, ctypes.POINTER(_D)],
+    "tbg_mlp_train": [_P, _P, _P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                      ctypes.c_float, ctypes.POINTER(_D)],
+    "tbg_gemm_bf16": [_P, _I, _I, _I, _P, _P, _P],
 }
 _RESTYPE = {"tbg_last_error": ctypes.c_char_p}
 

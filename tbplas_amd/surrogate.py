@@ -1,0 +1,344 @@
+"""Learned DOS surrogate with TBPM as fallback and judge.
+
+A dense network maps (vacancy fraction, Anderson disorder W, uniaxial
+strain) of a graphene sample to its DOS on a fixed 1024-point grid in
+[-9, 9] eV.  It is trained on the GPU by the library's own BF16-MFMA
+forward/backward and Adam kernels (``tbg_mlp_*``), on spectra produced by
+the GPU TBPM path itself.  At inference a short TBPM run (a few time steps)
+checks the prediction: the predicted DOS implies a correlation function
+C(t) = sum_E rho(E) exp(-i E t dt) dE, and if it departs from the measured
+C(t) by more than a calibrated threshold the full TBPM result is used
+instead.
+"""
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _lib, states
+from .sample import (make_graphene_sample, random_vacancies, add_anderson_disorder,
+                     apply_uniaxial_strain)
+from .tbpm import Hamiltonian, bessel_series, corr_dos_batch
+
+__all__ = ["MLP", "DOS_GRID", "PARAM_RANGES", "sample_params", "build_sample",
+           "tbpm_dos", "dos_from_corr", "corr_from_dos", "DOSSurrogate", "l1_error"]
+
+DOS_GRID = np.linspace(-9.0, 9.0, 1024)
+DOS_STEP = DOS_GRID[1] - DOS_GRID[0]
+PARAM_RANGES = np.array([[0.0, 0.05],    # vacancy fraction
+                         [0.0, 3.0],     # Anderson W (eV): orb_eng += U[-W/2, W/2]
+                         [0.0, 0.05]])   # uniaxial strain along x
+
+
+class MLP:
+    """The network on the GPU (tbg_mlp), FP32 master weights."""
+
+    def __init__(self, widths=(3, 256, 256, 1024), seed=0, handle=None):
+        self.widths = tuple(int(w) for w in widths)
+        self.handle = handle or _lib.default_handle(0)
+        L = len(self.widths) - 1
+        w = (ctypes.c_int * (L + 1))(*self.widths)
+        out = ctypes.c_void_p()
+        _lib.check(_lib.lib().tbg_mlp_create(self.handle.ptr, L, w, ctypes.byref(out)))
+        self._m = out
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().tbg_mlp_num_params(self._m, ctypes.byref(n)))
+        self.n_params = n.value
+        self.set_params(self.init_params(seed))
+
+    def init_params(self, seed):
+        """U(-1/sqrt(in), 1/sqrt(in)) for weights and biases (seeded)."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        for a, b in zip(self.widths[:-1], self.widths[1:]):
+            s = 1.0 / math.sqrt(a)
+            parts.append(rng.uniform(-s, s, a * b))
+            parts.append(rng.uniform(-s, s, b))
+        return np.concatenate(parts).astype(np.float32)
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _lib.check(_lib.lib().tbg_mlp_set_params(self._m, p.ctypes.data, len(p)))
+
+    def get_params(self):
+        p = np.empty(self.n_params, dtype=np.float32)
+        _lib.check(_lib.lib().tbg_mlp_get_params(self._m, p.ctypes.data, len(p)))
+        return p
+
+    def _dev(self, a):
+        return _lib.dev(np.ascontiguousarray(a, dtype=np.float32), np.float32, self.handle.device)
+
+    def forward(self, x):
+        torch = _lib.torch_cuda()
+        x = np.atleast_2d(x)
+        xd = self._dev(x)
+        y = torch.empty((x.shape[0], self.widths[-1]), dtype=torch.float32,
+                        device=f"cuda:{self.handle.device}")
+        _lib.check(_lib.lib().tbg_mlp_forward(self._m, _lib.ptr(xd), x.shape[0], _lib.ptr(y)))
+        return y.cpu().numpy()
+
+    def gradients(self, x, t):
+        x, t = np.atleast_2d(x), np.atleast_2d(t)
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = ctypes.c_double()
+        # keep the device tensors referenced for the duration of the call
+        xd, td = self._dev(x), self._dev(t)
+        _lib.check(_lib.lib().tbg_mlp_gradients(self._m, _lib.ptr(xd), _lib.ptr(td), x.shape[0],
+                                                g.ctypes.data, ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, t, steps, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, chunk=100):
+        """Full-batch Adam on the MSE loss; returns the loss history."""
+        x, t = np.atleast_2d(x), np.atleast_2d(t)
+        xd, td = self._dev(x), self._dev(t)
+        hist = []
+        done = 0
+        while done < steps:
+            k = min(chunk, steps - done)
+            loss = ctypes.c_double()
+            lr_k = lr(done) if callable(lr) else lr
+            _lib.check(_lib.lib().tbg_mlp_train(self._m, _lib.ptr(xd), _lib.ptr(td), x.shape[0],
+                                                k, lr_k, betas[0], betas[1], eps,
+                                                ctypes.byref(loss)))
+            hist.append(loss.value)
+            done += k
+        return hist
+
+    def close(self):
+        if getattr(self, "_m", None):
+            _lib.lib().tbg_mlp_destroy(self._m)
+            self._m = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Data from the TBPM path
+# ---------------------------------------------------------------------------
+def sample_params(n, seed):
+    rng = np.random.default_rng(seed)
+    lo, hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
+    return lo + (hi - lo) * rng.random((n, 3))
+
+
+def build_sample(p, L, seed):
+    """Graphene L x L with the given (vacancy, W, strain), rescaled by the
+    row-sum bound of |H| (sample.pyx:75-114) so the expansion converges."""
+    vac, W, eps = p
+    vacs = random_vacancies(2 * L * L, vac, seed) if vac > 0 else None
+    s = make_graphene_sample(L, vacancies=vacs)
+    if W > 0:
+        add_anderson_disorder(s, W, seed + 1)
+    if eps > 0:
+        apply_uniaxial_strain(s, eps)
+    s.rescale_ham()
+    return s
+
+
+def dos_from_corr(corr, en_range, grid=DOS_GRID):
+    """Analyzer.calc_dos (analysis.py:89-143) on the sample's own grid, then
+    linear interpolation to the fixed grid and unit integral there."""
+    T = len(corr) - 1
+    cn = np.empty(2 * T, dtype=complex)
+    i = np.arange(T - 1)
+    w = 0.5 * (1 + np.cos(np.pi * i / T))
+    cn[T - 1] = corr[0]
+    cn[2 * T - 1] = 0.5 * (1 + math.cos(math.pi * (T - 1) / T)) * corr[T]
+    cn[T + i] = w * corr[i + 1]
+    cn[T - i - 2] = w * np.conj(corr[i + 1])
+    cf = np.fft.ifft(cn)
+    dos = np.empty(2 * T)
+    dos[T:] = np.abs(cf[:T])
+    dos[:T] = np.abs(cf[T:])
+    en = np.array([0.5 * k * en_range / T - en_range / 2. for k in range(2 * T)])
+    dos = dos / (np.sum(dos) * 0.5 * en_range / T)
+    out = np.interp(grid, en, dos, left=0.0, right=0.0)
+    return out / (np.sum(out) * (grid[1] - grid[0]))
+
+
+def corr_from_dos(dos, en_range, steps, grid=DOS_GRID):
+    """C(t) implied by a DOS on the grid, at the sample's time step
+    dt = 2 pi / en_range (solver.py:175-182)."""
+    dt = 2 * math.pi / en_range
+    t = np.arange(steps + 1)
+    ph = np.exp(-1j * np.outer(t * dt, grid))
+    return ph @ (dos * (grid[1] - grid[0]))
+
+
+def tbpm_dos(param_list, L, T=512, K=4, seed=1000, precision="fp32", batch=16,
+             return_corr=False):
+    """DOS of each parameter set from the GPU TBPM (batched runs)."""
+    dos, corrs, ranges = [], [], []
+    for a in range(0, len(param_list), batch):
+        hams, psis, ens = [], [], []
+        for j, p in enumerate(param_list[a:a + batch]):
+            s = build_sample(p, L, seed + 7919 * (a + j))
+            ip, ix, dt = s.build_ham_csr_device()
+            hams.append(Hamiltonian(s.num_orb, ip, ix, dt, precision=precision))
+            psis.append(states.random_states(s.num_orb,
+                                             states.sample_seeds(seed + a + j, K)))
+            ens.append(s.energy_range)
+        bes = bessel_series(math.pi, 1e-14)   # tau = dt * rescale = pi for every sample
+        out = corr_dos_batch(hams, bes, psis, T, chk_thr=1e-3)
+        for c, er in zip(out, ens):
+            cm = c.mean(axis=0)
+            corrs.append(cm)
+            ranges.append(er)
+            dos.append(dos_from_corr(cm, er))
+        for H in hams:
+            H.close()
+    dos = np.array(dos)
+    if return_corr:
+        return dos, corrs, ranges
+    return dos
+
+
+def l1_error(pred, ref):
+    """Integral of |rho_pred - rho_ref| over the grid (both unit integral)."""
+    return np.sum(np.abs(pred - ref), axis=-1) * DOS_STEP
+
+
+class DOSSurrogate:
+    """Network + normalisation + the short-TBPM gate."""
+
+    def __init__(self, widths=(3, 256, 256, 1024), seed=0, target_scale=10.0):
+        self.net = MLP(widths, seed)
+        self.target_scale = target_scale
+        self.gate_steps = 16
+        self.gate_thr = None
+
+    def _x(self, params):
+        p = np.atleast_2d(params)
+        lo, hi = PARAM_RANGES[:, 0], PARAM_RANGES[:, 1]
+        return (p - lo) / (hi - lo)
+
+    def fit(self, params, dos, steps=4000, lr=2e-3):
+        sched = lambda k: lr * (0.5 * (1 + math.cos(math.pi * min(k, steps) / steps)) + 0.02)
+        return self.net.train(self._x(params), dos * self.target_scale, steps, lr=sched)
+
+    def predict(self, params):
+        y = self.net.forward(self._x(params)) / self.target_scale
+        y = np.maximum(y, 0.0)
+        return y / (np.sum(y, axis=-1, keepdims=True) * DOS_STEP)
+
+    def gate_distance(self, dos_pred, corr_short, en_range):
+        """max_t |C_pred(t) - C_tbpm(t)| over the short run."""
+        c = corr_from_dos(dos_pred, en_range, len(corr_short) - 1)
+        return float(np.max(np.abs(c - corr_short)))
+
+    def calibrate(self, params, dos_ref, corr_short, en_ranges, tol=0.02):
+        """Threshold on the gate distance: the largest value such that every
+        calibration point below it has L1 error <= tol."""
+        pred = self.predict(params)
+        err = l1_error(pred, dos_ref)
+        d = np.array([self.gate_distance(p, c, e) for p, c, e in zip(pred, corr_short, en_ranges)])
+        bad = d[err > tol]
+        self.gate_thr = float(np.min(bad)) * 0.999 if len(bad) else float(np.max(d)) * 1.5
+        return d, err
+
+    def save(self, path):
+        np.savez(path, params=self.net.get_params(), widths=np.array(self.net.widths),
+                 target_scale=self.target_scale, gate_thr=np.nan if self.gate_thr is None
+                 else self.gate_thr, gate_steps=self.gate_steps)
+
+    def predict_or_fallback(self, params, L, K=4, seed=5000, T_full=512):
+        """Surrogate DOS where the gate accepts it, full TBPM elsewhere.
+        Returns (dos, used_fallback mask, timings)."""
+        t0 = time.perf_counter()
+        pred = self.predict(params)
+        t1 = time.perf_counter()
+        _, corr_s, ens = tbpm_dos(params, L, T=self.gate_steps, K=K, seed=seed, return_corr=True)
+        t2 = time.perf_counter()
+        d = np.array([self.gate_distance(p, c, e) for p, c, e in zip(pred, corr_s, ens)])
+        fb = d > self.gate_thr
+        out = pred.copy()
+        if np.any(fb):
+            out[fb] = tbpm_dos([params[i] for i in np.nonzero(fb)[0]], L, T=T_full, K=K,
+                               seed=seed)
+        t3 = time.perf_counter()
+        return out, fb, {"predict_s": t1 - t0, "gate_s": t2 - t1, "fallback_s": t3 - t2}
+
+
+def study(L=512, n_train=512, n_cal=64, n_test=64, T=512, K=4, steps=4000, tol=0.02,
+          widths=(3, 256, 256, 1024), ref_K=16, log=print):
+    """Train on GPU-TBPM spectra and report what the surrogate gains.
+
+    Training spectra use K random states; calibration and held-out spectra
+    use ref_K states so the reference is less noisy than the tolerance.
+    Reported: held-out L1 errors of the network, of a plain K-state TBPM run
+    (the method without the network, at the training cost per point), the
+    gate threshold and fallbacks, the final errors, and times per point."""
+    import torch
+    rep = {"L": L, "n_train": n_train, "n_cal": n_cal, "n_test": n_test, "T": T, "K": K,
+           "ref_K": ref_K, "widths": list(widths), "steps": steps, "tol": tol}
+    ptr = sample_params(n_train, seed=1)
+    pca = sample_params(n_cal, seed=2)
+    pte = sample_params(n_test, seed=3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dtr = tbpm_dos(ptr, L, T=T, K=K, seed=10_000)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rep["tbpm_s_per_point"] = (t1 - t0) / n_train
+    log(f"training data: {n_train} TBPM runs ({K} states) in {t1 - t0:.1f} s")
+    dca = tbpm_dos(pca, L, T=T, K=ref_K, seed=20_000)
+    dte = tbpm_dos(pte, L, T=T, K=ref_K, seed=30_000)
+    s = DOSSurrogate(widths, seed=0)
+    t2 = time.perf_counter()
+    hist = s.fit(ptr, dtr, steps=steps)
+    t3 = time.perf_counter()
+    rep["train_s"] = t3 - t2
+    rep["loss_first_last"] = [hist[0], hist[-1]]
+    log(f"trained {steps} Adam steps in {t3 - t2:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+
+    def summary(e):
+        return {"mean": float(e.mean()), "median": float(np.median(e)), "max": float(e.max()),
+                "frac_above_tol": float(np.mean(e > tol))}
+
+    t4 = time.perf_counter()
+    pred = s.predict(pte)
+    t5 = time.perf_counter()
+    rep["test_l1_network"] = summary(l1_error(pred, dte))
+    rep["network_s_per_point"] = (t5 - t4) / n_test
+    # the method without the network at the same cost per point: K states
+    rep["test_l1_tbpm_K"] = summary(l1_error(tbpm_dos(pte, L, T=T, K=K, seed=60_000), dte))
+    _, ccal, ecal = tbpm_dos(pca, L, T=s.gate_steps, K=ref_K, seed=40_000, return_corr=True)
+    d_cal, e_cal = s.calibrate(pca, dca, ccal, ecal, tol=tol)
+    rep["gate_threshold"] = s.gate_thr
+    rep["cal_l1_network"] = summary(e_cal)
+    t6 = time.perf_counter()
+    out, fb, tm = s.predict_or_fallback(pte, L, K=ref_K, seed=50_000, T_full=T)
+    t7 = time.perf_counter()
+    err = l1_error(out, dte)
+    rep["test_fallbacks"] = int(fb.sum())
+    rep["test_l1_final"] = {"mean": float(err.mean()), "max": float(err.max()),
+                            "max_accepted": float(err[~fb].max()) if np.any(~fb) else None}
+    rep["timing"] = {k: v / n_test for k, v in tm.items()}
+    rep["gated_s_per_point"] = (t7 - t6) / n_test
+    rep["speedup_network_vs_tbpm"] = rep["tbpm_s_per_point"] / rep["network_s_per_point"]
+    rep["speedup_gated_vs_tbpm"] = rep["tbpm_s_per_point"] / rep["gated_s_per_point"]
+    return rep
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="train and evaluate the DOS surrogate")
+    ap.add_argument("--L", type=int, default=512)
+    ap.add_argument("--train", type=int, default=512)
+    ap.add_argument("--cal", type=int, default=64)
+    ap.add_argument("--test", type=int, default=64)
+    ap.add_argument("--T", type=int, default=512)
+    ap.add_argument("--K", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--ref-K", type=int, default=16)
+    a = ap.parse_args()
+    import sys
+    r = study(a.L, a.train, a.cal, a.test, a.T, a.K, a.steps, ref_K=a.ref_K,
+              log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    print(json.dumps(r), flush=True)

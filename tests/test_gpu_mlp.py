@@ -1,0 +1,107 @@
+"""BF16-MFMA network kernels against the CPU oracle, and the surrogate
+pipeline end to end at small size."""
+import numpy as np
+import pytest
+
+from oracle import mlp as om
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_bits(x):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = a.view(np.uint32).astype(np.uint64)
+    return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)
+
+
+@pytest.mark.parametrize("M,N,K", [(64, 64, 8), (37, 70, 13), (3, 256, 512), (512, 1024, 256)])
+def test_gemm_bf16_vs_numpy(gpu, M, N, K):
+    import ctypes
+    from tbplas_amd import _lib
+    rng = np.random.default_rng(M * 7 + N)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    a, b = _bf16_bits(A), _bf16_bits(B)
+    torch = gpu
+    ad = torch.from_numpy(a.view(np.int16)).cuda()
+    bd = torch.from_numpy(b.view(np.int16)).cuda()
+    c = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    h = _lib.default_handle(0)
+    _lib.check(_lib.lib().tbg_gemm_bf16(h.ptr, M, N, K, _lib.ptr(ad), _lib.ptr(bd), _lib.ptr(c)))
+    ref = om.bf16_round(A) @ om.bf16_round(B)
+    got = c.cpu().numpy()
+    assert np.max(np.abs(got - ref)) < 1e-5 * max(1.0, np.max(np.abs(ref))) * np.sqrt(K)
+
+
+@pytest.mark.parametrize("widths,B", [((3, 32, 48, 20), 16), ((3, 256, 256, 1024), 24)])
+def test_forward_and_gradients_vs_oracle(gpu, widths, B):
+    from tbplas_amd.surrogate import MLP
+    rng = np.random.default_rng(1)
+    net = MLP(widths, seed=3)
+    p = net.get_params().astype(np.float64)
+    x = rng.random((B, widths[0]))
+    t = rng.normal(size=(B, widths[-1]))
+    loss_ref, g_ref, y_ref = om.forward_backward(p, widths, x, t, bf16=True)
+    y = net.forward(x)
+    assert np.max(np.abs(y - y_ref)) < 2e-2 * max(1.0, np.max(np.abs(y_ref)))
+    g, loss = net.gradients(x, t)
+    assert abs(loss - loss_ref) < 1e-3 * max(1.0, loss_ref)
+    rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    assert rel < 2e-2
+    # exact FP64 gradient: BF16 operands leave only a small relative error
+    _, g64, _ = om.forward_backward(p, widths, x, t, bf16=False)
+    assert np.linalg.norm(g - g64) / np.linalg.norm(g64) < 5e-2
+    net.close()
+
+
+def test_adam_step_vs_numpy(gpu):
+    from tbplas_amd.surrogate import MLP
+    widths = (3, 16, 8)
+    rng = np.random.default_rng(2)
+    net = MLP(widths, seed=4)
+    p0 = net.get_params().astype(np.float64)
+    x = rng.random((10, 3))
+    t = rng.normal(size=(10, 8))
+    g, _ = net.gradients(x, t)
+    lr, b1, b2, eps = 1e-2, 0.9, 0.999, 1e-8
+    net.train(x, t, 1, lr=lr, betas=(b1, b2), eps=eps)
+    m = (1 - b1) * g
+    v = (1 - b2) * g.astype(np.float64) ** 2
+    p1 = p0 - lr * (m / (1 - b1)) / (np.sqrt(v / (1 - b2)) + eps)
+    assert np.max(np.abs(net.get_params() - p1)) < 1e-6
+    net.close()
+
+
+def test_training_fits_smooth_targets(gpu):
+    from tbplas_amd.surrogate import MLP
+    rng = np.random.default_rng(5)
+    x = rng.random((128, 3))
+    grid = np.linspace(-1, 1, 64)
+    t = np.exp(-((grid[None, :] - (x[:, :1] - 0.5)) ** 2) / (0.1 + 0.2 * x[:, 1:2]))
+    net = MLP((3, 64, 64, 64), seed=6)
+    hist = net.train(x, t, 1500, lr=3e-3)
+    assert hist[-1] < 0.05 * hist[0]
+    net.close()
+
+
+def test_surrogate_pipeline_small(gpu):
+    """Tiny version of the 512x512 study: TBPM data -> training -> held-out
+    L1 error -> gate with TBPM fallback."""
+    from tbplas_amd import surrogate as sg
+    L, T, K = 48, 128, 2
+    ptr = sg.sample_params(40, seed=11)
+    pte = sg.sample_params(6, seed=12)
+    dtr, ctr, etr = sg.tbpm_dos(ptr, L, T=T, K=K, seed=100, return_corr=True)
+    dte = sg.tbpm_dos(pte, L, T=T, K=K, seed=200)
+    assert np.allclose(np.sum(dtr, axis=1) * sg.DOS_STEP, 1.0)
+    s = sg.DOSSurrogate((3, 64, 64, 1024), seed=1)
+    hist = s.fit(ptr, dtr, steps=1500, lr=2e-3)
+    assert hist[-1] < hist[0]
+    err = sg.l1_error(s.predict(pte), dte)
+    assert np.mean(err) < 0.35
+    _, cs, es = sg.tbpm_dos(ptr[:20], L, T=s.gate_steps, K=K, seed=300, return_corr=True)
+    s.calibrate(ptr[:20], dtr[:20], cs, es, tol=0.1)
+    out, fb, tm = s.predict_or_fallback(pte, L, K=K, T_full=T)
+    assert out.shape == dte.shape and fb.dtype == bool
+    final = sg.l1_error(out, dte)
+    assert np.all(final[fb] < 0.2)
