@@ -12,7 +12,8 @@ A "step" is one time step of all 8 states; ``value`` is step*vectors/s over
 the timed steps with the states resident in HBM.  Printed as one JSON line,
 with
   * ``roofline``: algorithmic bytes per Chebyshev term
-      nnz*(s_val + 4) + 4*(n + 1) + K*5*n*s_vec   (s = 8 B for complex64)
+      nnz*(s_val + 4) + 4*(n + 1) + K*5*n*s_vec   (s_vec = 8 B for complex64;
+      s_val = 4 B: graphene's hoppings are real and stored as float)
     divided by the average time of one term over the timed region (HIP
     events on the library's stream), against 8 TB/s;
   * ``fp64_k1``: the plain FP64 single-state path on the same sample (the
@@ -57,6 +58,9 @@ def parse():
                    help="use the default launch shape instead of the autotuner")
     p.add_argument("--copy-probe", action="store_true",
                    help="time a plain device copy of known size (counter calibration)")
+    p.add_argument("--workload", choices=("dos", "dc", "batch"), default="dos",
+                   help="dos: the metric's configuration (default); dc: disordered "
+                        "field-threaded DC conductivity; batch: 32 uneven instances")
     p.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
 
@@ -157,10 +161,13 @@ def cpu_baseline(args):
 # ---------------------------------------------------------------------------
 # GPU legs
 # ---------------------------------------------------------------------------
-def term_bytes(n, nnz, K, s):
+def term_bytes(n, nnz, K, s, real=False):
     """Algorithmic bytes of one Chebyshev term: H (values + int32 columns +
-    row pointers) and five vector streams per state."""
-    return nnz * (s + 4) + 4 * (n + 1) + K * 5 * n * s
+    row pointers) and five vector streams per state.  s = bytes of one
+    complex vector element; H values take s bytes, or s/2 when the
+    Hamiltonian is stored with real values (no imaginary parts)."""
+    vs = s // 2 if real else s
+    return nnz * (vs + 4) + 4 * (n + 1) + K * 5 * n * s
 
 
 def gpu_run(H, bes, psi, warmup, steps):
@@ -174,6 +181,112 @@ def gpu_run(H, bes, psi, warmup, steps):
     return ms, corr
 
 
+def dc_workload(args):
+    """Graphene 1024x1024, 1% vacancies, Anderson W = 1 eV (U[-0.5, 0.5]),
+    Peierls phases for B = 20 T (gauge 0), rescale 9; DC correlation at the 5
+    grid energies nearest -1, -0.5, 0, 0.5, 1 eV; 16 states, 1024 steps,
+    complex64 (FP64 quasi-eigenstates and correlations)."""
+    import numpy as np
+    import tbplas_amd as tb
+    from tbplas_amd import tbpm, _lib, tune
+    L, K, T = 1024, 16, args.steps if args.steps != 2048 else 1024
+    s = tb.make_graphene_sample(L, vacancies=tb.random_vacancies(2 * L * L, 0.01, 2024))
+    tb.add_anderson_disorder(s, 1.0, 2025)
+    s.set_magnetic_field(20.0, 0)
+    s.rescale_ham(9.0)
+    ip, ix, hop, dx, dy = s.build_ham_dxy_device()
+    n = s.num_orb
+    en = np.array([0.5 * i * s.energy_range / T - s.energy_range / 2. for i in range(2 * T)])
+    inds = np.array([int(np.argmin(np.abs(en - e))) for e in (-1.0, -0.5, 0.0, 0.5, 1.0)])
+    ts = 2 * math.pi / s.energy_range
+    bes = tbpm.bessel_series(ts * s.rescale, 1e-14)
+    h = _lib.default_handle(0)
+    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32", handle=h)
+    tune.apply(H, 2 * K, bes, not args.no_tune)
+    psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d, c = tbpm.corr_dc(H, bes, psi, T, ts, en, inds, chk_thr=1e-4)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # phase 1 propagates 2K states for T steps, phase 2 2K states for T-1 steps
+    value = 2 * K * (2 * T - 1) / dt
+    out = {"metric": "DC correlation step·vectors/s (disordered field-threaded graphene)",
+           "value": round(value, 2), "unit": "step·vectors/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "fp32", "seconds": round(dt, 3),
+           "config": {"workload": f"graphene {L}x{L}, 1% vacancies, Anderson W=1 eV, B=20 T, "
+                                  f"DC at 5 energies, {K} states, {T} steps",
+                      "n_orbitals": n, "nnz": int(ix.numel()), "energies": en[inds].tolist()},
+           "dc_corr_t1_mean": [float(np.mean(c[:, a, :, 0]).real) for a in range(2)]}
+    print(json.dumps(out), flush=True)
+
+
+def batch_workload(args):
+    """32 independent disordered graphene instances of uneven size (edges from
+    {128, 256, 384, 512, 768, 1024}, vacancies 0-3%, Anderson W 0-2 eV, all
+    seeded), rescaled per instance by the row-sum bound, 4 states each, DOS
+    correlation for 1024 steps in one launch per Chebyshev term, replayed
+    from a hipGraph, launch shape from the autotuner; complex64, with the
+    per-instance error against the complex128 run on the same states."""
+    import numpy as np
+    import tbplas_amd as tb
+    from tbplas_amd import tbpm, _lib, tune
+    rng = np.random.default_rng(4)
+    edges = rng.choice([128, 256, 384, 512, 768, 1024], 32)
+    vac = rng.uniform(0.0, 0.03, 32)
+    Ws = rng.uniform(0.0, 2.0, 32)
+    K, T = 4, args.steps if args.steps != 2048 else 1024
+    h = _lib.default_handle(0)
+    insts = []
+    for i in range(32):
+        L = int(edges[i])
+        s = tb.make_graphene_sample(L, vacancies=tb.random_vacancies(2 * L * L, vac[i], 100 + i))
+        tb.add_anderson_disorder(s, Ws[i], 200 + i)
+        s.rescale_ham()
+        ip, ix, dt = s.build_ham_csr_device()
+        insts.append((s.num_orb, int(ix.numel()), ip, ix, dt,
+                      tb.states.random_states(s.num_orb, tb.states.sample_seeds(300 + i, K))))
+    bes = tbpm.bessel_series(math.pi, 1e-14)
+    res = {}
+    for prec in ("fp32", "fp64"):
+        hams = [tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h)
+                for (n, nnz, ip, ix, dt, _) in insts]
+        shape = tune.apply(hams[max(range(32), key=lambda j: insts[j][0])], K, bes,
+                           not args.no_tune)
+        run = tbpm.DosRun(hams, bes, [x[5] for x in insts], T + 16)
+        run.advance(16)
+        ms = run.advance(T)
+        res[prec] = (ms, run.result(chk_step=0), shape,
+                     all(H.info()["real_values"] for H in hams))
+        run.close()
+        for H in hams:
+            H.close()
+    ms, c32, shape, _ = res["fp32"]
+    c64 = res["fp64"][1]
+    errs = [float(np.max(np.abs(a - b))) for a, b in zip(c32, c64)]
+    n_tot = sum(x[0] for x in insts)
+    nnz_tot = sum(x[1] for x in insts)
+    t_step = ms / 1e3 / T
+    nterm = len(bes) - 1
+    bt = (term_bytes(n_tot, nnz_tot, K, 8, res["fp32"][3]) - 4 * (n_tot + 1)
+          + sum(4 * (x[0] + 1) for x in insts))
+    ach = bt / (t_step / nterm) / 1e9
+    out = {"metric": "uneven batch DOS step·vectors/s (32 instances, one launch per term)",
+           "value": round(32 * K / t_step, 2), "unit": "instance-step·vectors/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "fp32", "ms_per_step": round(t_step * 1e3, 4),
+           "config": {"workload": f"32 disordered graphene instances, edges {sorted(set(edges.tolist()))}, "
+                                  f"{K} states each, {T} steps", "n_orbitals_total": n_tot,
+                      "nnz_total": nnz_tot, "launch_shape": {"blocks_per_cu": shape[0],
+                                                             "lds_stage": shape[1]}},
+           "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)},
+           "fp64_ms_per_step": round(res["fp64"][0] / T, 4),
+           "ct_err_fp32_vs_fp64_per_instance_max": max(errs),
+           "ct_err_fp32_vs_fp64_per_instance_median": float(np.median(errs))}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.cpu_child:
@@ -181,6 +294,12 @@ def main():
         return 0
     if args.copy_probe:
         copy_probe()
+        return 0
+    if args.workload != "dos":
+        if args.gpus != 1:
+            print("bench.py drives one GPU", file=sys.stderr)
+            return 2
+        (dc_workload if args.workload == "dc" else batch_workload)(args)
         return 0
     if args.gpus != 1:
         print(f"bench.py drives one GPU; --gpus {args.gpus} is not supported by this build "
@@ -207,13 +326,14 @@ def main():
 
     from tbplas_amd import tune
     H32 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp32", handle=h)
+    real = H32.info()["real_values"]
     shape32 = tune.apply(H32, K, bes, not args.no_tune)
     ms32, c32 = gpu_run(H32, bes, psi, args.warmup, args.steps)
     H32.close()
     t_step = ms32 / 1e3 / args.steps
     value = K / t_step
     nterm = len(bes) - 1
-    bt = term_bytes(n, nnz, K, 8)
+    bt = term_bytes(n, nnz, K, 8, real)
     achieved = bt / (t_step / nterm) / 1e9
     log(f"fp32 K={K}: {ms32:.1f} ms for {args.steps} steps -> {value:.1f} step·vectors/s, "
         f"{achieved:.0f} GB/s algorithmic")
@@ -244,9 +364,10 @@ def main():
         H64 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp64", handle=h)
         shape64 = tune.apply(H64, 1, bes, not args.no_tune)
         ms64, c64 = gpu_run(H64, bes, psi[:1], args.warmup, steps64)
+        real64 = H64.info()["real_values"]
         H64.close()
         t64 = ms64 / 1e3 / steps64
-        b64 = term_bytes(n, nnz, 1, 16)
+        b64 = term_bytes(n, nnz, 1, 16, real64)
         tmax = min(args.warmup + args.steps, args.warmup + steps64)
         err = float(np.max(np.abs(c32[0, :tmax + 1] - c64[0, :tmax + 1])))
         out["fp64_k1"] = {"value": round(1.0 / t64, 3), "unit": "step·vectors/s",

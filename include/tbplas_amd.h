@@ -49,6 +49,13 @@ int tbg_set_graph(tbg_handle *h, int enable);
  * unit (1..64, default 16) and LDS staging of each slice's own rows (default
  * 1).  No reference counterpart; set by the tuner (tbplas_amd/tune.py). */
 int tbg_set_tuning(tbg_handle *h, int blocks_per_cu, int stage);
+/* Value layout of the Hamiltonians created on this handle from now on: with
+ * enable = 1 (the default) a Hamiltonian whose hopping terms all have an
+ * exactly zero imaginary part stores real values, which halves the value
+ * bytes the propagation reads and gives the same sums.  No reference
+ * counterpart (the reference always stores complex128,
+ * tbplas/fortran/csr.f90:9-14). */
+int tbg_set_real_layout(tbg_handle *h, int enable);
 
 /* ---- CSR assembly --------------------------------------------------------
  * Replaces the Cython build_ham_dxy_fast (tbplas/cython/sample.pyx:148-257,
@@ -116,6 +123,8 @@ int tbg_ham_destroy(tbg_ham *H);
 /* Stored elements (with slice padding) and maximum row width. */
 int tbg_ham_info(const tbg_ham *H, int64_t *n, int64_t *nnz,
                  int64_t *padded, int *max_width);
+/* *real = 1 when H stores real values (see tbg_set_real_layout). */
+int tbg_ham_is_real(const tbg_ham *H, int *real);
 
 /* ---- correlation functions ----------------------------------------------
  * psi0: K initial states, complex128 (K, n) row-major (state k contiguous),

@@ -29,6 +29,8 @@ _SIGS = {
     "tbg_destroy": [_P],
     "tbg_set_graph": [_P, _I],
     "tbg_set_tuning": [_P, _I, _I],
+    "tbg_set_real_layout": [_P, _I],
+    "tbg_ham_is_real": [_P, ctypes.POINTER(_I)],
     "tbg_ham_dxy_count": [_P, _I64, _I64, _P, _P, _P, _D, _P, ctypes.POINTER(_I64)],
     "tbg_ham_dxy_fill": [_P, _I64, _I64, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P, _P],
     "tbg_ham_csr_count": [_P, _I64, _I64, _P, _P, _P, _P, _P, ctypes.POINTER(_I64)],
@@ -91,6 +93,11 @@ def lib():
     global _lib
     with _lock:
         if _lib is None:
+            # torch's wheel bundles its own HIP runtime (file libamdhip64.so,
+            # soname libamdhip64.so.7).  Load it first so this library binds
+            # to the same runtime; loaded the other way round, torch would map
+            # a second runtime and one of the two sees no device.
+            import torch  # noqa: F401
             if not os.path.exists(LIB_PATH):
                 raise ImportError(
                     f"{LIB_PATH} not found: build the HIP extension first "
@@ -146,6 +153,11 @@ class Handle:
     def set_tuning(self, blocks_per_cu=16, stage=True):
         check(lib().tbg_set_tuning(self._h, int(blocks_per_cu), 1 if stage else 0))
         self.tuning = (int(blocks_per_cu), bool(stage))
+
+    def set_real_layout(self, enable=True):
+        """Store real values for Hamiltonians without imaginary parts
+        (applies to Hamiltonians created afterwards)."""
+        check(lib().tbg_set_real_layout(self._h, 1 if enable else 0))
 
     def close(self):
         if getattr(self, "_h", None):

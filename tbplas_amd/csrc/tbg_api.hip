@@ -250,6 +250,13 @@ int tbg_set_tuning(tbg_handle* h, int blocks_per_cu, int stage) {
     });
 }
 
+int tbg_set_real_layout(tbg_handle* h, int enable) {
+    return guard([&] {
+        check_handle(h);
+        h->real_layout = enable ? 1 : 0;
+    });
+}
+
 int tbg_ham_dxy_count(tbg_handle* h, int64_t n, int64_t m, const double* orb_eng,
                       const int64_t* hop_i, const int64_t* hop_j, double cutoff, int64_t* indptr,
                       int64_t* nnz) {
@@ -378,6 +385,13 @@ int tbg_ham_info(const tbg_ham* H, int64_t* n, int64_t* nnz, int64_t* padded, in
         if (nnz) *nnz = H->nnz;
         if (padded) *padded = H->padded;
         if (max_width) *max_width = H->max_width;
+    });
+}
+
+int tbg_ham_is_real(const tbg_ham* H, int* real) {
+    return guard([&] {
+        TBG_REQUIRE(H != nullptr && real != nullptr, "null argument");
+        *real = H->real_vals ? 1 : 0;
     });
 }
 

@@ -9,6 +9,7 @@ namespace tbg {
 struct Concat {
     DevBuf cols, vals, soff, sw;
     int64_t nslices = 0;
+    int real_vals = 0;
     MatView view() const {
         MatView m;
         m.cols = cols.as<int>();
@@ -16,6 +17,7 @@ struct Concat {
         m.soff = soff.as<int64_t>();
         m.sw = sw.as<int>();
         m.nslices = nslices;
+        m.real_vals = real_vals;
         return m;
     }
 };

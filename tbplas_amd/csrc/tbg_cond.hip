@@ -36,14 +36,32 @@ __global__ void k_add_i32(const int* src, int64_t n, int add, int* dst) {
     if (i < n) dst[i] = src[i] + add;
 }
 
+// Real values widened to complex (imaginary part 0) for a batch that mixes
+// real and complex Hamiltonians.
+template <typename R>
+__global__ void k_real_to_cplx(const R* src, int64_t n, typename CplxT<R>::T* dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        typename CplxT<R>::T o;
+        o.x = src[i];
+        o.y = R(0);
+        dst[i] = o;
+    }
+}
+
 void concat_hams(tbg_handle* h, int n, const tbg_ham* const* hams, Concat& out) {
     int64_t ns = 0, ne = 0;
+    bool all_real = true;
     for (int i = 0; i < n; ++i) {
         ns += hams[i]->nslices;
         ne += hams[i]->padded;
+        all_real = all_real && hams[i]->real_vals;
     }
     TBG_REQUIRE(ns * 64 < (int64_t(1) << 31), "batch too large for 32-bit columns");
-    const size_t vb = hams[0]->precision == TBG_FP32 ? sizeof(float2) : sizeof(double2);
+    const bool f32 = hams[0]->precision == TBG_FP32;
+    const size_t cb = f32 ? sizeof(float2) : sizeof(double2);
+    const size_t vb = all_real ? cb / 2 : cb;
+    out.real_vals = all_real ? 1 : 0;
     out.nslices = ns;
     out.cols.alloc((size_t)std::max<int64_t>(ne, 1) * sizeof(int));
     out.vals.alloc((size_t)std::max<int64_t>(ne, 1) * vb);
@@ -57,8 +75,21 @@ void concat_hams(tbg_handle* h, int n, const tbg_ham* const* hams, Concat& out) 
                                H->cols.as<int>(), H->padded, (int)(s0 * 64),
                                out.cols.as<int>() + e0);
             TBG_HIP(hipGetLastError());
-            TBG_HIP(hipMemcpyAsync(reinterpret_cast<char*>(out.vals.p) + e0 * vb, H->vals.p,
-                                   (size_t)H->padded * vb, hipMemcpyDeviceToDevice, h->stream));
+            char* dst = reinterpret_cast<char*>(out.vals.p) + e0 * vb;
+            if (H->real_vals && !all_real) {
+                if (f32)
+                    hipLaunchKernelGGL(k_real_to_cplx<float>, dim3(nblk(H->padded)), dim3(256), 0,
+                                       h->stream, H->vals.as<float>(), H->padded,
+                                       reinterpret_cast<float2*>(dst));
+                else
+                    hipLaunchKernelGGL(k_real_to_cplx<double>, dim3(nblk(H->padded)), dim3(256), 0,
+                                       h->stream, H->vals.as<double>(), H->padded,
+                                       reinterpret_cast<double2*>(dst));
+                TBG_HIP(hipGetLastError());
+            } else {
+                TBG_HIP(hipMemcpyAsync(dst, H->vals.p, (size_t)H->padded * vb,
+                                       hipMemcpyDeviceToDevice, h->stream));
+            }
         }
         hipLaunchKernelGGL(k_add_i64, dim3(nblk(H->nslices + 1)), dim3(256), 0, h->stream,
                            H->slice_off.as<int64_t>(), H->nslices + 1, e0,

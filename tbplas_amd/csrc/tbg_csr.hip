@@ -469,11 +469,12 @@ __global__ void k_slice_width(int64_t n, int64_t nslices, const int64_t* indptr,
     selems[s] = w * 64;
 }
 
-template <typename R>
+// REAL: every imaginary part of H is zero; values are stored as R.
+template <typename R, bool REAL>
 __global__ void k_sell_fill(int64_t n, int64_t nslices, const int64_t* indptr,
                             const int64_t* indices, const double2* hop, const double* dx,
                             const double* dy, double cur_scale, const int64_t* soff,
-                            const int* sw, int* cols, typename CplxT<R>::T* vals,
+                            const int* sw, int* cols, void* vals_,
                             typename CplxT<R>::T* jx, typename CplxT<R>::T* jy) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= nslices * 64) return;
@@ -504,12 +505,18 @@ __global__ void k_sell_fill(int64_t n, int64_t nslices, const int64_t* indptr,
             }
         }
         cols[e] = col;
-        vals[e] = v;
+        if constexpr (REAL) reinterpret_cast<R*>(vals_)[e] = v.x;
+        else reinterpret_cast<typename CplxT<R>::T*>(vals_)[e] = v;
         if (jx) {
             jx[e] = x;
             jy[e] = y;
         }
     }
+}
+
+__global__ void k_any_imag(int64_t nnz, const double2* hop, int* flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nnz && hop[i].y != 0.0) atomicOr(flag, 1);
 }
 
 __global__ void k_check_cols(int64_t nnz, const int64_t* indices, int64_t n, int* bad) {
@@ -573,28 +580,45 @@ void build_sell(tbg_handle* h, tbg_ham* H, const int64_t* indptr, const int64_t*
     H->padded = padded;
     H->max_width = 0;
     for (int w : hw) H->max_width = std::max(H->max_width, w);
-    const size_t vb = H->precision == TBG_FP32 ? sizeof(float2) : sizeof(double2);
+    {
+        DevBuf flag(sizeof(int));
+        TBG_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), h->stream));
+        if (nnz > 0) {
+            hipLaunchKernelGGL(k_any_imag, dim3(blocks_for(nnz)), dim3(256), 0, h->stream, nnz,
+                               reinterpret_cast<const double2*>(hop), flag.as<int>());
+            TBG_HIP(hipGetLastError());
+        }
+        int hf = 0;
+        TBG_HIP(hipMemcpyAsync(&hf, flag.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        TBG_HIP(hipStreamSynchronize(h->stream));
+        H->real_vals = h->real_layout && hf == 0;
+    }
+    const bool f32 = H->precision == TBG_FP32;
+    const size_t cb = f32 ? sizeof(float2) : sizeof(double2);
+    const size_t vb = H->real_vals ? cb / 2 : cb;
     H->cols.alloc((size_t)std::max<int64_t>(padded, 1) * sizeof(int));
     H->vals.alloc((size_t)std::max<int64_t>(padded, 1) * vb);
     H->has_current = dx != nullptr && dy != nullptr;
     if (H->has_current) {
-        H->jx.alloc((size_t)std::max<int64_t>(padded, 1) * vb);
-        H->jy.alloc((size_t)std::max<int64_t>(padded, 1) * vb);
+        H->jx.alloc((size_t)std::max<int64_t>(padded, 1) * cb);
+        H->jy.alloc((size_t)std::max<int64_t>(padded, 1) * cb);
     }
     const unsigned nb = blocks_for(ns * 64);
-    if (H->precision == TBG_FP32) {
-        hipLaunchKernelGGL(k_sell_fill<float>, dim3(nb), dim3(256), 0, h->stream, n, ns, indptr,
-                           indices, reinterpret_cast<const double2*>(hop), dx, dy, cur_scale,
-                           H->slice_off.as<int64_t>(), H->slice_w.as<int>(), H->cols.as<int>(),
-                           H->vals.as<float2>(), H->has_current ? H->jx.as<float2>() : nullptr,
-                           H->has_current ? H->jy.as<float2>() : nullptr);
+#define TBG_FILL(R, REAL)                                                                  \
+    hipLaunchKernelGGL((k_sell_fill<R, REAL>), dim3(nb), dim3(256), 0, h->stream, n, ns,     \
+                       indptr, indices, reinterpret_cast<const double2*>(hop), dx, dy,       \
+                       cur_scale, H->slice_off.as<int64_t>(), H->slice_w.as<int>(),          \
+                       H->cols.as<int>(), H->vals.p,                                         \
+                       H->has_current ? H->jx.as<typename CplxT<R>::T>() : nullptr,          \
+                       H->has_current ? H->jy.as<typename CplxT<R>::T>() : nullptr)
+    if (f32) {
+        if (H->real_vals) TBG_FILL(float, true);
+        else TBG_FILL(float, false);
     } else {
-        hipLaunchKernelGGL(k_sell_fill<double>, dim3(nb), dim3(256), 0, h->stream, n, ns, indptr,
-                           indices, reinterpret_cast<const double2*>(hop), dx, dy, cur_scale,
-                           H->slice_off.as<int64_t>(), H->slice_w.as<int>(), H->cols.as<int>(),
-                           H->vals.as<double2>(), H->has_current ? H->jx.as<double2>() : nullptr,
-                           H->has_current ? H->jy.as<double2>() : nullptr);
+        if (H->real_vals) TBG_FILL(double, true);
+        else TBG_FILL(double, false);
     }
+#undef TBG_FILL
     TBG_HIP(hipGetLastError());
     TBG_HIP(hipStreamSynchronize(h->stream));
 }

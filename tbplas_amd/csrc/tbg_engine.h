@@ -16,6 +16,7 @@ struct MatView {
     const int64_t* soff = nullptr;
     const int* sw = nullptr;
     int64_t nslices = 0;
+    int real_vals = 0;       // vals holds R instead of complex<R>
 };
 
 inline MatView view_of(const tbg_ham* H) {
@@ -25,6 +26,7 @@ inline MatView view_of(const tbg_ham* H) {
     m.soff = H->slice_off.as<int64_t>();
     m.sw = H->slice_w.as<int>();
     m.nslices = H->nslices;
+    m.real_vals = H->real_vals ? 1 : 0;
     return m;
 }
 
@@ -120,6 +122,7 @@ struct Engine {
         a.cls_mask = cls;
         a.nv = 3 * Kp;
         a.stage = h->stage;
+        a.real_vals = M.real_vals;
         return a;
     }
 

@@ -90,6 +90,7 @@ struct tbg_handle {
     // persistent blocks per CU and LDS staging of the slice's own rows.
     int blocks_per_cu = 16;
     int stage = 1;
+    int real_layout = 1;      // store real values when H has no imaginary part
     tbg::DevBuf scratch;      // reduction partials and small tables
     tbg::DevBuf counter;      // step counter for graph-replayed reductions
 };
@@ -108,7 +109,8 @@ struct tbg_ham {
     tbg::DevBuf slice_off;    // int64 [nslices+1] element offsets
     tbg::DevBuf slice_w;      // int32 [nslices]
     tbg::DevBuf cols;         // int32 [padded]
-    tbg::DevBuf vals;         // complex<R> [padded]
+    tbg::DevBuf vals;         // complex<R> [padded], or R [padded] when real_vals
+    bool real_vals = false;   // every imaginary part of H is exactly zero
     tbg::DevBuf jx, jy;       // complex<R> [padded] current operators (optional)
     bool has_current = false;
 };

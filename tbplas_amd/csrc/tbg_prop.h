@@ -41,6 +41,7 @@ struct TermArgs {
     int nv;                  // values per partial row (3K for dot+norm)
     int* counter;            // incremented once by block 0 when non-null
     int stage;               // stage the slice's own rows of pin in LDS
+    int real_vals;           // vals holds R (all imaginary parts zero)
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).

@@ -79,7 +79,12 @@ __device__ __forceinline__ void vst(void* base, int64_t idx, const Vec<R, V>& o)
 // most of them: the +-1 neighbours) and the v_{m-1} term of the
 // accumulation are then served from LDS, and only the remaining gathers go
 // through L2.  Each wave owns its own 64*K-element LDS area.
-template <typename R, int V, int FL, bool STAGE>
+//
+// RV: the values are stored as R (every imaginary part of H is zero), which
+// halves the value bytes of the sweep; the products then skip the terms
+// that would multiply an exact zero, so the sums are the same as with the
+// complex values.
+template <typename R, int V, int FL, bool STAGE, bool RV>
 __global__ void __launch_bounds__(256) k_term(TermArgs a) {
     using C = typename CplxT<R>::T;
     extern __shared__ __align__(16) unsigned char tbg_smem[];
@@ -120,6 +125,7 @@ __global__ void __launch_bounds__(256) k_term(TermArgs a) {
         const int w = a.sw[sl];
         const int* colp = a.cols + off;
         const C* valp = reinterpret_cast<const C*>(a.vals) + off;
+        const R* rvalp = reinterpret_cast<const R*>(a.vals) + off;
         const int64_t row0 = sl << 6;
         if constexpr (STAGE) {
             for (int p = 0; p < G; ++p) {
@@ -142,7 +148,6 @@ __global__ void __launch_bounds__(256) k_term(TermArgs a) {
             Vec<R, V> s = vzero<R, V>();
             for (int j = 0; j < w; ++j) {
                 const int col = colp[j * 64 + lr];
-                const C hv = valp[j * 64 + lr];
                 Vec<R, V> x;
                 if constexpr (STAGE) {
                     const uint32_t d = (uint32_t)((int64_t)col - row0);
@@ -151,12 +156,22 @@ __global__ void __launch_bounds__(256) k_term(TermArgs a) {
                 } else {
                     x = vld<R, V>(a.pin, (int64_t)col * K + k0);
                 }
+                if constexpr (RV) {
+                    const R hv = rvalp[j * 64 + lr];
 #pragma unroll
-                for (int v = 0; v < V; ++v) {
-                    s.re[v] = fma(hv.x, x.re[v], s.re[v]);
-                    s.re[v] = fma(-hv.y, x.im[v], s.re[v]);
-                    s.im[v] = fma(hv.x, x.im[v], s.im[v]);
-                    s.im[v] = fma(hv.y, x.re[v], s.im[v]);
+                    for (int v = 0; v < V; ++v) {
+                        s.re[v] = fma(hv, x.re[v], s.re[v]);
+                        s.im[v] = fma(hv, x.im[v], s.im[v]);
+                    }
+                } else {
+                    const C hv = valp[j * 64 + lr];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        s.re[v] = fma(hv.x, x.re[v], s.re[v]);
+                        s.re[v] = fma(-hv.y, x.im[v], s.re[v]);
+                        s.im[v] = fma(hv.x, x.im[v], s.im[v]);
+                        s.im[v] = fma(hv.y, x.re[v], s.im[v]);
+                    }
                 }
             }
             Vec<R, V> cur;
@@ -369,13 +384,13 @@ __global__ void k_fill_zero(uint4* p, int64_t n16) {
     X(F_PREV | F_ACC | F_ACCRMW | F_MID | F_NORM)                              \
     X(F_PREV | F_ACC | F_ACCRMW | F_MID | F_NORM | F_DOT)
 
-template <typename R, int V, bool STAGE>
+template <typename R, int V, bool STAGE, bool RV>
 static void launch_term_v(int flags, const TermArgs& a, int nchunks, hipStream_t s) {
     const size_t lds = STAGE ? (size_t)4 * 64 * a.K * sizeof(typename CplxT<R>::T) : 0;
     switch (flags) {
 #define TBG_CASE(F)                                                            \
     case (F):                                                                  \
-        hipLaunchKernelGGL((k_term<R, V, (F), STAGE>), dim3(nchunks), dim3(256), lds, s, a); \
+        hipLaunchKernelGGL((k_term<R, V, (F), STAGE, RV>), dim3(nchunks), dim3(256), lds, s, a); \
         break;
         TBG_TERM_FLAG_LIST(TBG_CASE)
 #undef TBG_CASE
@@ -393,15 +408,25 @@ void launch_term(int flags, const TermArgs& a, int nchunks, hipStream_t s) {
     // Stage the slice in LDS while one wave's area stays within 4 KB
     // (complex64 K <= 8, complex128 K <= 4): 16 KB per block.
     const bool stage = a.stage && 64 * a.K * sizeof(typename CplxT<R>::T) <= 4096;
+    const bool rv = a.real_vals != 0;
+#define TBG_TERM_V(VV)                                                         \
+    do {                                                                       \
+        if (stage) {                                                           \
+            if (rv) launch_term_v<R, VV, true, true>(flags, a, nchunks, s);    \
+            else launch_term_v<R, VV, true, false>(flags, a, nchunks, s);      \
+        } else {                                                               \
+            if (rv) launch_term_v<R, VV, false, true>(flags, a, nchunks, s);   \
+            else launch_term_v<R, VV, false, false>(flags, a, nchunks, s);     \
+        }                                                                      \
+    } while (0)
     if constexpr (std::is_same<R, float>::value) {
         if (V == 2) {
-            if (stage) launch_term_v<R, 2, true>(flags, a, nchunks, s);
-            else launch_term_v<R, 2, false>(flags, a, nchunks, s);
+            TBG_TERM_V(2);
             return;
         }
     }
-    if (stage) launch_term_v<R, 1, true>(flags, a, nchunks, s);
-    else launch_term_v<R, 1, false>(flags, a, nchunks, s);
+    TBG_TERM_V(1);
+#undef TBG_TERM_V
 }
 template void launch_term<float>(int, const TermArgs&, int, hipStream_t);
 template void launch_term<double>(int, const TermArgs&, int, hipStream_t);

@@ -73,8 +73,10 @@ class Hamiltonian:
         w = ctypes.c_int()
         _lib.check(_lib.lib().tbg_ham_info(self._h, ctypes.byref(n), ctypes.byref(nnz),
                                            ctypes.byref(pad), ctypes.byref(w)))
+        real = ctypes.c_int()
+        _lib.check(_lib.lib().tbg_ham_is_real(self._h, ctypes.byref(real)))
         return {"n": n.value, "nnz": nnz.value, "padded": pad.value,
-                "max_width": w.value}
+                "max_width": w.value, "real_values": bool(real.value)}
 
     def close(self):
         if getattr(self, "_h", None):

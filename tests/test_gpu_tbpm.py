@@ -133,6 +133,55 @@ def test_launch_shapes_agree(gpu, prec, K):
         h.set_tuning(8, True)
 
 
+@pytest.mark.parametrize("prec,K", [("fp32", 8), ("fp32", 1), ("fp64", 1), ("fp64", 4)])
+def test_real_layout_equals_complex_layout(gpu, prec, K):
+    """Real-valued storage of a Hamiltonian without imaginary parts skips
+    only products with exact zeros: identical correlations."""
+    s = tb.make_graphene_sample(24, vacancies=tb.random_vacancies(2 * 24 * 24, 0.02, 1))
+    tb.add_anderson_disorder(s, 0.5, 2)
+    s.rescale_ham(9.0)
+    ip, ix, dt = _csr(s)
+    assert not np.any(dt.imag)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, K)
+    h = tb._lib.Handle(0)
+    try:
+        Hr = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec, handle=h)
+        h.set_real_layout(False)
+        Hc = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec, handle=h)
+        assert Hr.info()["real_values"] and not Hc.info()["real_values"]
+        a = tbpm.corr_dos(Hr, bes, psi, 20, chk_thr=1e-4)
+        b = tbpm.corr_dos(Hc, bes, psi, 20, chk_thr=1e-4)
+        np.testing.assert_array_equal(a, b)
+        Hr.close()
+        Hc.close()
+    finally:
+        h.close()
+
+
+def test_batch_mixed_real_and_complex(gpu):
+    """A batch of a real and a complex (Peierls) Hamiltonian widens the real
+    values to complex; each instance still matches the oracle."""
+    hams, psis, refs = [], [], []
+    bes = None
+    for i, L in enumerate((10, 12, 7)):
+        s = tb.make_graphene_sample(L)
+        if i == 1:
+            s.set_magnetic_field(30.0, 0)
+        s.rescale_ham(9.0)
+        ip, ix, dt = _csr(s)
+        bes = tbpm.bessel_series(tau_of(s), 1e-15)
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+        assert H.info()["real_values"] == (i != 1)
+        psi = _states(s.num_orb, 2, seed=200 + i)
+        hams.append(H)
+        psis.append(psi)
+        refs.append(oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 24))
+    out = tbpm.corr_dos_batch(hams, bes, psis, 24)
+    for o, r in zip(out, refs):
+        assert np.max(np.abs(o - r)) < 1e-12
+
+
 def test_dos_run_pieces_equal_one_call(gpu):
     s = graphene_sample(12)
     ip, ix, dt = _csr(s)

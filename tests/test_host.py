@@ -119,3 +119,20 @@ def test_product_has_no_oracle_import():
             if f.endswith((".py", ".hip", ".h", ".cpp")):
                 txt = open(os.path.join(dp, f)).read()
                 assert "import oracle" not in txt and "from oracle" not in txt, f
+
+
+def test_one_hip_runtime_when_library_loads_first():
+    """The host generators may load the library before anything imports
+    torch (Sample.set_magnetic_field does); the process must still map a
+    single HIP runtime, the one torch brings."""
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from tbplas_amd import _lib\n"
+            "_lib.lib()\n"
+            "import torch\n"
+            "maps = open('/proc/self/maps').read().split('\\n')\n"
+            "libs = sorted({l.split()[-1] for l in maps if 'libamdhip64' in l})\n"
+            "print(len(libs), libs)\n" % ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.startswith("1 "), r.stdout
