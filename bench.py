@@ -58,9 +58,12 @@ def parse():
                    help="use the default launch shape instead of the autotuner")
     p.add_argument("--copy-probe", action="store_true",
                    help="time a plain device copy of known size (counter calibration)")
-    p.add_argument("--workload", choices=("dos", "dc", "batch"), default="dos",
+    p.add_argument("--workload", choices=("dos", "dc", "batch", "ac", "latency"),
+                   default="dos",
                    help="dos: the metric's configuration (default); dc: disordered "
-                        "field-threaded DC conductivity; batch: 32 uneven instances")
+                        "field-threaded DC conductivity; batch: 32 uneven instances; "
+                        "ac: AC conductivity on the 8.4M-orbital sample; latency: "
+                        "131,072-orbital single-state steps")
     p.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
 
@@ -222,6 +225,94 @@ def dc_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def ac_workload(args):
+    """Pristine graphene 2048x2048 (rescale 9), AC conductivity correlation
+    of 8 states at 300 K, mu = 0: the Fermi and one-minus-Fermi polynomials
+    (FFT coefficients at 2^15 points, precision 1e-12) applied once, then
+    J_x psi1, J_y psi1 and psi2 of every state (24 vectors) propagated
+    together with tau = pi/2 (16 Bessel terms at 1e-14) and the four current
+    correlations formed every step; complex64 storage, FP64 sums."""
+    import numpy as np
+    import torch
+    import tbplas_amd as tb
+    from tbplas_amd import tbpm, _lib, tune
+    L, K = 2048, 8
+    T = args.steps
+    s = tb.make_graphene_sample(L)
+    s.rescale_ham(9.0)
+    ip, ix, hop, dx, dy = s.build_ham_dxy_device()
+    n = s.num_orb
+    ts = 2 * math.pi / s.energy_range * 0.5
+    bes = tbpm.bessel_series(ts * s.rescale, 1e-14)
+    beta = 1.0 / (8.617333262e-5 * 300.0) * s.rescale
+    cF = tbpm.fermi_cheb_coef(2 ** 15, beta, 0.0, False, 1e-12)
+    cO = tbpm.fermi_cheb_coef(2 ** 15, beta, 0.0, True, 1e-12)
+    h = _lib.default_handle(0)
+    h.set_graph(not args.no_graph)
+    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32",
+                         handle=h)
+    shape = tune.apply(H, 3 * K, bes, not args.no_tune)
+    psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
+    # Fermi operators alone (T = 1 also forms one correlation step)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tbpm.corr_ac(H, bes, cF, cO, psi, 1, chk_thr=1e-4)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    c = tbpm.corr_ac(H, bes, cF, cO, psi, T, chk_thr=1e-4)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    t_prop = (t2 - t1) - (t1 - t0)
+    value = K * (T - 1) / t_prop
+    out = {"metric": "AC correlation state·steps/s (8.4M-orbital graphene, 3 propagated "
+                     "vectors per state)",
+           "value": round(value, 2), "unit": "state·steps/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "fp32",
+           "ms_per_step": round(t_prop / (T - 1) * 1e3, 4),
+           "fermi_setup_s": round(t1 - t0, 3), "total_s": round(t2 - t1, 3),
+           "config": {"workload": f"graphene {L}x{L} AC, {K} states (24 vectors), {T} steps, "
+                                  f"{len(bes)} Bessel terms, Fermi terms {len(cF)}",
+                      "n_orbitals": n, "nnz": int(ix.numel()),
+                      "launch_shape": {"blocks_per_cu": shape[0], "lds_stage": shape[1]}},
+           "corr_xx_t2_mean": complex(np.mean(c[:, 0, 1])).real}
+    H.close()
+    print(json.dumps(out), flush=True)
+
+
+def latency_workload(args):
+    """Pristine graphene 256x256 (131,072 orbitals, rescale 9), one state,
+    1024 DOS steps (21 Bessel terms at 1e-15): the case whose vectors fit in
+    the last-level cache, reported as microseconds per step, complex128 and
+    complex64, hipGraph replay."""
+    import tbplas_amd as tb
+    from tbplas_amd import tbpm, _lib, tune
+    L, T = 256, 1024
+    s = tb.make_graphene_sample(L)
+    s.rescale_ham(9.0)
+    ip, ix, dt = s.build_ham_csr_device()
+    n = s.num_orb
+    bes = tbpm.bessel_series(2 * math.pi / s.energy_range * s.rescale, 1e-15)
+    psi = tb.states.random_states(n, tb.states.sample_seeds(1337, 1))
+    h = _lib.default_handle(0)
+    h.set_graph(not args.no_graph)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        H = tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h)
+        shape = tune.apply(H, 1, bes, not args.no_tune)
+        ms, _ = gpu_run(H, bes, psi, 32, T)
+        H.close()
+        res[prec] = {"us_per_step": round(ms / T * 1e3, 2),
+                     "us_per_term": round(ms / T / (len(bes) - 1) * 1e3, 3),
+                     "launch_shape": {"blocks_per_cu": shape[0], "lds_stage": shape[1]}}
+    out = {"metric": "TBPM microseconds per step, 131,072-orbital graphene, 1 state",
+           "value": res["fp64"]["us_per_step"], "unit": "us/step", "n_gpus": 1,
+           "higher_is_better": False, "dtype": "fp64",
+           "config": {"workload": f"graphene {L}x{L} DOS, 1 state, {T} steps",
+                      "n_orbitals": n, "nnz": int(ix.numel()), "hipgraph": not args.no_graph},
+           "fp64": res["fp64"], "fp32": res["fp32"]}
+    print(json.dumps(out), flush=True)
+
+
 def batch_workload(args):
     """32 independent disordered graphene instances of uneven size (edges from
     {128, 256, 384, 512, 768, 1024}, vacancies 0-3%, Anderson W 0-2 eV, all
@@ -299,7 +390,8 @@ def main():
         if args.gpus != 1:
             print("bench.py drives one GPU", file=sys.stderr)
             return 2
-        (dc_workload if args.workload == "dc" else batch_workload)(args)
+        {"dc": dc_workload, "batch": batch_workload, "ac": ac_workload,
+         "latency": latency_workload}[args.workload](args)
         return 0
     if args.gpus != 1:
         print(f"bench.py drives one GPU; --gpus {args.gpus} is not supported by this build "

@@ -93,7 +93,7 @@ struct Engine {
         M = m;
         Kp = slots_for(k_needed);
         TBG_REQUIRE(Kp <= 64, "too many vectors for one launch (at most 64 slots)");
-        const int V = (std::is_same<R, float>::value && Kp >= 2) ? 2 : 1;
+        const int V = vectors_per_lane<R>(Kp);
         lgG = 0;
         while ((1 << lgG) * V < Kp) ++lgG;
         make_plan(h, plan, inst_slices, chunk_target);

@@ -44,6 +44,20 @@ struct TermArgs {
     int real_vals;           // vals holds R (all imaginary parts zero)
 };
 
+// Vectors one lane of the term kernel carries for Kp slots; the rest of a
+// row is shared by G = Kp/V lanes and a wave covers its slice in G passes.
+// Fewer passes put more gathers in flight per pass, more vectors per lane
+// cost registers.  Chosen from sweeps on 8.4M-orbital graphene (best launch
+// shape each): complex64 K=8 runs 2% faster with V=4 than V=2, K=4 and K=32
+// 1-2% faster with V=2, V=8 loses 10% (occupancy 2-4 waves per SIMD);
+// complex128 uses up to 4.
+template <typename R>
+inline int vectors_per_lane(int Kp) {
+    if (sizeof(R) == 8) return Kp < 4 ? Kp : 4;
+    if (Kp == 8) return 4;
+    return Kp < 2 ? Kp : 2;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
 // Give each XCD one contiguous run of chunks so the neighbour rows a chunk
 // gathers were touched by the same XCD's L2.
