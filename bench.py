@@ -64,6 +64,10 @@ def parse():
                         "field-threaded DC conductivity; batch: 32 uneven instances; "
                         "ac: AC conductivity on the 8.4M-orbital sample; latency: "
                         "131,072-orbital single-state steps")
+    p.add_argument("--order", choices=("caller", "tiles"), default="caller",
+                   help="row order of the device layout: the sample's own, or square "
+                        "tiles of --tile nm (Sample.tile_order)")
+    p.add_argument("--tile", type=float, default=2.0)
     p.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
 
@@ -343,8 +347,7 @@ def batch_workload(args):
     for prec in ("fp32", "fp64"):
         hams = [tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h)
                 for (n, nnz, ip, ix, dt, _) in insts]
-        shape = tune.apply(hams[max(range(32), key=lambda j: insts[j][0])], K, bes,
-                           not args.no_tune)
+        shape = tune.apply(hams, K, bes, not args.no_tune)
         run = tbpm.DosRun(hams, bes, [x[5] for x in insts], T + 16)
         run.advance(16)
         ms = run.advance(T)
@@ -417,7 +420,8 @@ def main():
     h.set_graph(not args.no_graph)
 
     from tbplas_amd import tune
-    H32 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp32", handle=h)
+    order = s.tile_order(args.tile) if args.order == "tiles" else None
+    H32 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp32", handle=h, order=order)
     real = H32.info()["real_values"]
     shape32 = tune.apply(H32, K, bes, not args.no_tune)
     ms32, c32 = gpu_run(H32, bes, psi, args.warmup, args.steps)
@@ -440,6 +444,8 @@ def main():
                                 "complex64 storage/FP64 correlation"),
                    "n_orbitals": n, "nnz": nnz, "states": K, "bessel_terms": len(bes),
                    "terms_per_step": nterm, "hipgraph": not args.no_graph,
+                   "row_order": args.order if args.order == "caller"
+                   else f"tiles of {args.tile} nm",
                    "launch_shape": {"blocks_per_cu": shape32[0], "lds_stage": shape32[1]}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -453,7 +459,8 @@ def main():
 
     if not args.only_fp32:
         steps64 = args.fp64_steps or args.steps
-        H64 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp64", handle=h)
+        H64 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp64", handle=h,
+                               order=order)
         shape64 = tune.apply(H64, 1, bes, not args.no_tune)
         ms64, c64 = gpu_run(H64, bes, psi[:1], args.warmup, steps64)
         real64 = H64.info()["real_values"]

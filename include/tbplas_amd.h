@@ -119,6 +119,18 @@ int tbg_ham_create(tbg_handle *h, int64_t n, const int64_t *indptr,
                    const int64_t *indices, const double *hop,
                    const double *dx, const double *dy, double cur_scale,
                    int precision, tbg_ham **out);
+/* tbg_ham_create with a row order for the device layout: perm (device,
+ * int64, n) lists the caller's rows in the order the layout stores them, so
+ * rows that are close in space can be placed in the same slices (fewer
+ * far gathers).  Inputs and outputs of every entry point stay in the
+ * caller's order; each row keeps its entries' order, so its sums are
+ * unchanged.  No reference counterpart (the reference propagates in the
+ * builder's order, tbplas/fortran/csr.F90:71-101). */
+int tbg_ham_create_ordered(tbg_handle *h, int64_t n, const int64_t *indptr,
+                           const int64_t *indices, const double *hop,
+                           const double *dx, const double *dy,
+                           double cur_scale, int precision,
+                           const int64_t *perm, tbg_ham **out);
 int tbg_ham_destroy(tbg_ham *H);
 /* Stored elements (with slice padding) and maximum row width. */
 int tbg_ham_info(const tbg_ham *H, int64_t *n, int64_t *nnz,

@@ -37,6 +37,7 @@ _SIGS = {
     "tbg_ham_csr_fill": [_P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P],
     "tbg_get_rescale": [_P, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(_D)],
     "tbg_ham_create": [_P, _I64, _P, _P, _P, _P, _P, _D, _I, ctypes.POINTER(_P)],
+    "tbg_ham_create_ordered": [_P, _I64, _P, _P, _P, _P, _P, _D, _I, _P, ctypes.POINTER(_P)],
     "tbg_ham_destroy": [_P],
     "tbg_ham_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                      ctypes.POINTER(_I64), ctypes.POINTER(_I)],

@@ -92,7 +92,7 @@ struct DosRun : tbg_run {
         for (int i = 0; i < ni; ++i) {
             char* base = reinterpret_cast<char*>(psi.p) +
                          (size_t)inst_slices[i] * 64 * Kp * sizeof(typename CplxT<R>::T);
-            launch_pack<R>(psi0[i], K, hams[i]->n, base, Kp, 0, 0, h->stream);
+            launch_pack<R>(psi0[i], K, hams[i]->n, base, Kp, 0, hams[i]->perm_ptr(), h->stream);
         }
         TBG_HIP(hipMemcpyAsync(e.X.p, psi.p, e.vbytes, hipMemcpyDeviceToDevice, h->stream));
         stride = (int64_t)(T + 1) * nv;
@@ -182,9 +182,9 @@ static void step_impl(tbg_handle* h, const tbg_ham* H, const StepPoly& sp, doubl
                       uint64_t bwd) {
     Engine<R> e;
     e.init(h, view_of(H), {0, H->nslices}, K, chunk_target(h, H->nslices));
-    launch_pack<R>(psi, K, H->n, e.X.p, e.Kp, 0, 0, h->stream);
+    launch_pack<R>(psi, K, H->n, e.X.p, e.Kp, 0, H->perm_ptr(), h->stream);
     e.step(sp, bwd, 0, 0, nullptr, nullptr);
-    launch_unpack<R>(e.X.p, e.Kp, 0, H->n, psi, K, h->stream);
+    launch_unpack<R>(e.X.p, e.Kp, 0, H->n, psi, K, H->perm_ptr(), h->stream);
     TBG_HIP(hipStreamSynchronize(h->stream));
 }
 
@@ -355,6 +355,14 @@ int tbg_peierls_host(int64_t m, const int64_t* hop_i, const int64_t* hop_j, doub
 int tbg_ham_create(tbg_handle* h, int64_t n, const int64_t* indptr, const int64_t* indices,
                    const double* hop, const double* dx, const double* dy, double cur_scale,
                    int precision, tbg_ham** out) {
+    return tbg_ham_create_ordered(h, n, indptr, indices, hop, dx, dy, cur_scale, precision,
+                                  nullptr, out);
+}
+
+int tbg_ham_create_ordered(tbg_handle* h, int64_t n, const int64_t* indptr,
+                           const int64_t* indices, const double* hop, const double* dx,
+                           const double* dy, double cur_scale, int precision,
+                           const int64_t* perm, tbg_ham** out) {
     return guard([&] {
         check_handle(h);
         TBG_REQUIRE(out && indptr && indices && hop, "null argument");
@@ -365,7 +373,7 @@ int tbg_ham_create(tbg_handle* h, int64_t n, const int64_t* indptr, const int64_
         H->h = h;
         H->n = n;
         H->precision = precision;
-        build_sell(h, H.get(), indptr, indices, hop, dx, dy, cur_scale);
+        build_sell(h, H.get(), indptr, indices, hop, dx, dy, cur_scale, perm);
         *out = H.release();
     });
 }

@@ -160,15 +160,17 @@ k_accorr(const int* __restrict__ cols, const typename CplxT<R>::T* __restrict__ 
 }
 
 // QE(r, e, s) initialised to psi0 (FP64, tbpm.f90:600-603).
+// psi0 is in the caller's row order; layout row r holds caller row perm[r].
 __global__ void k_qe_init(const double2* __restrict__ psi0, int64_t n, int K, int n_en,
-                          int64_t n_pad, double2* __restrict__ qe) {
+                          int64_t n_pad, const int* __restrict__ perm, double2* __restrict__ qe) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nq = (int64_t)n_en * K;
     if (i >= n_pad * nq) return;
     const int64_t r = i / nq;
     const int q = (int)(i - r * nq);
     const int s = q % K;
-    qe[i] = r < n ? psi0[(int64_t)s * n + r] : make_double2(0.0, 0.0);
+    const int64_t c = r < n ? (perm ? (int64_t)perm[r] : r) : 0;
+    qe[i] = r < n ? psi0[(int64_t)s * n + c] : make_double2(0.0, 0.0);
 }
 
 // After step t (counter = t): DOS correlation <psi0|psi(+t)> and the
@@ -227,7 +229,8 @@ k_qe(const int64_t* __restrict__ chunk, const typename CplxT<R>::T* __restrict__
 // Grid (chunks, q tiles of 256); part[c][q][3].
 __global__ void __launch_bounds__(256)
 k_qe_stats(const int64_t* __restrict__ chunk, const double2* __restrict__ qe, int64_t nq, int K,
-           const double2* __restrict__ psi0, int64_t n, double* __restrict__ part) {
+           const double2* __restrict__ psi0, int64_t n, const int* __restrict__ perm,
+           double* __restrict__ part) {
     __shared__ double sh[256 * 3];
     const int c = remap_chunk(blockIdx.x, gridDim.x);
     const int64_t r0 = chunk[c] * 64, r1 = std::min<int64_t>(chunk[c + 1] * 64, n);
@@ -241,7 +244,7 @@ k_qe_stats(const int64_t* __restrict__ chunk, const double2* __restrict__ qe, in
     if (grp < ng) {
         for (int64_t r = r0 + grp; r < r1; r += ng) {
             const double2 v = qe[r * nq + q];
-            const double2 z = psi0[(int64_t)s * n + r];
+            const double2 z = psi0[(int64_t)s * n + (perm ? (int64_t)perm[r] : r)];
             dr = fma(z.x, v.x, fma(z.y, v.y, dr));
             di = fma(z.x, v.y, fma(-z.y, v.x, di));
             nn = fma(v.x, v.x, fma(v.y, v.y, nn));
@@ -372,11 +375,11 @@ void corr_ac_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
     const hipStream_t st = h->stream;
     DevBuf psi(e.vbytes);
     launch_fill_zero(psi.p, e.vbytes, st);
-    launch_pack<R>(psi0, K, H->n, psi.p, Kp, 0, 0, st);
+    launch_pack<R>(psi0, K, H->n, psi.p, Kp, 0, H->perm_ptr(), st);
     // psi1_x = J_x psi0, psi1_y = J_y psi0, psi2 = psi0 (tbpm.f90:279-281)
     launch_spmv<R>(H, H->jx.p, psi.p, Kp, 0, e.X.p, Kp, 0, K, st);
     launch_spmv<R>(H, H->jy.p, psi.p, Kp, 0, e.X.p, Kp, K, K, st);
-    launch_pack<R>(psi0, K, H->n, e.X.p, Kp, 2 * K, 0, st);
+    launch_pack<R>(psi0, K, H->n, e.X.p, Kp, 2 * K, H->perm_ptr(), st);
     // Fermi operators: class 0 = 1 - F on psi1, class 1 = F on psi2.
     uint64_t cls = 0;
     for (int k = 2 * K; k < 3 * K; ++k) cls |= 1ull << k;
@@ -449,16 +452,16 @@ void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
     const int64_t nq = (int64_t)n_en * K;
     DevBuf psi(e.vbytes);
     launch_fill_zero(psi.p, e.vbytes, st);
-    launch_pack<R>(psi0, K, n, psi.p, Kp, 0, 0, st);
-    launch_pack<R>(psi0, K, n, e.X.p, Kp, 0, 0, st);
-    launch_pack<R>(psi0, K, n, e.X.p, Kp, K, 0, st);
+    launch_pack<R>(psi0, K, n, psi.p, Kp, 0, H->perm_ptr(), st);
+    launch_pack<R>(psi0, K, n, e.X.p, Kp, 0, H->perm_ptr(), st);
+    launch_pack<R>(psi0, K, n, e.X.p, Kp, K, H->perm_ptr(), st);
     uint64_t bwd = 0;
     for (int k = K; k < 2 * K; ++k) bwd |= 1ull << k;
     DevBuf qe((size_t)std::max<int64_t>(n_pad * nq, 1) * sizeof(double2));
     if (nq > 0)
         hipLaunchKernelGGL(k_qe_init, dim3(nblk(n_pad * nq)), dim3(256), 0, st,
                            reinterpret_cast<const double2*>(psi0), n, K, n_en, n_pad,
-                           qe.as<double2>());
+                           H->perm_ptr(), qe.as<double2>());
     // Phase tables P(t, e) = exp(i E t dt), W(t) = Hanning (tbpm.f90:625-629).
     std::vector<double> ptab((size_t)T * std::max(n_en, 1) * 2), wtab(T);
     const double pi = 3.14159265358979323846;
@@ -505,7 +508,8 @@ void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
         for (int pass = 0; pass < 2; ++pass) {
             hipLaunchKernelGGL(k_qe_stats, dim3(e.plan.nchunks, (unsigned)ntiles), dim3(256), 0, st,
                                e.plan.chunk.template as<int64_t>(), qe.as<double2>(), nq, K,
-                               reinterpret_cast<const double2*>(psi0), n, qpart.as<double>());
+                               reinterpret_cast<const double2*>(psi0), n, H->perm_ptr(),
+                               qpart.as<double>());
             TBG_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_sum_rows, dim3(nblk(nq * 3)), dim3(256), 0, st, qpart.as<double>(),
                                e.plan.nchunks, nq * 3, qsum.as<double>());

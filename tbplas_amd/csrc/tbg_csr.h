@@ -26,7 +26,10 @@ int64_t canon_build(tbg_handle* h, int64_t n, int64_t m, const double* orb_eng,
 double get_rescale(tbg_handle* h, int64_t n, int64_t m, const double* orb_eng,
                    const int64_t* hop_i, const int64_t* hop_j, const double* hop_v);
 
+// perm (device, n, optional): new row -> caller row; the layout stores the
+// rows in that order with renumbered columns.
 void build_sell(tbg_handle* h, tbg_ham* H, const int64_t* indptr, const int64_t* indices,
-                const double* hop, const double* dx, const double* dy, double cur_scale);
+                const double* hop, const double* dx, const double* dy, double cur_scale,
+                const int64_t* perm);
 
 }  // namespace tbg

@@ -456,14 +456,35 @@ double get_rescale(tbg_handle* h, int64_t n, int64_t m, const double* orb_eng,
 // ---------------------------------------------------------------------------
 // Sliced-ELL layout for propagation
 // ---------------------------------------------------------------------------
-__global__ void k_slice_width(int64_t n, int64_t nslices, const int64_t* indptr, int* sw,
-                              int64_t* selems) {
+// perm (optional, new row -> caller row) orders the rows of the layout;
+// inv is its inverse.  Without perm both are the identity.
+__global__ void k_invert_perm(int64_t n, const int64_t* perm, int* inv, int* bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p = perm[i];
+    if (p < 0 || p >= n) {
+        atomicOr(bad, 1);
+        return;
+    }
+    if (atomicExch(&inv[p], (int)i) != -1) atomicOr(bad, 2);
+}
+
+__global__ void k_perm_i32(int64_t n, const int64_t* perm, int* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (int)perm[i];
+}
+
+__global__ void k_slice_width(int64_t n, int64_t nslices, const int64_t* indptr, const int* perm,
+                              int* sw, int64_t* selems) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslices) return;
     int64_t w = 0;
     for (int r = 0; r < 64; ++r) {
         const int64_t row = s * 64 + r;
-        if (row < n) w = max(w, indptr[row + 1] - indptr[row]);
+        if (row < n) {
+            const int64_t src = perm ? perm[row] : row;
+            w = max(w, indptr[src + 1] - indptr[src]);
+        }
     }
     sw[s] = (int)w;
     selems[s] = w * 64;
@@ -473,8 +494,8 @@ __global__ void k_slice_width(int64_t n, int64_t nslices, const int64_t* indptr,
 template <typename R, bool REAL>
 __global__ void k_sell_fill(int64_t n, int64_t nslices, const int64_t* indptr,
                             const int64_t* indices, const double2* hop, const double* dx,
-                            const double* dy, double cur_scale, const int64_t* soff,
-                            const int* sw, int* cols, void* vals_,
+                            const double* dy, double cur_scale, const int* perm, const int* inv,
+                            const int64_t* soff, const int* sw, int* cols, void* vals_,
                             typename CplxT<R>::T* jx, typename CplxT<R>::T* jy) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= nslices * 64) return;
@@ -482,8 +503,11 @@ __global__ void k_sell_fill(int64_t n, int64_t nslices, const int64_t* indptr,
     const int lr = (int)(row & 63);
     const int64_t off = soff[s];
     const int w = sw[s];
-    const int64_t p0 = row < n ? indptr[row] : 0;
-    const int64_t len = row < n ? indptr[row + 1] - p0 : 0;
+    // entries keep the caller's order within the row (same sums); only the
+    // row position and the column numbers follow the permutation
+    const int64_t src = row < n ? (perm ? (int64_t)perm[row] : row) : 0;
+    const int64_t p0 = row < n ? indptr[src] : 0;
+    const int64_t len = row < n ? indptr[src + 1] - p0 : 0;
     for (int j = 0; j < w; ++j) {
         const int64_t e = off + (int64_t)j * 64 + lr;
         typename CplxT<R>::T v, x, y;
@@ -491,7 +515,7 @@ __global__ void k_sell_fill(int64_t n, int64_t nslices, const int64_t* indptr,
         int col = (int)(row < n ? row : 0);
         if (j < len) {
             const int64_t p = p0 + j;
-            col = (int)indices[p];
+            col = inv ? inv[indices[p]] : (int)indices[p];
             const double2 hv = hop[p];
             v.x = (R)hv.x;
             v.y = (R)hv.y;
@@ -530,7 +554,8 @@ __global__ void k_check_indptr(int64_t n, const int64_t* indptr, int* bad) {
 }
 
 void build_sell(tbg_handle* h, tbg_ham* H, const int64_t* indptr, const int64_t* indices,
-                const double* hop, const double* dx, const double* dy, double cur_scale) {
+                const double* hop, const double* dx, const double* dy, double cur_scale,
+                const int64_t* perm) {
     const int64_t n = H->n;
     const int64_t ns = (n + 63) / 64;
     H->nslices = ns;
@@ -561,10 +586,32 @@ void build_sell(tbg_handle* h, tbg_ham* H, const int64_t* indptr, const int64_t*
         TBG_HIP(hipStreamSynchronize(h->stream));
         TBG_REQUIRE(hb == 0, "column index out of range [0, n)");
     }
+    DevBuf inv;
+    H->ordered = perm != nullptr;
+    if (H->ordered) {
+        TBG_REQUIRE(n > 0, "a row order needs at least one row");
+        H->perm.alloc((size_t)n * sizeof(int));
+        inv.alloc((size_t)n * sizeof(int));
+        DevBuf bad(sizeof(int));
+        TBG_HIP(hipMemsetAsync(inv.p, 0xff, (size_t)n * sizeof(int), h->stream));
+        TBG_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), h->stream));
+        hipLaunchKernelGGL(k_invert_perm, dim3(blocks_for(n)), dim3(256), 0, h->stream, n, perm,
+                           inv.as<int>(), bad.as<int>());
+        TBG_HIP(hipGetLastError());
+        int hb = 0;
+        TBG_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        TBG_HIP(hipStreamSynchronize(h->stream));
+        TBG_REQUIRE(hb == 0, "row order is not a permutation of [0, n)");
+        hipLaunchKernelGGL(k_perm_i32, dim3(blocks_for(n)), dim3(256), 0, h->stream, n, perm,
+                           H->perm.as<int>());
+        TBG_HIP(hipGetLastError());
+    }
+    const int* pm = H->ordered ? H->perm.as<int>() : nullptr;
+    const int* iv = H->ordered ? inv.as<int>() : nullptr;
     H->slice_w.alloc((size_t)ns * sizeof(int));
     DevBuf selems((size_t)ns * sizeof(int64_t));
     hipLaunchKernelGGL(k_slice_width, dim3(blocks_for(ns)), dim3(256), 0, h->stream, n, ns, indptr,
-                       H->slice_w.as<int>(), selems.as<int64_t>());
+                       pm, H->slice_w.as<int>(), selems.as<int64_t>());
     TBG_HIP(hipGetLastError());
     H->slice_off.alloc((size_t)(ns + 1) * sizeof(int64_t));
     DevBuf tmp;
@@ -607,7 +654,7 @@ void build_sell(tbg_handle* h, tbg_ham* H, const int64_t* indptr, const int64_t*
 #define TBG_FILL(R, REAL)                                                                  \
     hipLaunchKernelGGL((k_sell_fill<R, REAL>), dim3(nb), dim3(256), 0, h->stream, n, ns,     \
                        indptr, indices, reinterpret_cast<const double2*>(hop), dx, dy,       \
-                       cur_scale, H->slice_off.as<int64_t>(), H->slice_w.as<int>(),          \
+                       cur_scale, pm, iv, H->slice_off.as<int64_t>(), H->slice_w.as<int>(),  \
                        H->cols.as<int>(), H->vals.p,                                         \
                        H->has_current ? H->jx.as<typename CplxT<R>::T>() : nullptr,          \
                        H->has_current ? H->jy.as<typename CplxT<R>::T>() : nullptr)

@@ -113,4 +113,9 @@ struct tbg_ham {
     bool real_vals = false;   // every imaginary part of H is exactly zero
     tbg::DevBuf jx, jy;       // complex<R> [padded] current operators (optional)
     bool has_current = false;
+    // Optional row order: layout row i holds the caller's row perm[i]
+    // (int32 [n]); states are packed and unpacked through it.
+    tbg::DevBuf perm;
+    bool ordered = false;
+    const int* perm_ptr() const { return ordered ? perm.as<int>() : nullptr; }
 };

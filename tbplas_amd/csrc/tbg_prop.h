@@ -89,12 +89,13 @@ template <typename R> void launch_term(int flags, const TermArgs& a, int nchunks
 void launch_reduce(const double* part, int nv, const int* inst_chunk, int ninst,
                    double* out, int64_t inst_stride, const int* counter, int toff,
                    hipStream_t s);
-// pack complex128 (K, n) -> interleaved [n_pad][Kp] complex<R> at slot offset
+// pack complex128 (K, n) -> interleaved [n_pad][Kp] complex<R> at slot offset;
+// with perm (layout row -> caller row) layout row r takes caller row perm[r].
 template <typename R> void launch_pack(const double* src, int K, int64_t n,
-                                       void* dst, int Kp, int slot0, int64_t n_pad,
+                                       void* dst, int Kp, int slot0, const int* perm,
                                        hipStream_t s);
 template <typename R> void launch_unpack(const void* src, int Kp, int slot0, int64_t n,
-                                         double* dst, int K, hipStream_t s);
+                                         double* dst, int K, const int* perm, hipStream_t s);
 // Per-vector dot/norm of packed vectors a (conj) and b -> partials (3 per vector).
 template <typename R> void launch_dotnorm(const Plan& p, const void* a, const void* b,
                                           int Kp, int K, double* part, hipStream_t s);

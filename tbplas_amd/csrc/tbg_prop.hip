@@ -383,13 +383,17 @@ k_reduce(const double* part, int nv, const int* inst_chunk, double* out,
     }
 }
 
+// Thread i handles layout row r = i % n of vector k = i / n; it reads or
+// writes the caller's row perm[r] (r without a permutation).
 template <typename R>
 __global__ void k_pack(const double2* __restrict__ src, int K, int64_t n,
-                       typename CplxT<R>::T* __restrict__ dst, int Kp, int slot0) {
+                       typename CplxT<R>::T* __restrict__ dst, int Kp, int slot0,
+                       const int* __restrict__ perm) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)K * n) return;
     const int64_t k = i / n, r = i - k * n;
-    const double2 v = src[i];
+    const int64_t c = perm ? (int64_t)perm[r] : r;
+    const double2 v = src[k * n + c];
     typename CplxT<R>::T o;
     o.x = (R)v.x;
     o.y = (R)v.y;
@@ -398,12 +402,13 @@ __global__ void k_pack(const double2* __restrict__ src, int K, int64_t n,
 
 template <typename R>
 __global__ void k_unpack(const typename CplxT<R>::T* __restrict__ src, int Kp, int slot0,
-                         int64_t n, double2* __restrict__ dst, int K) {
+                         int64_t n, double2* __restrict__ dst, int K, const int* __restrict__ perm) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)K * n) return;
     const int64_t k = i / n, r = i - k * n;
+    const int64_t c = perm ? (int64_t)perm[r] : r;
     const typename CplxT<R>::T v = src[r * Kp + slot0 + k];
-    dst[i] = make_double2((double)v.x, (double)v.y);
+    dst[k * n + c] = make_double2((double)v.x, (double)v.y);
 }
 
 // conj(a) . b and |b|^2 per vector over the first K slots, per-chunk partials.
@@ -551,30 +556,29 @@ void launch_reduce(const double* part, int nv, const int* inst_chunk, int ninst,
 
 template <typename R>
 void launch_pack(const double* src, int K, int64_t n, void* dst, int Kp, int slot0,
-                 int64_t n_pad, hipStream_t s) {
-    (void)n_pad;
+                 const int* perm, hipStream_t s) {
     const int64_t tot = (int64_t)K * n;
     if (tot == 0) return;
     hipLaunchKernelGGL(k_pack<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
                        reinterpret_cast<const double2*>(src), K, n,
-                       reinterpret_cast<typename CplxT<R>::T*>(dst), Kp, slot0);
+                       reinterpret_cast<typename CplxT<R>::T*>(dst), Kp, slot0, perm);
     TBG_HIP(hipGetLastError());
 }
-template void launch_pack<float>(const double*, int, int64_t, void*, int, int, int64_t, hipStream_t);
-template void launch_pack<double>(const double*, int, int64_t, void*, int, int, int64_t, hipStream_t);
+template void launch_pack<float>(const double*, int, int64_t, void*, int, int, const int*, hipStream_t);
+template void launch_pack<double>(const double*, int, int64_t, void*, int, int, const int*, hipStream_t);
 
 template <typename R>
 void launch_unpack(const void* src, int Kp, int slot0, int64_t n, double* dst, int K,
-                   hipStream_t s) {
+                   const int* perm, hipStream_t s) {
     const int64_t tot = (int64_t)K * n;
     if (tot == 0) return;
     hipLaunchKernelGGL(k_unpack<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
                        reinterpret_cast<const typename CplxT<R>::T*>(src), Kp, slot0, n,
-                       reinterpret_cast<double2*>(dst), K);
+                       reinterpret_cast<double2*>(dst), K, perm);
     TBG_HIP(hipGetLastError());
 }
-template void launch_unpack<float>(const void*, int, int, int64_t, double*, int, hipStream_t);
-template void launch_unpack<double>(const void*, int, int, int64_t, double*, int, hipStream_t);
+template void launch_unpack<float>(const void*, int, int, int64_t, double*, int, const int*, hipStream_t);
+template void launch_unpack<double>(const void*, int, int, int64_t, double*, int, const int*, hipStream_t);
 
 template <typename R>
 void launch_dotnorm(const Plan& p, const void* a, const void* b, int Kp, int K, double* part,

@@ -172,6 +172,20 @@ class Sample:
         if self.hop_i is None:
             raise ValueError("hopping arrays not set")
 
+    def tile_order(self, tile=4.0):
+        """Row order for the device layout (``Hamiltonian(order=...)``):
+        orbitals binned into square tiles of ``tile`` nm in the x-y plane,
+        tiles row by row, the original order inside a tile.  Neighbours then
+        mostly share a 64-row slice or sit in the next tile, instead of a
+        whole lattice row away."""
+        self.init_orb_pos()
+        xy = self.orb_pos[:, :2]
+        lo = xy.min(axis=0)
+        b = np.floor((xy - lo) / float(tile)).astype(np.int64)
+        nx = int(b[:, 0].max()) + 1
+        key = b[:, 1] * nx + b[:, 0]
+        return np.argsort(key, kind="stable").astype(np.int64)
+
     def save_array(self, data_dir="sample"):
         """Same files as the reference (sample.py:220-240)."""
         os.makedirs(data_dir, exist_ok=True)

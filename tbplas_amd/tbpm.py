@@ -29,11 +29,14 @@ class Hamiltonian:
 
     indptr/indices/hop may be numpy arrays or device tensors (int64, int64,
     complex128); dx/dy (float64) enable the current operators
-    J = (0, cur_scale) * hop * d (funcs.f90:12-33).
+    J = (0, cur_scale) * hop * d (funcs.f90:12-33).  ``order`` (optional,
+    int64 (n,)) lists the rows in the order the device layout stores them,
+    e.g. ``Sample.tile_order()``; results are in the caller's row order
+    either way.
     """
 
     def __init__(self, n, indptr, indices, hop, dx=None, dy=None, cur_scale=1.0,
-                 precision="fp64", handle=None, device=0):
+                 precision="fp64", handle=None, device=0, order=None):
         torch = _lib.torch_cuda()
         self.handle = handle or _lib.default_handle(device)
         dv = f"cuda:{self.handle.device}"
@@ -56,12 +59,17 @@ class Hamiltonian:
         self.n = int(n)
         self.precision = precision_code(precision)
         self.has_current = dx is not None
+        od = to_dev(order, torch.int64)
+        if od is not None and od.numel() != self.n:
+            raise ValueError("order must list every row once")
+        self.ordered = od is not None
         out = ctypes.c_void_p()
-        _lib.check(_lib.lib().tbg_ham_create(
+        _lib.check(_lib.lib().tbg_ham_create_ordered(
             self.handle.ptr, self.n, _lib.ptr(ip), _lib.ptr(ix), _lib.ptr(hv),
             _lib.ptr(dxd) if dxd is not None else None,
             _lib.ptr(dyd) if dyd is not None else None,
-            float(cur_scale), self.precision, ctypes.byref(out)))
+            float(cur_scale), self.precision,
+            _lib.ptr(od) if od is not None else None, ctypes.byref(out)))
         self._h = out
 
     @property

@@ -3,8 +3,9 @@
 For a prepared Hamiltonian, a state count and a precision it times a few
 Chebyshev steps (HIP events on the library stream, after warm-up steps) for
 every candidate (persistent blocks per CU, LDS staging of the slice rows)
-and keeps the fastest.  Results are cached per (device name, rows, stored
-elements, states, precision) in memory and in a JSON file
+and keeps the fastest.  Results are cached per (device name, library
+build, instances, rows, stored elements, widest slice, value layout, states,
+precision) in memory and in a JSON file
 (``$TBPLAS_AMD_TUNE_CACHE``, default ``~/.cache/tbplas_amd/tune.json``).
 """
 import json
@@ -50,18 +51,29 @@ def _save_disk(d):
         pass
 
 
-def _key(H, K):
+def _key(hams, K):
     import torch
-    info = H.info()
-    dev = torch.cuda.get_device_name(H.handle.device)
-    return f"{dev}|n={info['n']}|pad={info['padded']}|w={info['max_width']}|K={K}|p={H.precision}"
+    infos = [H.info() for H in hams]
+    H0 = hams[0]
+    dev = torch.cuda.get_device_name(H0.handle.device)
+    st = os.stat(_lib.LIB_PATH)          # a rebuilt library is tuned afresh
+    lib = f"{st.st_size}.{int(st.st_mtime)}"
+    n = sum(i["n"] for i in infos)
+    pad = sum(i["padded"] for i in infos)
+    w = max(i["max_width"] for i in infos)
+    real = all(i["real_values"] for i in infos)
+    return (f"{dev}|lib={lib}|inst={len(hams)}|n={n}|pad={pad}|w={w}|real={int(real)}"
+            f"|K={K}|p={H0.precision}")
 
 
 def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
-    """Pick (blocks_per_cu, stage) for Hamiltonian H and K states; returns
-    (best, timings) and leaves the handle set to the best shape."""
+    """Pick (blocks_per_cu, stage) for Hamiltonian H (or a list of
+    Hamiltonians run as one batch) and K states each; returns (best,
+    timings) and leaves the handle set to the best shape."""
     from .tbpm import DosRun
-    key = _key(H, K)
+    hams = list(H) if isinstance(H, (list, tuple)) else [H]
+    H = hams[0]
+    key = _key(hams, K)
     with _lock:
         if use_cache:
             if key in _mem:
@@ -75,12 +87,14 @@ def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
                 H.handle.set_tuning(*best)
                 return best, None
         rng = np.random.default_rng(0)
-        psi = np.exp(2j * np.pi * rng.random((K, H.n)))
-        psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+        psis = []
+        for Hi in hams:
+            psi = np.exp(2j * np.pi * rng.random((K, Hi.n)))
+            psis.append(psi / np.linalg.norm(psi, axis=1, keepdims=True))
         timings = {}
         for cand in candidates or CANDIDATES:
             H.handle.set_tuning(*cand)
-            run = DosRun(H, bes, psi, warmup + steps)
+            run = DosRun(hams, bes, psis, warmup + steps)
             run.advance(warmup)
             timings[cand] = run.advance(steps) / steps
             run.close()
@@ -95,8 +109,10 @@ def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
 
 
 def apply(H, K, bes, enabled=True):
-    """Tune when enabled, otherwise use the default shape."""
+    """Tune when enabled, otherwise use the default shape (H: one
+    Hamiltonian or the list of a batch)."""
     if enabled:
         return autotune(H, K, bes)[0]
-    H.handle.set_tuning(*DEFAULT)
+    h = (H[0] if isinstance(H, (list, tuple)) else H).handle
+    h.set_tuning(*DEFAULT)
     return DEFAULT

@@ -159,6 +159,56 @@ def test_real_layout_equals_complex_layout(gpu, prec, K):
         h.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_tile_order_equals_caller_order(gpu, prec):
+    """Each row keeps its entries' order, so the states are the same; only
+    the FP64 correlation sums run over the rows in another order."""
+    s = tb.make_graphene_sample(40, vacancies=tb.random_vacancies(2 * 40 * 40, 0.02, 4))
+    tb.add_anderson_disorder(s, 0.7, 5)
+    s.rescale_ham(9.0)
+    ip, ix, dt = _csr(s)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, 4)
+    a = tbpm.corr_dos(tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec), bes, psi, 40)
+    for order in (s.tile_order(1.0), _random_order(s.num_orb, 3)):
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec, order=order)
+        b = tbpm.corr_dos(H, bes, psi, 40)
+        assert np.max(np.abs(a - b)) < 1e-13
+        H.close()
+
+
+def test_order_must_be_a_permutation(gpu):
+    s = graphene_sample(4)
+    ip, ix, dt = _csr(s)
+    bad = np.arange(s.num_orb, dtype=np.int64)
+    bad[3] = 2
+    with pytest.raises(tb._lib.TBGError, match="permutation"):
+        tbpm.Hamiltonian(s.num_orb, ip, ix, dt, order=bad)
+    bad[3] = s.num_orb
+    with pytest.raises(tb._lib.TBGError, match="permutation"):
+        tbpm.Hamiltonian(s.num_orb, ip, ix, dt, order=bad)
+
+
+def test_batch_ordered_and_plain(gpu):
+    hams, psis, refs = [], [], []
+    bes = None
+    for i, L in enumerate((9, 14, 6)):
+        s = tb.make_graphene_sample(L)
+        tb.add_anderson_disorder(s, 0.4, 30 + i)
+        s.rescale_ham(9.0)
+        ip, ix, dt = _csr(s)
+        bes = tbpm.bessel_series(tau_of(s), 1e-15)
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64",
+                             order=s.tile_order(1.0) if i != 1 else None)
+        psi = _states(s.num_orb, 2, seed=300 + i)
+        hams.append(H)
+        psis.append(psi)
+        refs.append(oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 24))
+    out = tbpm.corr_dos_batch(hams, bes, psis, 24)
+    for o, r in zip(out, refs):
+        assert np.max(np.abs(o - r)) < 1e-12
+
+
 def test_batch_mixed_real_and_complex(gpu):
     """A batch of a real and a complex (Peierls) Hamiltonian widens the real
     values to complex; each instance still matches the oracle."""
@@ -196,7 +246,8 @@ def test_dos_run_pieces_equal_one_call(gpu):
     run.close()
 
 
-def test_cheb_step_and_poly(gpu):
+@pytest.mark.parametrize("ordered", [False, True])
+def test_cheb_step_and_poly(gpu, ordered):
     s = tb.make_graphene_sample(10)
     tb.add_anderson_disorder(s, 0.5, 9)
     s.set_magnetic_field(15.0, 1)
@@ -204,7 +255,8 @@ def test_cheb_step_and_poly(gpu):
     ip, ix, dt = _csr(s)
     bes = tbpm.bessel_series(tau_of(s), 1e-15)
     psi = _states(s.num_orb, 3)
-    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64",
+                         order=_random_order(s.num_orb, 9) if ordered else None)
     for fwd in (True, False):
         g = tbpm.cheb_step(H, bes, psi, fwd)
         for k in range(3):
@@ -299,11 +351,19 @@ def test_batch_equals_individual(gpu):
         assert np.max(np.abs(o - r)) < 1e-12
 
 
-def test_ac_reference_fixture(gpu):
+def _random_order(n, seed):
+    return np.random.default_rng(seed).permutation(n).astype(np.int64)
+
+
+@pytest.mark.parametrize("ordered", [False, True])
+def test_ac_reference_fixture(gpu, ordered):
+    """ordered: the device layout stores the rows in a random order (every
+    gather leaves its slice); results stay in the caller's order."""
     g = golden("ref_ac.npz")
     n = len(g["indptr"]) - 1
     H = tbpm.Hamiltonian(n, g["indptr"], g["indices"], g["hop"], g["dx"], g["dy"],
-                         cur_scale=float(g["rescale"]), precision="fp64")
+                         cur_scale=float(g["rescale"]), precision="fp64",
+                         order=_random_order(n, 7) if ordered else None)
     cF = tbpm.fermi_cheb_coef(2 ** 15, float(g["beta_re"]), float(g["mu_re"]), False, 1e-12)
     cO = tbpm.fermi_cheb_coef(2 ** 15, float(g["beta_re"]), float(g["mu_re"]), True, 1e-12)
     c = tbpm.corr_ac(H, g["bes"], cF, cO, g["psi0"], int(g["T"]))
@@ -348,11 +408,13 @@ def test_fermi_coef_product_vs_reference_fixture(gpu):
     assert np.max(np.abs(c - g["coef_F_cold"])) < 1e-14
 
 
-def test_dc_reference_fixture(gpu):
+@pytest.mark.parametrize("ordered", [False, True])
+def test_dc_reference_fixture(gpu, ordered):
     g = golden("ref_dc.npz")
     n = len(g["indptr"]) - 1
     H = tbpm.Hamiltonian(n, g["indptr"], g["indices"], g["hop"], g["dx"], g["dy"],
-                         cur_scale=float(g["rescale"]), precision="fp64")
+                         cur_scale=float(g["rescale"]), precision="fp64",
+                         order=_random_order(n, 8) if ordered else None)
     dos, dc = tbpm.corr_dc(H, g["bes"], g["psi0"], int(g["T"]), float(g["t_step"]),
                            g["energies"], g["en_inds"])
     S = dos.shape[0]
