@@ -64,10 +64,11 @@ def parse():
                         "field-threaded DC conductivity; batch: 32 uneven instances; "
                         "ac: AC conductivity on the 8.4M-orbital sample; latency: "
                         "131,072-orbital single-state steps")
-    p.add_argument("--order", choices=("caller", "tiles"), default="caller",
+    p.add_argument("--order", choices=("caller", "tiles"), default="tiles",
                    help="row order of the device layout: the sample's own, or square "
-                        "tiles of --tile nm (Sample.tile_order)")
-    p.add_argument("--tile", type=float, default=2.0)
+                        "tiles of --tile nm (Sample.tile_order), which also lets the "
+                        "tuner choose three Chebyshev terms per pass")
+    p.add_argument("--tile", type=float, default=3.5)
     p.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
 
@@ -168,6 +169,18 @@ def cpu_baseline(args):
 # ---------------------------------------------------------------------------
 # GPU legs
 # ---------------------------------------------------------------------------
+def layout_name(args, shape):
+    """Row order and pass mode of a run (matches the traffic profile)."""
+    base = "caller order" if args.order == "caller" else f"tiles of {args.tile} nm"
+    return base + (", three terms per pass" if len(shape) > 2 and shape[2] else
+                   ", one term per pass")
+
+
+def shape_json(shape):
+    return {"blocks_per_cu": shape[0], "lds_stage": bool(shape[1]),
+            "three_terms_per_pass": bool(shape[2]) if len(shape) > 2 else False}
+
+
 def term_bytes(n, nnz, K, s, real=False):
     """Algorithmic bytes of one Chebyshev term: H (values + int32 columns +
     row pointers) and five vector streams per state.  s = bytes of one
@@ -277,7 +290,7 @@ def ac_workload(args):
            "config": {"workload": f"graphene {L}x{L} AC, {K} states (24 vectors), {T} steps, "
                                   f"{len(bes)} Bessel terms, Fermi terms {len(cF)}",
                       "n_orbitals": n, "nnz": int(ix.numel()),
-                      "launch_shape": {"blocks_per_cu": shape[0], "lds_stage": shape[1]}},
+                      "launch_shape": shape_json(shape)},
            "corr_xx_t2_mean": complex(np.mean(c[:, 0, 1])).real}
     H.close()
     print(json.dumps(out), flush=True)
@@ -307,7 +320,7 @@ def latency_workload(args):
         H.close()
         res[prec] = {"us_per_step": round(ms / T * 1e3, 2),
                      "us_per_term": round(ms / T / (len(bes) - 1) * 1e3, 3),
-                     "launch_shape": {"blocks_per_cu": shape[0], "lds_stage": shape[1]}}
+                     "launch_shape": shape_json(shape)}
     out = {"metric": "TBPM microseconds per step, 131,072-orbital graphene, 1 state",
            "value": res["fp64"]["us_per_step"], "unit": "us/step", "n_gpus": 1,
            "higher_is_better": False, "dtype": "fp64",
@@ -371,8 +384,7 @@ def batch_workload(args):
            "higher_is_better": True, "dtype": "fp32", "ms_per_step": round(t_step * 1e3, 4),
            "config": {"workload": f"32 disordered graphene instances, edges {sorted(set(edges.tolist()))}, "
                                   f"{K} states each, {T} steps", "n_orbitals_total": n_tot,
-                      "nnz_total": nnz_tot, "launch_shape": {"blocks_per_cu": shape[0],
-                                                             "lds_stage": shape[1]}},
+                      "nnz_total": nnz_tot, "launch_shape": shape_json(shape)},
            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)},
            "fp64_ms_per_step": round(res["fp64"][0] / T, 4),
@@ -420,8 +432,10 @@ def main():
     h.set_graph(not args.no_graph)
 
     from tbplas_amd import tune
-    order = s.tile_order(args.tile) if args.order == "tiles" else None
-    H32 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp32", handle=h, order=order)
+    order, tiles = (s.tile_order(args.tile, return_tiles=True) if args.order == "tiles"
+                    else (None, None))
+    H32 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp32", handle=h, order=order,
+                           tiles=tiles)
     real = H32.info()["real_values"]
     shape32 = tune.apply(H32, K, bes, not args.no_tune)
     ms32, c32 = gpu_run(H32, bes, psi, args.warmup, args.steps)
@@ -444,23 +458,31 @@ def main():
                                 "complex64 storage/FP64 correlation"),
                    "n_orbitals": n, "nnz": nnz, "states": K, "bessel_terms": len(bes),
                    "terms_per_step": nterm, "hipgraph": not args.no_graph,
-                   "row_order": args.order if args.order == "caller"
-                   else f"tiles of {args.tile} nm",
-                   "launch_shape": {"blocks_per_cu": shape32[0], "lds_stage": shape32[1]}},
+                   "layout": layout_name(args, shape32),
+                   "launch_shape": shape_json(shape32)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_term": bt,
                      "term_avg_us": round(t_step / nterm * 1e6, 2)},
     }
+    layout = layout_name(args, shape32)
+    if shape32[2]:
+        out["roofline"]["note"] = (
+            "achieved = the one-term-per-pass algorithmic bytes per term over the "
+            "measured time per term; three terms per pass read the state vectors "
+            "fewer times, so it can exceed what one term per pass could reach "
+            "(traffic: bytes per term actually moved, from the counters)")
     traffic_file = os.path.join(HERE, "profiles", "traffic_term_fp32_k8.json")
     if os.path.exists(traffic_file) and args.L == 2048 and K == 8:
         with open(traffic_file) as f:
-            out["roofline"]["traffic"] = json.load(f).get("bytes_per_term")
+            tr = json.load(f)
+        if tr.get("layout") == layout:
+            out["roofline"]["traffic"] = tr.get("bytes_per_term")
 
     if not args.only_fp32:
         steps64 = args.fp64_steps or args.steps
         H64 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp64", handle=h,
-                               order=order)
+                               order=order, tiles=tiles)
         shape64 = tune.apply(H64, 1, bes, not args.no_tune)
         ms64, c64 = gpu_run(H64, bes, psi[:1], args.warmup, steps64)
         real64 = H64.info()["real_values"]
@@ -472,7 +494,7 @@ def main():
         out["fp64_k1"] = {"value": round(1.0 / t64, 3), "unit": "step·vectors/s",
                           "ms_per_step": round(t64 * 1e3, 4),
                           "roofline_frac": round(b64 / (t64 / nterm) / 1e9 / HBM_PEAK_GBS, 4),
-                          "launch_shape": {"blocks_per_cu": shape64[0], "lds_stage": shape64[1]}}
+                          "launch_shape": shape_json(shape64)}
         out["speedup_vs_fp64_k1"] = round(value * t64, 3)
         out["ct_err_fp32_vs_fp64"] = err
         out["ct_err_steps"] = tmax

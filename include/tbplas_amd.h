@@ -56,6 +56,12 @@ int tbg_set_tuning(tbg_handle *h, int blocks_per_cu, int stage);
  * counterpart (the reference always stores complex128,
  * tbplas/fortran/csr.f90:9-14). */
 int tbg_set_real_layout(tbg_handle *h, int enable);
+/* Three terms per pass for Hamiltonians created with spatial tiles
+ * (tbg_ham_create_ordered); enable = 0 runs them one term per pass (same
+ * states, for comparison).  max_rows > 0 caps the rows one workgroup owns
+ * (tiles are merged up to it; 0 = sized from the LDS).  Default (1, 0).
+ * No reference counterpart. */
+int tbg_set_blocking(tbg_handle *h, int enable, int max_rows);
 
 /* ---- CSR assembly --------------------------------------------------------
  * Replaces the Cython build_ham_dxy_fast (tbplas/cython/sample.pyx:148-257,
@@ -120,23 +126,35 @@ int tbg_ham_create(tbg_handle *h, int64_t n, const int64_t *indptr,
                    const double *dx, const double *dy, double cur_scale,
                    int precision, tbg_ham **out);
 /* tbg_ham_create with a row order for the device layout: perm (device,
- * int64, n) lists the caller's rows in the order the layout stores them, so
- * rows that are close in space can be placed in the same slices (fewer
- * far gathers).  Inputs and outputs of every entry point stay in the
- * caller's order; each row keeps its entries' order, so its sums are
- * unchanged.  No reference counterpart (the reference propagates in the
- * builder's order, tbplas/fortran/csr.F90:71-101). */
+ * int64, n, may be NULL) lists the caller's rows in the order the layout
+ * stores them, so rows that are close in space can be placed in the same
+ * slices (fewer far gathers).  tile_ptr (HOST, int64, ntiles + 1, may be
+ * NULL) splits that order into spatial tiles (row offsets, 0 .. n); with
+ * tiles, propagation runs three Chebyshev terms per pass over groups of
+ * tiles plus their two halo rings (intermediate terms kept in LDS), which
+ * reads the state vectors from HBM fewer times.  Inputs and outputs of every
+ * entry point stay in the caller's order; each row keeps its entries'
+ * order, so the states are bit-identical to the plain layout's.  No
+ * reference counterpart (the reference propagates in the builder's order
+ * one term per pass, tbplas/fortran/propagation.f90:42-48). */
 int tbg_ham_create_ordered(tbg_handle *h, int64_t n, const int64_t *indptr,
                            const int64_t *indices, const double *hop,
                            const double *dx, const double *dy,
                            double cur_scale, int precision,
-                           const int64_t *perm, tbg_ham **out);
+                           const int64_t *perm, const int64_t *tile_ptr,
+                           int64_t ntiles, tbg_ham **out);
 int tbg_ham_destroy(tbg_ham *H);
 /* Stored elements (with slice padding) and maximum row width. */
 int tbg_ham_info(const tbg_ham *H, int64_t *n, int64_t *nnz,
                  int64_t *padded, int *max_width);
 /* *real = 1 when H stores real values (see tbg_set_real_layout). */
 int tbg_ham_is_real(const tbg_ham *H, int *real);
+/* Temporal blocking of H for `states` vectors of the given precision
+ * (building its tables if needed): *nblocks workgroup tiles (0 when H runs
+ * one term per pass) and *halo = rows computed per row owned in the first
+ * of the three terms.  No reference counterpart. */
+int tbg_ham_blocking_info(tbg_handle *h, const tbg_ham *H, int states,
+                          int precision, int *nblocks, double *halo);
 
 /* ---- correlation functions ----------------------------------------------
  * psi0: K initial states, complex128 (K, n) row-major (state k contiguous),

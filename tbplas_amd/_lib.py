@@ -37,7 +37,10 @@ _SIGS = {
     "tbg_ham_csr_fill": [_P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P],
     "tbg_get_rescale": [_P, _I64, _I64, _P, _P, _P, _P, ctypes.POINTER(_D)],
     "tbg_ham_create": [_P, _I64, _P, _P, _P, _P, _P, _D, _I, ctypes.POINTER(_P)],
-    "tbg_ham_create_ordered": [_P, _I64, _P, _P, _P, _P, _P, _D, _I, _P, ctypes.POINTER(_P)],
+    "tbg_ham_create_ordered": [_P, _I64, _P, _P, _P, _P, _P, _D, _I, _P, _P, _I64,
+                               ctypes.POINTER(_P)],
+    "tbg_set_blocking": [_P, _I, _I],
+    "tbg_ham_blocking_info": [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)],
     "tbg_ham_destroy": [_P],
     "tbg_ham_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                      ctypes.POINTER(_I64), ctypes.POINTER(_I)],
@@ -154,6 +157,12 @@ class Handle:
     def set_tuning(self, blocks_per_cu=16, stage=True):
         check(lib().tbg_set_tuning(self._h, int(blocks_per_cu), 1 if stage else 0))
         self.tuning = (int(blocks_per_cu), bool(stage))
+
+    def set_blocking(self, enable=True, max_rows=0):
+        """Three Chebyshev terms per pass for Hamiltonians with spatial
+        tiles (same states; off runs one term per pass); max_rows caps the
+        rows one workgroup owns (0: sized from the LDS)."""
+        check(lib().tbg_set_blocking(self._h, 1 if enable else 0, int(max_rows)))
 
     def set_real_layout(self, enable=True):
         """Store real values for Hamiltonians without imaginary parts

@@ -84,7 +84,7 @@ struct DosRun : tbg_run {
             concat_hams(h, ni, hams, cat);  // block-diagonal batch
             mv = cat.view();
         }
-        e.init(h, mv, inst_slices, K, chunk_target(h, mv.nslices));
+        e.init(h, mv, inst_slices, K, chunk_target(h, mv.nslices), ni == 1 ? hams[0] : nullptr);
         const int Kp = e.Kp;
         nv = 3 * Kp;
         psi.alloc(e.vbytes);
@@ -181,7 +181,7 @@ template <typename R>
 static void step_impl(tbg_handle* h, const tbg_ham* H, const StepPoly& sp, double* psi, int K,
                       uint64_t bwd) {
     Engine<R> e;
-    e.init(h, view_of(H), {0, H->nslices}, K, chunk_target(h, H->nslices));
+    e.init(h, view_of(H), {0, H->nslices}, K, chunk_target(h, H->nslices), H);
     launch_pack<R>(psi, K, H->n, e.X.p, e.Kp, 0, H->perm_ptr(), h->stream);
     e.step(sp, bwd, 0, 0, nullptr, nullptr);
     launch_unpack<R>(e.X.p, e.Kp, 0, H->n, psi, K, H->perm_ptr(), h->stream);
@@ -217,6 +217,7 @@ int tbg_create(tbg_handle** out, int device) {
         hipDeviceProp_t prop;
         TBG_HIP(hipGetDeviceProperties(&prop, device));
         h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        h->max_lds = prop.sharedMemPerBlock > 0 ? (int)prop.sharedMemPerBlock : 65536;
         TBG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         *out = h.release();
     });
@@ -247,6 +248,15 @@ int tbg_set_tuning(tbg_handle* h, int blocks_per_cu, int stage) {
         TBG_REQUIRE(blocks_per_cu >= 1 && blocks_per_cu <= 64, "blocks_per_cu must be in [1, 64]");
         h->blocks_per_cu = blocks_per_cu;
         h->stage = stage ? 1 : 0;
+    });
+}
+
+int tbg_set_blocking(tbg_handle* h, int enable, int max_rows) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(max_rows >= 0, "max_rows must be >= 0");
+        h->blocking = enable ? 1 : 0;
+        h->block_rows = max_rows;
     });
 }
 
@@ -356,13 +366,14 @@ int tbg_ham_create(tbg_handle* h, int64_t n, const int64_t* indptr, const int64_
                    const double* hop, const double* dx, const double* dy, double cur_scale,
                    int precision, tbg_ham** out) {
     return tbg_ham_create_ordered(h, n, indptr, indices, hop, dx, dy, cur_scale, precision,
-                                  nullptr, out);
+                                  nullptr, nullptr, 0, out);
 }
 
 int tbg_ham_create_ordered(tbg_handle* h, int64_t n, const int64_t* indptr,
                            const int64_t* indices, const double* hop, const double* dx,
                            const double* dy, double cur_scale, int precision,
-                           const int64_t* perm, tbg_ham** out) {
+                           const int64_t* perm, const int64_t* tile_ptr, int64_t ntiles,
+                           tbg_ham** out) {
     return guard([&] {
         check_handle(h);
         TBG_REQUIRE(out && indptr && indices && hop, "null argument");
@@ -373,6 +384,14 @@ int tbg_ham_create_ordered(tbg_handle* h, int64_t n, const int64_t* indptr,
         H->h = h;
         H->n = n;
         H->precision = precision;
+        if (tile_ptr != nullptr) {
+            TBG_REQUIRE(ntiles >= 1, "tiles need ntiles >= 1");
+            TBG_REQUIRE(tile_ptr[0] == 0 && tile_ptr[ntiles] == n,
+                        "tile_ptr must run from 0 to n");
+            for (int64_t t = 0; t < ntiles; ++t)
+                TBG_REQUIRE(tile_ptr[t + 1] > tile_ptr[t], "tiles must be non-empty and ordered");
+            H->tiles.assign(tile_ptr, tile_ptr + ntiles + 1);
+        }
         build_sell(h, H.get(), indptr, indices, hop, dx, dy, cur_scale, perm);
         *out = H.release();
     });
@@ -400,6 +419,21 @@ int tbg_ham_is_real(const tbg_ham* H, int* real) {
     return guard([&] {
         TBG_REQUIRE(H != nullptr && real != nullptr, "null argument");
         *real = H->real_vals ? 1 : 0;
+    });
+}
+
+int tbg_ham_blocking_info(tbg_handle* h, const tbg_ham* H, int states, int precision,
+                          int* nblocks, double* halo) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(H != nullptr && nblocks != nullptr && halo != nullptr, "null argument");
+        TBG_REQUIRE(states >= 1 && states <= 64, "states must be in [1, 64]");
+        TBG_REQUIRE(precision == TBG_FP64 || precision == TBG_FP32, "unknown precision");
+        const int Kp = slots_for(states);
+        const int cb = precision == TBG_FP32 ? (int)sizeof(float2) : (int)sizeof(double2);
+        auto b = get_blocking(h, H, Kp * cb);
+        *nblocks = b ? b->nblocks : 0;
+        *halo = b ? b->halo : 0.0;
     });
 }
 

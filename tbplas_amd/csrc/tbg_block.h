@@ -1,0 +1,70 @@
+// Temporal blocking: three Chebyshev terms per pass over a spatial tile of
+// rows (tbg_block.hip).
+#pragma once
+
+#include <memory>
+
+#include "tbg_prop.h"
+
+namespace tbg {
+
+// Rows of one block-tile: R = layout rows [r0, r0 + nR) (a run of the
+// caller's spatial tiles), halo ring E1 = neighbours of R outside R, ring
+// E2 = neighbours of E1 outside R and E1.  rows[offA ..] lists R, E1, E2 in
+// that order (A = R+E1+E2, B = R+E1 is a prefix).  Every row of A has w
+// entries (the widest row of A; shorter rows padded with zero values on
+// their own column), stored column-major so a workgroup reads them
+// coalesced: colG[offG + j*nA + l] (layout column), valG[offG + j*nA + l]
+// (value, the layout's element type) and, for the rows of B,
+// colL[offL + j*nB + l] (position of that column in A).
+struct BlockMeta {
+    int64_t offA, offL, offG;
+    int nR, nB, nA, w;
+};
+
+struct Blocking {
+    int row_bytes = 0;       // bytes of one row of states it was sized for
+    int nblocks = 0;
+    int nA_max = 0, nB_max = 0;
+    double halo = 0.0;       // (sum nA) / n: rows computed per row owned, term m
+    DevBuf meta;             // BlockMeta [nblocks]
+    DevBuf rows;             // int32 [sum nA]
+    DevBuf colL;             // int32 [sum w * nB]
+    DevBuf colG;             // int32 [sum w * nA]
+    DevBuf valG;             // value [sum w * nA]
+};
+
+// The blocking of H for rows of row_bytes bytes (built on first use and
+// kept on H), or null when H has no tiles, blocking is off on the handle, or
+// a block-tile would not fit the LDS of one workgroup.
+std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_bytes);
+
+// Arguments of the three-term kernel: from v_{m-2} (p0) and v_{m-1} (p1) it
+// forms v_m, v_{m+1}, v_{m+2} on each block-tile (v_m and v_{m+1} on the
+// halo rings, in LDS), writes v_{m+1} to q1 and v_{m+2} to q2 for the owned
+// rows, and adds c_m v_m + c_{m+1} v_{m+1} + c_{m+2} v_{m+2} to acc (first:
+// acc = c_{m-2} v_{m-2} + c_{m-1} v_{m-1} + ..., no read).
+struct Term3Args {
+    const BlockMeta* meta;
+    const int* rows;
+    const int* colL;
+    const int* colG;
+    const void* valG;
+    const void* p0;
+    const void* p1;
+    void* q1;
+    void* q2;
+    void* acc;
+    double ar[2], ai[2];
+    double beta;
+    double c[5][2];          // c_{m-2} .. c_{m+2} per coefficient class
+    uint64_t bwd_mask, cls_mask;
+    int K, lgG;
+    int nA_max;
+};
+
+template <typename R>
+void launch_term3(bool first, bool real_vals, const Term3Args& a, const Blocking& b,
+                  hipStream_t s);
+
+}  // namespace tbg

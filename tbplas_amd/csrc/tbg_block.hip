@@ -1,0 +1,336 @@
+// Temporal blocking of the Chebyshev recurrence on MI355X.
+//
+// The one-term kernel reads v_{m-1} (gathered) and v_{m-2} and writes v_m
+// for every term, ~3.6 vector streams per term.  Here one workgroup owns a
+// block-tile R of rows (a run of the caller's spatial tiles) and forms three
+// terms in one pass:
+//   phase A  v_m     on A = R + E1 + E2   from v_{m-1} (HBM/L2) and v_{m-2}
+//   phase B  v_{m+1} on B = R + E1        from v_m (LDS) and v_{m-1}
+//   phase C  v_{m+2} on R                 from v_{m+1} (LDS) and v_m (LDS)
+// E1, E2 are the two halo rings of R.  Only v_{m+1} and v_{m+2} of the owned
+// rows are written (the next pass needs exactly those two), and the step
+// output is updated once per three terms.  Per three terms and owned row:
+// reads of v_{m-1} and v_{m-2} on the rings (~1.3 each for compact tiles),
+// two writes, one output read-modify-write: ~2.2 streams per term.  The
+// halo rows are computed redundantly by neighbouring blocks.
+//
+// The entries of a block's rows are copied into per-block tables (columns,
+// values and, for the rows of B, positions in A), column-major, so each
+// phase reads its entries with one coalesced load before its gathers
+// instead of following row -> slice -> entry.  Workgroups are large (8 or 16
+// waves) because the tile's LDS leaves room for about two per CU.
+//
+// Every row sum, recurrence step and output update uses the helpers of
+// tbg_vec.h in the same order as the one-term kernel, so the states and the
+// step output are bit-identical to it (the correlations too: they are formed
+// by the one-term kernel of the step's last term).
+#include <algorithm>
+#include <cstring>
+
+#include "tbg_block.h"
+#include "tbg_vec.h"
+
+namespace tbg {
+
+// ---------------------------------------------------------------------------
+// Tables
+// ---------------------------------------------------------------------------
+// LDS per workgroup the blocking aims at (two workgroups per CU).
+constexpr int kBlockLdsTarget = 72 * 1024;
+
+std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_bytes) {
+    if (!h->blocking || H->tiles.size() < 2) return nullptr;
+    const bool same = H->blk_row_bytes == row_bytes && H->blk_rows_cap == h->block_rows;
+    if (H->blk && same) return H->blk;
+    if (H->blk_failed && same) return nullptr;
+    H->blk.reset();
+    H->blk_failed = false;
+    H->blk_row_bytes = row_bytes;
+    H->blk_rows_cap = h->block_rows;
+
+    const int64_t n = H->n, ns = H->nslices;
+    const size_t vsz = H->vals.bytes / (size_t)std::max<int64_t>(H->padded, 1);
+    std::vector<int> cols((size_t)H->padded), sw((size_t)ns);
+    std::vector<int64_t> soff((size_t)ns + 1);
+    std::vector<unsigned char> vals((size_t)H->padded * vsz);
+    TBG_HIP(hipMemcpyAsync(cols.data(), H->cols.p, cols.size() * sizeof(int),
+                           hipMemcpyDeviceToHost, h->stream));
+    TBG_HIP(hipMemcpyAsync(vals.data(), H->vals.p, vals.size(), hipMemcpyDeviceToHost,
+                           h->stream));
+    TBG_HIP(hipMemcpyAsync(sw.data(), H->slice_w.p, sw.size() * sizeof(int),
+                           hipMemcpyDeviceToHost, h->stream));
+    TBG_HIP(hipMemcpyAsync(soff.data(), H->slice_off.p, soff.size() * sizeof(int64_t),
+                           hipMemcpyDeviceToHost, h->stream));
+    TBG_HIP(hipStreamSynchronize(h->stream));
+    auto width = [&](int r) { return sw[r >> 6]; };
+    auto elem = [&](int r, int j) { return soff[r >> 6] + (int64_t)j * 64 + (r & 63); };
+    auto col = [&](int r, int j) { return cols[elem(r, j)]; };
+
+    // owned rows per block-tile: whole tiles, up to the row target
+    const int lds = std::min(h->max_lds, kBlockLdsTarget);
+    int64_t target = std::max<int64_t>(64, (int64_t)(lds / (2.6 * row_bytes)));
+    if (h->block_rows > 0) target = std::min<int64_t>(target, h->block_rows);
+    std::vector<int64_t> starts;
+    {
+        const std::vector<int64_t>& t = H->tiles;
+        size_t i = 0;
+        while (i + 1 < t.size()) {
+            starts.push_back(t[i]);
+            size_t j = i + 1;
+            while (j + 1 < t.size() && t[j + 1] - t[i] <= target) ++j;
+            i = j;
+        }
+        starts.push_back(n);
+    }
+    const int nb = (int)starts.size() - 1;
+    std::vector<BlockMeta> meta((size_t)nb);
+    std::vector<int> rows, colL, colG;
+    std::vector<unsigned char> valG;
+    rows.reserve((size_t)(n * 1.6));
+    colL.reserve((size_t)(n * 4));
+    colG.reserve((size_t)(n * 5));
+    valG.reserve((size_t)(n * 5) * vsz);
+    std::vector<int> mark((size_t)n, -1), loc((size_t)n, 0);
+    int nA_max = 0, nB_max = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int r0 = (int)starts[b], r1 = (int)starts[b + 1];
+        const size_t base = rows.size();
+        for (int r = r0; r < r1; ++r) {
+            mark[r] = b;
+            loc[r] = (int)(rows.size() - base);
+            rows.push_back(r);
+        }
+        auto ring = [&](size_t from, size_t to) {
+            for (size_t i = from; i < to; ++i) {
+                const int r = rows[i];
+                for (int j = 0, w = width(r); j < w; ++j) {
+                    const int c = col(r, j);
+                    if (mark[c] != b) {
+                        mark[c] = b;
+                        loc[c] = (int)(rows.size() - base);
+                        rows.push_back(c);
+                    }
+                }
+            }
+        };
+        const size_t endR = rows.size();
+        ring(base, endR);                    // E1
+        const size_t endB = rows.size();
+        ring(endR, endB);                    // E2
+        const int nR = r1 - r0, nB = (int)(endB - base), nA = (int)(rows.size() - base);
+        int wA = 0;
+        for (size_t i = base; i < rows.size(); ++i) wA = std::max(wA, width(rows[i]));
+        BlockMeta m;
+        m.offA = (int64_t)base;
+        m.offL = (int64_t)colL.size();
+        m.offG = (int64_t)colG.size();
+        m.nR = nR;
+        m.nB = nB;
+        m.nA = nA;
+        m.w = wA;
+        meta[b] = m;
+        colL.resize(colL.size() + (size_t)wA * nB);
+        colG.resize(colG.size() + (size_t)wA * nA);
+        valG.resize(valG.size() + (size_t)wA * nA * vsz, 0);
+        int* cl = colL.data() + m.offL;
+        int* cg = colG.data() + m.offG;
+        unsigned char* vg = valG.data() + (size_t)m.offG * vsz;
+        for (int l = 0; l < nA; ++l) {
+            const int r = rows[base + l];
+            const int w = width(r);
+            for (int j = 0; j < wA; ++j) {
+                const size_t q = (size_t)j * nA + l;
+                if (j < w) {
+                    cg[q] = col(r, j);
+                    std::memcpy(vg + q * vsz, vals.data() + (size_t)elem(r, j) * vsz, vsz);
+                } else {
+                    cg[q] = r;       // zero value on the row's own column
+                }
+                if (l < nB) cl[(size_t)j * nB + l] = j < w ? loc[col(r, j)] : l;
+            }
+        }
+        nA_max = std::max(nA_max, nA);
+        nB_max = std::max(nB_max, nB);
+    }
+    // every LDS index in range: phase B reads A positions (< nA), phase C
+    // reads B positions (< nB) for the owned rows
+    for (int b = 0; b < nb; ++b) {
+        const BlockMeta& m = meta[b];
+        for (int l = 0; l < m.nB; ++l) {
+            const int lim = l < m.nR ? m.nB : m.nA;
+            for (int j = 0; j < m.w; ++j) {
+                const int v = colL[m.offL + (int64_t)j * m.nB + l];
+                TBG_REQUIRE(v >= 0 && v < lim, "internal: blocking table out of range");
+            }
+        }
+        for (int64_t q = 0; q < (int64_t)m.w * m.nA; ++q) {
+            const int c = colG[m.offG + q];
+            TBG_REQUIRE(c >= 0 && c < n, "internal: blocking table out of range");
+        }
+    }
+    if ((int64_t)(nA_max + nB_max) * row_bytes > h->max_lds) {
+        H->blk_failed = true;
+        return nullptr;
+    }
+    auto B = std::make_shared<Blocking>();
+    B->row_bytes = row_bytes;
+    B->nblocks = nb;
+    B->nA_max = nA_max;
+    B->nB_max = nB_max;
+    B->halo = (double)rows.size() / (double)n;
+    B->meta.alloc(meta.size() * sizeof(BlockMeta));
+    B->rows.alloc(rows.size() * sizeof(int));
+    B->colL.alloc(std::max<size_t>(colL.size(), 1) * sizeof(int));
+    B->colG.alloc(std::max<size_t>(colG.size(), 1) * sizeof(int));
+    B->valG.alloc(std::max<size_t>(valG.size(), 16));
+    TBG_HIP(hipMemcpyAsync(B->meta.p, meta.data(), meta.size() * sizeof(BlockMeta),
+                           hipMemcpyHostToDevice, h->stream));
+    TBG_HIP(hipMemcpyAsync(B->rows.p, rows.data(), rows.size() * sizeof(int),
+                           hipMemcpyHostToDevice, h->stream));
+    if (!colL.empty())
+        TBG_HIP(hipMemcpyAsync(B->colL.p, colL.data(), colL.size() * sizeof(int),
+                               hipMemcpyHostToDevice, h->stream));
+    if (!colG.empty()) {
+        TBG_HIP(hipMemcpyAsync(B->colG.p, colG.data(), colG.size() * sizeof(int),
+                               hipMemcpyHostToDevice, h->stream));
+        TBG_HIP(hipMemcpyAsync(B->valG.p, valG.data(), valG.size(), hipMemcpyHostToDevice,
+                               h->stream));
+    }
+    TBG_HIP(hipStreamSynchronize(h->stream));
+    H->blk = B;
+    return B;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel
+// ---------------------------------------------------------------------------
+// Threads per workgroup: the tile's LDS limits a CU to about two
+// workgroups, so they are large (16 waves for one state per lane, 8
+// otherwise) to keep enough gathers in flight.  G = K/V lanes share a row,
+// NT/G rows per pass of a phase.
+template <int V> constexpr int term3_threads() { return V == 1 ? 1024 : 512; }
+
+template <typename R, int V, bool RV, bool FIRST>
+__global__ void __launch_bounds__(term3_threads<V>()) k_term3(Term3Args a) {
+    constexpr int NT = term3_threads<V>();
+    using C = typename CplxT<R>::T;
+    extern __shared__ __align__(16) unsigned char tbg_smem3[];
+    const int b = remap_chunk(blockIdx.x, gridDim.x);
+    const BlockMeta m = a.meta[b];
+    const int* rows = a.rows + m.offA;
+    const int* colL = a.colL + m.offL;
+    const int* colG = a.colG + m.offG;
+    const int K = a.K;
+    unsigned char* ldsA = tbg_smem3;
+    unsigned char* ldsB = tbg_smem3 + (size_t)a.nA_max * K * sizeof(C);
+    const int lgG = a.lgG;
+    const int g = threadIdx.x & ((1 << lgG) - 1);
+    const int rsub = threadIdx.x >> lgG;
+    const int rpp = NT >> lgG;
+    const int k0 = g * V;
+
+    R ar[V], ai[V], cf[5][V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int k = k0 + v;
+        const int cl = (int)((a.cls_mask >> k) & 1ull);
+        ar[v] = (R)a.ar[cl];
+        ai[v] = ((a.bwd_mask >> k) & 1ull) ? (R)(-a.ai[cl]) : (R)a.ai[cl];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) cf[q][v] = (R)a.c[q][cl];
+    }
+    const R beta = (R)a.beta;
+    constexpr size_t vsz = RV ? sizeof(R) : sizeof(C);
+    const unsigned char* vals = reinterpret_cast<const unsigned char*>(a.valG) + m.offG * vsz;
+    const int w = m.w, nA = m.nA, nB = m.nB;
+
+    // phase A: v_m on R + E1 + E2
+    for (int l = rsub; l < nA; l += rpp) {
+        const int gr = rows[l];
+        const Vec<R, V> prev = vld<R, V>(a.p0, (int64_t)gr * K + k0);
+        const Vec<R, V> s = row_sum<R, V, RV>(
+            w, vals, [&](int j) { return colG[j * nA + l]; }, [&](int j) { return j * nA + l; },
+            [&](int c) { return vld<R, V>(a.p1, (int64_t)c * K + k0); });
+        vst<R, V>(ldsA, (int64_t)l * K + k0, recur<R, V, true>(ar, ai, beta, s, prev));
+    }
+    __syncthreads();
+    // phase B: v_{m+1} on R + E1
+    for (int l = rsub; l < nB; l += rpp) {
+        const int gr = rows[l];
+        const Vec<R, V> prev = vld<R, V>(a.p1, (int64_t)gr * K + k0);
+        const Vec<R, V> s = row_sum<R, V, RV>(
+            w, vals, [&](int j) { return colL[j * nB + l]; }, [&](int j) { return j * nA + l; },
+            [&](int c) { return vld<R, V>(ldsA, (int64_t)c * K + k0); });
+        const Vec<R, V> v1 = recur<R, V, true>(ar, ai, beta, s, prev);
+        vst<R, V>(ldsB, (int64_t)l * K + k0, v1);
+        if (l < m.nR) vst<R, V>(a.q1, (int64_t)gr * K + k0, v1);
+    }
+    __syncthreads();
+    // phase C: v_{m+2} on R, and the step output
+    for (int l = rsub; l < m.nR; l += rpp) {
+        const int gr = rows[l];
+        const int64_t vidx = (int64_t)gr * K + k0;
+        const Vec<R, V> vm = vld<R, V>(ldsA, (int64_t)l * K + k0);
+        const Vec<R, V> s = row_sum<R, V, RV>(
+            w, vals, [&](int j) { return colL[j * nB + l]; }, [&](int j) { return j * nA + l; },
+            [&](int c) { return vld<R, V>(ldsB, (int64_t)c * K + k0); });
+        const Vec<R, V> v2 = recur<R, V, true>(ar, ai, beta, s, vm);
+        vst<R, V>(a.q2, vidx, v2);
+        const Vec<R, V> v1 = vld<R, V>(ldsB, (int64_t)l * K + k0);
+        Vec<R, V> o;
+        if constexpr (FIRST) {
+            o = vzero<R, V>();
+            axpy<R, V>(o, cf[0], vld<R, V>(a.p0, vidx));
+            axpy<R, V>(o, cf[1], vld<R, V>(a.p1, vidx));
+        } else {
+            o = vld<R, V>(a.acc, vidx);
+        }
+        axpy<R, V>(o, cf[2], vm);
+        axpy<R, V>(o, cf[3], v1);
+        axpy<R, V>(o, cf[4], v2);
+        vst<R, V>(a.acc, vidx, o);
+    }
+}
+
+template <typename R, int V, bool RV, bool FIRST>
+static void launch_f(const Term3Args& a, const Blocking& b, hipStream_t s) {
+    const size_t lds = (size_t)(b.nA_max + b.nB_max) * a.K * sizeof(typename CplxT<R>::T);
+    if (lds > 65536)
+        TBG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_term3<R, V, RV, FIRST>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_term3<R, V, RV, FIRST>), dim3(b.nblocks), dim3(term3_threads<V>()),
+                       lds, s, a);
+    TBG_HIP(hipGetLastError());
+}
+
+template <typename R, int V, bool RV>
+static void launch_v(bool first, const Term3Args& a, const Blocking& b, hipStream_t s) {
+    if (first) launch_f<R, V, RV, true>(a, b, s);
+    else launch_f<R, V, RV, false>(a, b, s);
+}
+
+template <typename R>
+void launch_term3(bool first, bool real_vals, const Term3Args& a, const Blocking& b,
+                  hipStream_t s) {
+    const int V = vectors_per_lane<R>(a.K);
+    if (a.K > 64 || (a.K & (a.K - 1)) || (1 << a.lgG) * V != a.K)
+        throw Error{TBG_EINVAL, "internal: bad vector slot count"};
+    if (b.row_bytes != a.K * (int)sizeof(typename CplxT<R>::T))
+        throw Error{TBG_EINVAL, "internal: blocking built for another row size"};
+#define TBG_T3(VV)                                                             \
+    do {                                                                       \
+        if (real_vals) launch_v<R, VV, true>(first, a, b, s);                  \
+        else launch_v<R, VV, false>(first, a, b, s);                           \
+    } while (0)
+    switch (V) {
+        case 1: TBG_T3(1); break;
+        case 2: TBG_T3(2); break;
+        case 4: TBG_T3(4); break;
+        default: throw Error{TBG_EINVAL, "internal: unsupported vectors per lane"};
+    }
+#undef TBG_T3
+}
+template void launch_term3<float>(bool, bool, const Term3Args&, const Blocking&, hipStream_t);
+template void launch_term3<double>(bool, bool, const Term3Args&, const Blocking&, hipStream_t);
+
+}  // namespace tbg

@@ -370,7 +370,7 @@ void corr_ac_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
                   int chk_step, double chk_thr, double* corr, int* fail_step) {
     using C = typename CplxT<R>::T;
     Engine<R> e;
-    e.init(h, view_of(H), {0, H->nslices}, 3 * K, chunks_for(h, H->nslices));
+    e.init(h, view_of(H), {0, H->nslices}, 3 * K, chunks_for(h, H->nslices), H);
     const int Kp = e.Kp;
     const hipStream_t st = h->stream;
     DevBuf psi(e.vbytes);
@@ -446,7 +446,7 @@ void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
     using C = typename CplxT<R>::T;
     const hipStream_t st = h->stream;
     Engine<R> e;
-    e.init(h, view_of(H), {0, H->nslices}, 2 * K, chunks_for(h, H->nslices));
+    e.init(h, view_of(H), {0, H->nslices}, 2 * K, chunks_for(h, H->nslices), H);
     const int Kp = e.Kp;
     const int64_t n = H->n, n_pad = e.n_pad;
     const int64_t nq = (int64_t)n_en * K;

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "tbg_block.h"
 #include "tbg_prop.h"
 
 namespace tbg {
@@ -86,9 +87,13 @@ struct Engine {
     int64_t n_pad = 0;
     DevBuf X, P, Y, part;
     size_t vbytes = 0;
+    // Temporal blocking (three terms per pass) and its two extra recurrence
+    // buffers, when the single Hamiltonian of the run has spatial tiles.
+    std::shared_ptr<Blocking> blk;
+    DevBuf Q, S;
 
     void init(tbg_handle* hh, const MatView& m, const std::vector<int64_t>& inst_slices,
-              int k_needed, int chunk_target) {
+              int k_needed, int chunk_target, const tbg_ham* H = nullptr) {
         h = hh;
         M = m;
         Kp = slots_for(k_needed);
@@ -106,6 +111,14 @@ struct Engine {
         launch_fill_zero(X.p, vbytes, h->stream);
         launch_fill_zero(P.p, vbytes, h->stream);
         launch_fill_zero(Y.p, vbytes, h->stream);
+        blk.reset();
+        if (H != nullptr) blk = get_blocking(h, H, Kp * (int)sizeof(typename CplxT<R>::T));
+        if (blk) {
+            Q.alloc(vbytes);
+            S.alloc(vbytes);
+            launch_fill_zero(Q.p, vbytes, h->stream);
+            launch_fill_zero(S.p, vbytes, h->stream);
+        }
     }
 
     TermArgs base_args(uint64_t bwd, uint64_t cls) const {
@@ -126,11 +139,93 @@ struct Engine {
         return a;
     }
 
+    // Three terms per pass: m = 1 alone, then triples (m, m+1, m+2) from
+    // m = 2 while at least one term is left for the last (single) launch,
+    // which also forms the correlations.  The step output receives the terms
+    // in the order m = 0, 1, 2, ... exactly as in the one-term schedule.
+    void step_blocked(const StepPoly& sp, uint64_t bwd, uint64_t cls, int epi,
+                      const void* psi0, int* counter) {
+        const int N = sp.N;
+        {
+            TermArgs a = base_args(bwd, cls);
+            a.acc = Y.p;
+            a.pin = X.p;
+            a.pout = P.p;
+            a.ar[0] = a.ar[1] = sp.ar1;
+            a.ai[0] = a.ai[1] = sp.ai1;
+            a.counter = counter;
+            launch_term<R>(F_STORE, a, plan.nchunks, h->stream);
+        }
+        void* b0 = X.p;   // v_{m-2}
+        void* b1 = P.p;   // v_{m-1}
+        void* b2 = Q.p;
+        void* b3 = S.p;
+        const int ntri = (N - 3) / 3;
+        int m = 2;
+        for (int i = 0; i < ntri; ++i, m += 3) {
+            Term3Args t{};
+            t.meta = blk->meta.as<BlockMeta>();
+            t.rows = blk->rows.as<int>();
+            t.colL = blk->colL.as<int>();
+            t.colG = blk->colG.as<int>();
+            t.valG = blk->valG.p;
+            t.p0 = b0;
+            t.p1 = b1;
+            t.q1 = b2;
+            t.q2 = b3;
+            t.acc = Y.p;
+            t.ar[0] = t.ar[1] = sp.ar;
+            t.ai[0] = t.ai[1] = sp.ai;
+            t.beta = sp.beta;
+            for (int q = 0; q < 5; ++q) {
+                t.c[q][0] = sp.c[0][m - 2 + q];
+                t.c[q][1] = sp.c[1][m - 2 + q];
+            }
+            t.bwd_mask = bwd;
+            t.cls_mask = cls;
+            t.K = Kp;
+            t.lgG = lgG;
+            t.nA_max = blk->nA_max;
+            launch_term3<R>(i == 0, M.real_vals != 0, t, *blk, h->stream);
+            std::swap(b0, b2);
+            std::swap(b1, b3);
+        }
+        for (; m < N; ++m) {
+            const bool last = (m == N - 1);
+            TermArgs a = base_args(bwd, cls);
+            a.acc = Y.p;
+            a.pin = b1;
+            a.pprev = b0;
+            a.pout = b0;
+            a.ar[0] = a.ar[1] = sp.ar;
+            a.ai[0] = a.ai[1] = sp.ai;
+            a.beta = sp.beta;
+            a.cc[0] = sp.c[0][m];
+            a.cc[1] = sp.c[1][m];
+            int fl = F_PREV | F_ACC | F_ACCRMW;
+            if (last) {
+                fl |= epi;
+                a.psi0 = psi0;
+                a.part = part.as<double>();
+            } else {
+                fl |= F_STORE;
+            }
+            launch_term<R>(fl, a, plan.nchunks, h->stream);
+            std::swap(b0, b1);
+        }
+        std::swap(X, Y);
+        parity ^= 1;
+    }
+
     // One step: X holds the input, the result is left in X (buffers swap).
     void step(const StepPoly& sp, uint64_t bwd, uint64_t cls, int epi, const void* psi0,
               int* counter) {
         const int N = sp.N;
         TBG_REQUIRE(N >= 3, "at least 3 expansion terms are required");
+        if (blk && N >= 6) {
+            step_blocked(sp, bwd, cls, epi, psi0, counter);
+            return;
+        }
         auto buf = [&](int j) { return (j % 2 == 0) ? X.p : P.p; };
         for (int m = 1; m < N; ++m) {
             TermArgs a = base_args(bwd, cls);

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -78,6 +79,8 @@ template <> struct CplxT<double> { using T = double2; };
 constexpr int kSlice = 64;
 constexpr int kWave = 64;
 
+struct Blocking;
+
 }  // namespace tbg
 
 // Library handle: one per host thread; all work is ordered on `stream`.
@@ -91,6 +94,9 @@ struct tbg_handle {
     int blocks_per_cu = 16;
     int stage = 1;
     int real_layout = 1;      // store real values when H has no imaginary part
+    int blocking = 1;         // three terms per pass where H has spatial tiles
+    int block_rows = 0;       // cap on rows owned per workgroup (0: from LDS)
+    int max_lds = 65536;      // LDS bytes one workgroup may use
     tbg::DevBuf scratch;      // reduction partials and small tables
     tbg::DevBuf counter;      // step counter for graph-replayed reductions
 };
@@ -118,4 +124,11 @@ struct tbg_ham {
     tbg::DevBuf perm;
     bool ordered = false;
     const int* perm_ptr() const { return ordered ? perm.as<int>() : nullptr; }
+    // Spatial tiles of the layout order (host, ntiles + 1 row offsets) and
+    // the temporal-blocking tables built from them on first use.
+    std::vector<int64_t> tiles;
+    mutable std::shared_ptr<tbg::Blocking> blk;
+    mutable bool blk_failed = false;
+    mutable int blk_row_bytes = 0;
+    mutable int blk_rows_cap = 0;
 };

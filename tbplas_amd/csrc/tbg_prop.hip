@@ -24,185 +24,10 @@
 #include <type_traits>
 
 #include "tbg_prop.h"
+#include "tbg_vec.h"
 
 namespace tbg {
 
-
-template <typename R, int V> struct Vec {
-    R re[V];
-    R im[V];
-};
-
-template <typename R, int V>
-__device__ __forceinline__ Vec<R, V> vzero() {
-    Vec<R, V> o;
-#pragma unroll
-    for (int v = 0; v < V; ++v) { o.re[v] = R(0); o.im[v] = R(0); }
-    return o;
-}
-
-// idx is in complex elements; V consecutive complex values from idx (idx
-// even when V is even: complex64 pairs are read as one 16-byte load).
-template <typename R, int V>
-__device__ __forceinline__ Vec<R, V> vld(const void* base, int64_t idx) {
-    Vec<R, V> o;
-    if constexpr (std::is_same<R, float>::value && V % 2 == 0) {
-#pragma unroll
-        for (int q = 0; q < V / 2; ++q) {
-            const float4 t = reinterpret_cast<const float4*>(base)[(idx >> 1) + q];
-            o.re[2 * q] = t.x; o.im[2 * q] = t.y; o.re[2 * q + 1] = t.z; o.im[2 * q + 1] = t.w;
-        }
-    } else if constexpr (std::is_same<R, float>::value) {
-        const float2 t = reinterpret_cast<const float2*>(base)[idx];
-        o.re[0] = t.x; o.im[0] = t.y;
-    } else {
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            const double2 t = reinterpret_cast<const double2*>(base)[idx + v];
-            o.re[v] = t.x; o.im[v] = t.y;
-        }
-    }
-    return o;
-}
-
-template <typename R, int V>
-__device__ __forceinline__ void vst(void* base, int64_t idx, const Vec<R, V>& o) {
-    if constexpr (std::is_same<R, float>::value && V % 2 == 0) {
-#pragma unroll
-        for (int q = 0; q < V / 2; ++q)
-            reinterpret_cast<float4*>(base)[(idx >> 1) + q] =
-                make_float4(o.re[2 * q], o.im[2 * q], o.re[2 * q + 1], o.im[2 * q + 1]);
-    } else if constexpr (std::is_same<R, float>::value) {
-        reinterpret_cast<float2*>(base)[idx] = make_float2(o.re[0], o.im[0]);
-    } else {
-#pragma unroll
-        for (int v = 0; v < V; ++v)
-            reinterpret_cast<double2*>(base)[idx + v] = make_double2(o.re[v], o.im[v]);
-    }
-}
-
-// Streams touched once per term (v_{m-2}, the step output, psi0, v_m) can be
-// loaded and stored with the non-temporal hint, so the L2 keeps the rows of
-// v_{m-1} that neighbouring slices gather.  Measured on 8.4M-orbital
-// graphene this gains 8-9% with one vector per lane (complex128 or
-// complex64, K = 1) and loses 10-35% with two or more (K >= 4), so the term
-// kernel uses it for V = 1 only (stream_ld / stream_st).
-typedef float tbg_f4 __attribute__((ext_vector_type(4)));
-typedef float tbg_f2 __attribute__((ext_vector_type(2)));
-typedef double tbg_d2 __attribute__((ext_vector_type(2)));
-
-template <typename R, int V>
-__device__ __forceinline__ Vec<R, V> vld_nt(const void* base, int64_t idx) {
-    Vec<R, V> o;
-    if constexpr (std::is_same<R, float>::value && V % 2 == 0) {
-#pragma unroll
-        for (int q = 0; q < V / 2; ++q) {
-            const tbg_f4 t = __builtin_nontemporal_load(
-                reinterpret_cast<const tbg_f4*>(base) + (idx >> 1) + q);
-            o.re[2 * q] = t.x; o.im[2 * q] = t.y; o.re[2 * q + 1] = t.z; o.im[2 * q + 1] = t.w;
-        }
-    } else if constexpr (std::is_same<R, float>::value) {
-        const tbg_f2 t = __builtin_nontemporal_load(reinterpret_cast<const tbg_f2*>(base) + idx);
-        o.re[0] = t.x; o.im[0] = t.y;
-    } else {
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            const tbg_d2 t = __builtin_nontemporal_load(
-                reinterpret_cast<const tbg_d2*>(base) + idx + v);
-            o.re[v] = t.x; o.im[v] = t.y;
-        }
-    }
-    return o;
-}
-
-template <typename R, int V>
-__device__ __forceinline__ void vst_nt(void* base, int64_t idx, const Vec<R, V>& o) {
-    if constexpr (std::is_same<R, float>::value && V % 2 == 0) {
-#pragma unroll
-        for (int q = 0; q < V / 2; ++q) {
-            const tbg_f4 t = {o.re[2 * q], o.im[2 * q], o.re[2 * q + 1], o.im[2 * q + 1]};
-            __builtin_nontemporal_store(t, reinterpret_cast<tbg_f4*>(base) + (idx >> 1) + q);
-        }
-    } else if constexpr (std::is_same<R, float>::value) {
-        const tbg_f2 t = {o.re[0], o.im[0]};
-        __builtin_nontemporal_store(t, reinterpret_cast<tbg_f2*>(base) + idx);
-    } else {
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            const tbg_d2 t = {o.re[v], o.im[v]};
-            __builtin_nontemporal_store(t, reinterpret_cast<tbg_d2*>(base) + idx + v);
-        }
-    }
-}
-
-template <typename R, int V>
-__device__ __forceinline__ Vec<R, V> stream_ld(const void* base, int64_t idx) {
-    if constexpr (V == 1) return vld_nt<R, V>(base, idx);
-    else return vld<R, V>(base, idx);
-}
-template <typename R, int V>
-__device__ __forceinline__ void stream_st(void* base, int64_t idx, const Vec<R, V>& o) {
-    if constexpr (V == 1) vst_nt<R, V>(base, idx, o);
-    else vst<R, V>(base, idx, o);
-}
-
-// One row of a slice as seen by one lane during the sparse product.
-template <typename R>
-struct RowCtx {
-    const int* colp;                      // slice columns (element j*64 + lr)
-    const typename CplxT<R>::T* valp;     // slice values, complex layout
-    const R* rvalp;                       // slice values, real layout
-    int lr;                               // local row in the slice
-    int64_t row0;                         // first row of the slice
-    const unsigned char* stage;           // the slice's rows of pin in LDS
-    const void* pin;                      // gathered vector
-    int K, k0;
-};
-
-// s += sum_{j0 <= j < j0+NJ} H[row][j] x[col_j] for V vectors of the row.
-// All column and value loads are issued first, then all gathers, then the
-// products in j order, so NJ gathers are in flight at once (a loop over j
-// would wait for each column before its gather).
-template <typename R, int V, bool STAGE, bool RV, int NJ>
-__device__ __forceinline__ void row_group(Vec<R, V>& s, const RowCtx<R>& rc, int j0) {
-    using C = typename CplxT<R>::T;
-    int col[NJ];
-    R hr[NJ];
-    C hc[NJ];
-#pragma unroll
-    for (int u = 0; u < NJ; ++u) {
-        const int e = (j0 + u) * 64 + rc.lr;
-        col[u] = rc.colp[e];
-        if constexpr (RV) hr[u] = rc.rvalp[e];
-        else hc[u] = rc.valp[e];
-    }
-    Vec<R, V> x[NJ];
-#pragma unroll
-    for (int u = 0; u < NJ; ++u) {
-        if constexpr (STAGE) {
-            const uint32_t d = (uint32_t)((int64_t)col[u] - rc.row0);
-            if (d < 64u) x[u] = vld<R, V>(rc.stage, (int64_t)d * rc.K + rc.k0);
-            else x[u] = vld<R, V>(rc.pin, (int64_t)col[u] * rc.K + rc.k0);
-        } else {
-            x[u] = vld<R, V>(rc.pin, (int64_t)col[u] * rc.K + rc.k0);
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < NJ; ++u) {
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            if constexpr (RV) {
-                s.re[v] = fma(hr[u], x[u].re[v], s.re[v]);
-                s.im[v] = fma(hr[u], x[u].im[v], s.im[v]);
-            } else {
-                s.re[v] = fma(hc[u].x, x[u].re[v], s.re[v]);
-                s.re[v] = fma(-hc[u].y, x[u].im[v], s.re[v]);
-                s.im[v] = fma(hc[u].x, x[u].im[v], s.im[v]);
-                s.im[v] = fma(hc[u].y, x[u].re[v], s.im[v]);
-            }
-        }
-    }
-}
 
 // STAGE: the slice's own 64 rows of v_{m-1} are first read once, coalesced,
 // into LDS; gathers whose column falls inside the slice (for lattice models
@@ -254,8 +79,8 @@ __global__ void __launch_bounds__(256) k_term(TermArgs a) {
         const int64_t off = a.soff[sl];
         const int w = a.sw[sl];
         const int* colp = a.cols + off;
-        const C* valp = reinterpret_cast<const C*>(a.vals) + off;
-        const R* rvalp = reinterpret_cast<const R*>(a.vals) + off;
+        const void* valp = reinterpret_cast<const unsigned char*>(a.vals) +
+                           off * (RV ? sizeof(R) : sizeof(C));
         const int64_t row0 = sl << 6;
         if constexpr (STAGE) {
             for (int p = 0; p < G; ++p) {
@@ -275,43 +100,24 @@ __global__ void __launch_bounds__(256) k_term(TermArgs a) {
                 else mid = vld<R, V>(a.pin, vidx);
             }
             if constexpr ((FL & F_ACCRMW) != 0) accv = stream_ld<R, V>(a.acc, vidx);
-            Vec<R, V> s = vzero<R, V>();
-            const RowCtx<R> rc{colp, valp, rvalp, lr, row0, stage, a.pin, K, k0};
-            int j0 = 0;
-            for (; j0 + 4 <= w; j0 += 4) row_group<R, V, STAGE, RV, 4>(s, rc, j0);
-            switch (w - j0) {
-                case 1: row_group<R, V, STAGE, RV, 1>(s, rc, j0); break;
-                case 2: row_group<R, V, STAGE, RV, 2>(s, rc, j0); break;
-                case 3: row_group<R, V, STAGE, RV, 3>(s, rc, j0); break;
-                default: break;
-            }
-            Vec<R, V> cur;
-#pragma unroll
-            for (int v = 0; v < V; ++v) {
-                cur.re[v] = ar[v] * s.re[v] - ai[v] * s.im[v];
-                cur.im[v] = ar[v] * s.im[v] + ai[v] * s.re[v];
-                if constexpr ((FL & F_PREV) != 0) {
-                    cur.re[v] = fma(beta, prev.re[v], cur.re[v]);
-                    cur.im[v] = fma(beta, prev.im[v], cur.im[v]);
+            const auto idx = [&](int j) { return colp[j * 64 + lr]; };
+            const auto ent = [&](int j) { return j * 64 + lr; };
+            const auto gather = [&](int col) {
+                if constexpr (STAGE) {
+                    const uint32_t d = (uint32_t)((int64_t)col - row0);
+                    if (d < 64u) return vld<R, V>(stage, (int64_t)d * K + k0);
                 }
-            }
+                return vld<R, V>(a.pin, (int64_t)col * K + k0);
+            };
+            const Vec<R, V> s = row_sum<R, V, RV>(w, valp, idx, ent, gather);
+            const Vec<R, V> cur = recur<R, V, (FL & F_PREV) != 0>(ar, ai, beta, s, prev);
             if constexpr ((FL & F_STORE) != 0) stream_st<R, V>(a.pout, vidx, cur);
             if constexpr ((FL & F_ACC) != 0) {
                 Vec<R, V> o;
                 if constexpr ((FL & F_ACCRMW) != 0) o = accv; else o = vzero<R, V>();
-#pragma unroll
-                for (int v = 0; v < V; ++v) {
-                    if constexpr ((FL & F_ACCPREV) != 0) {
-                        o.re[v] = fma(cp[v], prev.re[v], o.re[v]);
-                        o.im[v] = fma(cp[v], prev.im[v], o.im[v]);
-                    }
-                    if constexpr ((FL & F_MID) != 0) {
-                        o.re[v] = fma(cm[v], mid.re[v], o.re[v]);
-                        o.im[v] = fma(cm[v], mid.im[v], o.im[v]);
-                    }
-                    o.re[v] = fma(cc[v], cur.re[v], o.re[v]);
-                    o.im[v] = fma(cc[v], cur.im[v], o.im[v]);
-                }
+                if constexpr ((FL & F_ACCPREV) != 0) axpy<R, V>(o, cp, prev);
+                if constexpr ((FL & F_MID) != 0) axpy<R, V>(o, cm, mid);
+                axpy<R, V>(o, cc, cur);
                 stream_st<R, V>(a.acc, vidx, o);
                 if constexpr ((FL & F_DOT) != 0) {
                     const Vec<R, V> p0 = stream_ld<R, V>(a.psi0, vidx);
@@ -494,6 +300,7 @@ __global__ void k_fill_zero(uint4* p, int64_t n16) {
     X(F_PREV | F_ACC | F_ACCRMW | F_ACCPREV | F_MID | F_NORM)                  \
     X(F_PREV | F_ACC | F_ACCRMW | F_ACCPREV | F_MID | F_NORM | F_DOT)          \
     X(F_PREV | F_ACC | F_ACCRMW)                                               \
+    X(F_PREV | F_STORE | F_ACC | F_ACCRMW)                                     \
     X(F_PREV | F_ACC | F_ACCRMW | F_NORM)                                      \
     X(F_PREV | F_ACC | F_ACCRMW | F_NORM | F_DOT)                              \
     X(F_PREV | F_ACC | F_ACCRMW | F_MID)                                       \

@@ -172,19 +172,25 @@ class Sample:
         if self.hop_i is None:
             raise ValueError("hopping arrays not set")
 
-    def tile_order(self, tile=4.0):
+    def tile_order(self, tile=4.0, return_tiles=False):
         """Row order for the device layout (``Hamiltonian(order=...)``):
         orbitals binned into square tiles of ``tile`` nm in the x-y plane,
         tiles row by row, the original order inside a tile.  Neighbours then
         mostly share a 64-row slice or sit in the next tile, instead of a
-        whole lattice row away."""
+        whole lattice row away.  With ``return_tiles`` also the row offsets
+        of the non-empty tiles in that order (``Hamiltonian(tiles=...)``)."""
         self.init_orb_pos()
         xy = self.orb_pos[:, :2]
         lo = xy.min(axis=0)
         b = np.floor((xy - lo) / float(tile)).astype(np.int64)
         nx = int(b[:, 0].max()) + 1
         key = b[:, 1] * nx + b[:, 0]
-        return np.argsort(key, kind="stable").astype(np.int64)
+        perm = np.argsort(key, kind="stable").astype(np.int64)
+        if not return_tiles:
+            return perm
+        ks = key[perm]
+        starts = np.flatnonzero(np.r_[True, ks[1:] != ks[:-1]])
+        return perm, np.r_[starts, len(perm)].astype(np.int64)
 
     def save_array(self, data_dir="sample"):
         """Same files as the reference (sample.py:220-240)."""

@@ -31,12 +31,13 @@ class Hamiltonian:
     complex128); dx/dy (float64) enable the current operators
     J = (0, cur_scale) * hop * d (funcs.f90:12-33).  ``order`` (optional,
     int64 (n,)) lists the rows in the order the device layout stores them,
-    e.g. ``Sample.tile_order()``; results are in the caller's row order
-    either way.
+    e.g. ``Sample.tile_order()``; ``tiles`` (optional, int64 row offsets
+    0..n in that order) marks spatial tiles, which enables three Chebyshev
+    terms per pass.  Results are in the caller's row order either way.
     """
 
     def __init__(self, n, indptr, indices, hop, dx=None, dy=None, cur_scale=1.0,
-                 precision="fp64", handle=None, device=0, order=None):
+                 precision="fp64", handle=None, device=0, order=None, tiles=None):
         torch = _lib.torch_cuda()
         self.handle = handle or _lib.default_handle(device)
         dv = f"cuda:{self.handle.device}"
@@ -63,13 +64,17 @@ class Hamiltonian:
         if od is not None and od.numel() != self.n:
             raise ValueError("order must list every row once")
         self.ordered = od is not None
+        tp = None if tiles is None else np.ascontiguousarray(tiles, dtype=np.int64)
+        self.tiled = tp is not None
         out = ctypes.c_void_p()
         _lib.check(_lib.lib().tbg_ham_create_ordered(
             self.handle.ptr, self.n, _lib.ptr(ip), _lib.ptr(ix), _lib.ptr(hv),
             _lib.ptr(dxd) if dxd is not None else None,
             _lib.ptr(dyd) if dyd is not None else None,
             float(cur_scale), self.precision,
-            _lib.ptr(od) if od is not None else None, ctypes.byref(out)))
+            _lib.ptr(od) if od is not None else None,
+            tp.ctypes.data if tp is not None else None,
+            0 if tp is None else len(tp) - 1, ctypes.byref(out)))
         self._h = out
 
     @property
@@ -85,6 +90,15 @@ class Hamiltonian:
         _lib.check(_lib.lib().tbg_ham_is_real(self._h, ctypes.byref(real)))
         return {"n": n.value, "nnz": nnz.value, "padded": pad.value,
                 "max_width": w.value, "real_values": bool(real.value)}
+
+    def blocking_info(self, states, precision=None):
+        """(workgroup tiles, halo factor) of the three-term pass for
+        ``states`` vectors; (0, 0.0) when H runs one term per pass."""
+        nb, halo = ctypes.c_int(), ctypes.c_double()
+        prec = self.precision if precision is None else precision_code(precision)
+        _lib.check(_lib.lib().tbg_ham_blocking_info(self.handle.ptr, self._h, int(states), prec,
+                                                    ctypes.byref(nb), ctypes.byref(halo)))
+        return nb.value, halo.value
 
     def close(self):
         if getattr(self, "_h", None):

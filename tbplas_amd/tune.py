@@ -3,7 +3,8 @@
 For a prepared Hamiltonian, a state count and a precision it times a few
 Chebyshev steps (HIP events on the library stream, after warm-up steps) for
 every candidate (persistent blocks per CU, LDS staging of the slice rows)
-and keeps the fastest.  Results are cached per (device name, library
+and keeps the fastest; for a Hamiltonian with spatial tiles it then times
+the best shape with three terms per pass and keeps that if it is faster.  Results are cached per (device name, library
 build, instances, rows, stored elements, widest slice, value layout, states,
 precision) in memory and in a JSON file
 (``$TBPLAS_AMD_TUNE_CACHE``, default ``~/.cache/tbplas_amd/tune.json``).
@@ -19,7 +20,7 @@ from . import _lib
 __all__ = ["autotune", "apply", "CANDIDATES"]
 
 CANDIDATES = [(bpc, stage) for bpc in (4, 8, 12, 16) for stage in (True, False)]
-DEFAULT = (16, True)
+DEFAULT = (16, True, True)
 
 _lock = threading.Lock()
 _mem = {}
@@ -67,9 +68,9 @@ def _key(hams, K):
 
 
 def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
-    """Pick (blocks_per_cu, stage) for Hamiltonian H (or a list of
-    Hamiltonians run as one batch) and K states each; returns (best,
-    timings) and leaves the handle set to the best shape."""
+    """Pick (blocks_per_cu, stage, three terms per pass) for Hamiltonian H
+    (or a list of Hamiltonians run as one batch) and K states each; returns
+    (best, timings) and leaves the handle set to the best shape."""
     from .tbpm import DosRun
     hams = list(H) if isinstance(H, (list, tuple)) else [H]
     H = hams[0]
@@ -78,34 +79,47 @@ def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
         if use_cache:
             if key in _mem:
                 best = tuple(_mem[key])
-                H.handle.set_tuning(*best)
+                _set(H.handle, best)
                 return best, None
             disk = _load_disk()
-            if key in disk:
-                best = (int(disk[key][0]), bool(disk[key][1]))
+            if key in disk and len(disk[key]) == 3:
+                best = (int(disk[key][0]), bool(disk[key][1]), bool(disk[key][2]))
                 _mem[key] = best
-                H.handle.set_tuning(*best)
+                _set(H.handle, best)
                 return best, None
         rng = np.random.default_rng(0)
         psis = []
         for Hi in hams:
             psi = np.exp(2j * np.pi * rng.random((K, Hi.n)))
             psis.append(psi / np.linalg.norm(psi, axis=1, keepdims=True))
-        timings = {}
-        for cand in candidates or CANDIDATES:
-            H.handle.set_tuning(*cand)
+        def timed(shape):
+            _set(H.handle, shape)
             run = DosRun(hams, bes, psis, warmup + steps)
             run.advance(warmup)
-            timings[cand] = run.advance(steps) / steps
+            ms = run.advance(steps) / steps
             run.close()
+            return ms
+
+        timings = {}
+        for cand in candidates or CANDIDATES:
+            timings[cand + (False,)] = timed(cand + (False,))
         best = min(timings, key=timings.get)
-        H.handle.set_tuning(*best)
+        if len(hams) == 1 and getattr(H, "tiled", False):
+            blocked = best[:2] + (True,)
+            timings[blocked] = timed(blocked)
+            best = min(timings, key=timings.get)
+        _set(H.handle, best)
         _mem[key] = best
         if use_cache:
             disk = _load_disk()
-            disk[key] = [best[0], best[1]]
+            disk[key] = [best[0], best[1], best[2]]
             _save_disk(disk)
         return best, timings
+
+
+def _set(handle, shape):
+    handle.set_tuning(shape[0], shape[1])
+    handle.set_blocking(shape[2])
 
 
 def apply(H, K, bes, enabled=True):
@@ -114,5 +128,5 @@ def apply(H, K, bes, enabled=True):
     if enabled:
         return autotune(H, K, bes)[0]
     h = (H[0] if isinstance(H, (list, tuple)) else H).handle
-    h.set_tuning(*DEFAULT)
+    _set(h, DEFAULT)
     return DEFAULT

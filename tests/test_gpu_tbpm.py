@@ -177,6 +177,74 @@ def test_tile_order_equals_caller_order(gpu, prec):
         H.close()
 
 
+def _blocked_sample(L=48):
+    s = tb.make_graphene_sample(L, vacancies=tb.random_vacancies(2 * L * L, 0.02, 11))
+    tb.add_anderson_disorder(s, 0.6, 12)
+    s.rescale_ham(9.0)
+    return s
+
+
+@pytest.mark.parametrize("prec,K", [("fp32", 8), ("fp32", 3), ("fp32", 1), ("fp64", 1),
+                                    ("fp64", 4)])
+@pytest.mark.parametrize("nb_prec", [1e-15, 1e-14, 1e-6])
+def test_three_term_pass_bitwise(gpu, prec, K, nb_prec):
+    """Three terms per pass over spatial tiles gives the same states and
+    correlations, bit for bit, as one term per pass; series lengths 21, 20
+    and 12 end the step with one, three and two single terms."""
+    s = _blocked_sample()
+    ip, ix, dt = _csr(s)
+    order, tiles = s.tile_order(0.8, return_tiles=True)
+    bes = tbpm.bessel_series(tau_of(s), nb_prec)
+    psi = _states(s.num_orb, K, seed=21)
+    h = tb._lib.Handle(0)
+    try:
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec, handle=h, order=order,
+                             tiles=tiles)
+        nblk, halo = H.blocking_info(K)
+        assert nblk > 1 and 1.0 < halo < 4.0
+        a = tbpm.corr_dos(H, bes, psi, 30, chk_thr=1e-4)
+        h.set_blocking(False)
+        assert H.blocking_info(K) == (0, 0.0)
+        b = tbpm.corr_dos(H, bes, psi, 30, chk_thr=1e-4)
+        np.testing.assert_array_equal(a, b)
+        ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 30)
+        assert np.max(np.abs(a - ref)) < (1e-12 if prec == "fp64" else 2e-5)
+        H.close()
+    finally:
+        h.close()
+
+
+def test_three_term_pass_graph_and_step(gpu):
+    """hipGraph replay of blocked steps equals launched steps; one blocked
+    step (cheb_step) equals the oracle's step."""
+    s = _blocked_sample(32)
+    ip, ix, dt = _csr(s)
+    order, tiles = s.tile_order(0.7, return_tiles=True)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, 4, seed=22)
+    h = tb._lib.Handle(0)
+    try:
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp32", handle=h, order=order,
+                             tiles=tiles)
+        assert H.blocking_info(4)[0] > 1
+        h.set_graph(True)
+        a = tbpm.corr_dos(H, bes, psi, 70)
+        h.set_graph(False)
+        b = tbpm.corr_dos(H, bes, psi, 70)
+        np.testing.assert_array_equal(a, b)
+        H64 = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64", handle=h,
+                               order=order, tiles=tiles)
+        assert H64.blocking_info(3)[0] > 1
+        g = tbpm.cheb_step(H64, bes, psi[:3], True)
+        for k in range(3):
+            r = oracle.orc_step(ip, ix, dt, bes, psi[k], True)
+            assert np.max(np.abs(g[k] - r)) < 1e-14
+        H.close()
+        H64.close()
+    finally:
+        h.close()
+
+
 def test_order_must_be_a_permutation(gpu):
     s = graphene_sample(4)
     ip, ix, dt = _csr(s)
@@ -355,15 +423,26 @@ def _random_order(n, seed):
     return np.random.default_rng(seed).permutation(n).astype(np.int64)
 
 
-@pytest.mark.parametrize("ordered", [False, True])
-def test_ac_reference_fixture(gpu, ordered):
+def _row_tiles(n, rows):
+    return np.r_[np.arange(0, n, rows), n].astype(np.int64)
+
+
+@pytest.mark.parametrize("layout", ["plain", "ordered", "tiles"])
+def test_ac_reference_fixture(gpu, layout):
     """ordered: the device layout stores the rows in a random order (every
-    gather leaves its slice); results stay in the caller's order."""
+    gather leaves its slice); tiles: three terms per pass, workgroups of 32
+    rows; results stay in the caller's order."""
     g = golden("ref_ac.npz")
     n = len(g["indptr"]) - 1
+    hd = tb._lib.Handle(0) if layout == "tiles" else None
+    if hd is not None:
+        hd.set_blocking(True, 32)
     H = tbpm.Hamiltonian(n, g["indptr"], g["indices"], g["hop"], g["dx"], g["dy"],
-                         cur_scale=float(g["rescale"]), precision="fp64",
-                         order=_random_order(n, 7) if ordered else None)
+                         cur_scale=float(g["rescale"]), precision="fp64", handle=hd,
+                         order=_random_order(n, 7) if layout == "ordered" else None,
+                         tiles=_row_tiles(n, 32) if layout == "tiles" else None)
+    if layout == "tiles":
+        assert H.blocking_info(3 * g["psi0"].shape[0])[0] == 4
     cF = tbpm.fermi_cheb_coef(2 ** 15, float(g["beta_re"]), float(g["mu_re"]), False, 1e-12)
     cO = tbpm.fermi_cheb_coef(2 ** 15, float(g["beta_re"]), float(g["mu_re"]), True, 1e-12)
     c = tbpm.corr_ac(H, g["bes"], cF, cO, g["psi0"], int(g["T"]))
@@ -374,6 +453,9 @@ def test_ac_reference_fixture(gpu, ordered):
     ref = g["corr"]
     assert np.all(avg[:, 0] == 0)
     assert np.max(np.abs(avg - ref)) < 1e-11 * max(1.0, np.max(np.abs(ref)))
+    H.close()
+    if hd is not None:
+        hd.close()
 
 
 def test_ac_fp32_vs_oracle(gpu):
@@ -408,13 +490,19 @@ def test_fermi_coef_product_vs_reference_fixture(gpu):
     assert np.max(np.abs(c - g["coef_F_cold"])) < 1e-14
 
 
-@pytest.mark.parametrize("ordered", [False, True])
-def test_dc_reference_fixture(gpu, ordered):
+@pytest.mark.parametrize("layout", ["plain", "ordered", "tiles"])
+def test_dc_reference_fixture(gpu, layout):
     g = golden("ref_dc.npz")
     n = len(g["indptr"]) - 1
+    hd = tb._lib.Handle(0) if layout == "tiles" else None
+    if hd is not None:
+        hd.set_blocking(True, 32)
     H = tbpm.Hamiltonian(n, g["indptr"], g["indices"], g["hop"], g["dx"], g["dy"],
-                         cur_scale=float(g["rescale"]), precision="fp64",
-                         order=_random_order(n, 8) if ordered else None)
+                         cur_scale=float(g["rescale"]), precision="fp64", handle=hd,
+                         order=_random_order(n, 8) if layout == "ordered" else None,
+                         tiles=_row_tiles(n, 32) if layout == "tiles" else None)
+    if layout == "tiles":
+        assert H.blocking_info(2 * g["psi0"].shape[0])[0] == 4
     dos, dc = tbpm.corr_dc(H, g["bes"], g["psi0"], int(g["T"]), float(g["t_step"]),
                            g["energies"], g["en_inds"])
     S = dos.shape[0]
@@ -426,6 +514,9 @@ def test_dc_reference_fixture(gpu, ordered):
         acc[:, :ne, :] = acc[:, :ne, :] + dc[s] / S
     assert np.max(np.abs(adc - g["dos"])) < 1e-12
     assert np.max(np.abs(acc - g["dc"])) < 1e-10 * max(1.0, np.max(np.abs(g["dc"])))
+    H.close()
+    if hd is not None:
+        hd.close()
 
 
 def test_dc_fp32_vs_oracle(gpu):
