@@ -300,7 +300,8 @@ def latency_workload(args):
     """Pristine graphene 256x256 (131,072 orbitals, rescale 9), one state,
     1024 DOS steps (21 Bessel terms at 1e-15): the case whose vectors fit in
     the last-level cache, reported as microseconds per step, complex128 and
-    complex64, hipGraph replay."""
+    complex64, hipGraph replay, 2 nm tiles (the tuner picks one or three
+    terms per pass)."""
     import tbplas_amd as tb
     from tbplas_amd import tbpm, _lib, tune
     L, T = 256, 1024
@@ -312,14 +313,19 @@ def latency_workload(args):
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, 1))
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
+    # 2 nm tiles (~150 rows): enough workgroups for 256 CUs on this size
+    order, tiles = (s.tile_order(2.0, return_tiles=True) if args.order == "tiles"
+                    else (None, None))
     res = {}
     for prec in ("fp64", "fp32"):
-        H = tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h)
+        H = tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h, order=order, tiles=tiles)
         shape = tune.apply(H, 1, bes, not args.no_tune)
         ms, _ = gpu_run(H, bes, psi, 32, T)
         H.close()
         res[prec] = {"us_per_step": round(ms / T * 1e3, 2),
                      "us_per_term": round(ms / T / (len(bes) - 1) * 1e3, 3),
+                     "layout": ("caller order" if order is None else "tiles of 2.0 nm")
+                     + (", three terms per pass" if shape[2] else ", one term per pass"),
                      "launch_shape": shape_json(shape)}
     out = {"metric": "TBPM microseconds per step, 131,072-orbital graphene, 1 state",
            "value": res["fp64"]["us_per_step"], "unit": "us/step", "n_gpus": 1,

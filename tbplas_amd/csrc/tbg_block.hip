@@ -69,6 +69,8 @@ std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_
     // owned rows per block-tile: whole tiles, up to the row target
     const int lds = std::min(h->max_lds, kBlockLdsTarget);
     int64_t target = std::max<int64_t>(64, (int64_t)(lds / (2.6 * row_bytes)));
+    // small samples: at least two workgroups per CU
+    target = std::min<int64_t>(target, std::max<int64_t>(64, n / (2 * (int64_t)h->num_cus)));
     if (h->block_rows > 0) target = std::min<int64_t>(target, h->block_rows);
     std::vector<int64_t> starts;
     {
@@ -244,20 +246,24 @@ __global__ void __launch_bounds__(term3_threads<V>()) k_term3(Term3Args a) {
     const unsigned char* vals = reinterpret_cast<const unsigned char*>(a.valG) + m.offG * vsz;
     const int w = m.w, nA = m.nA, nB = m.nB;
 
-    // phase A: v_m on R + E1 + E2
+    // phase A: v_m on R + E1 + E2.  The rows of B also park their v_{m-1}
+    // in the (still unused) B area of the LDS for phase B: read now, next
+    // to this block's gathers of the same rows, it mostly hits L2.
     for (int l = rsub; l < nA; l += rpp) {
         const int gr = rows[l];
         const Vec<R, V> prev = vld<R, V>(a.p0, (int64_t)gr * K + k0);
+        if (l < nB) vst<R, V>(ldsB, (int64_t)l * K + k0, vld<R, V>(a.p1, (int64_t)gr * K + k0));
         const Vec<R, V> s = row_sum<R, V, RV>(
             w, vals, [&](int j) { return colG[j * nA + l]; }, [&](int j) { return j * nA + l; },
             [&](int c) { return vld<R, V>(a.p1, (int64_t)c * K + k0); });
         vst<R, V>(ldsA, (int64_t)l * K + k0, recur<R, V, true>(ar, ai, beta, s, prev));
     }
     __syncthreads();
-    // phase B: v_{m+1} on R + E1
+    // phase B: v_{m+1} on R + E1 (each lane reads its own row's parked
+    // v_{m-1} before overwriting it with v_{m+1})
     for (int l = rsub; l < nB; l += rpp) {
         const int gr = rows[l];
-        const Vec<R, V> prev = vld<R, V>(a.p1, (int64_t)gr * K + k0);
+        const Vec<R, V> prev = vld<R, V>(ldsB, (int64_t)l * K + k0);
         const Vec<R, V> s = row_sum<R, V, RV>(
             w, vals, [&](int j) { return colL[j * nB + l]; }, [&](int j) { return j * nA + l; },
             [&](int c) { return vld<R, V>(ldsA, (int64_t)c * K + k0); });

@@ -537,3 +537,47 @@ def test_dc_fp32_vs_oracle(gpu):
         d, c = tbpm.corr_dc(H, bes, psi, T, t_step, en, inds, chk_thr=1e-4)
         assert np.max(np.abs(d - rd)) < tol, prec
         assert np.max(np.abs(c - rc)) < tol * max(1.0, np.max(np.abs(rc))), prec
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-5)])
+def test_diagonal_hamiltonian_closed_form(gpu, prec, tol):
+    """H = diag(e) (no hoppings): C(t) = sum_i |psi_i|^2 exp(-i e_i tau t);
+    n = 1 and n = 5, with and without three terms per pass."""
+    tau = 1.3
+    bes = tbpm.bessel_series(tau, 1e-15)
+    for e in (np.array([0.37]), np.array([-0.9, -0.2, 0.0, 0.45, 0.8])):
+        n = len(e)
+        ip = np.arange(n + 1, dtype=np.int64)
+        ix = np.arange(n, dtype=np.int64)
+        rng = np.random.default_rng(n)
+        psi = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
+        psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+        T = 40
+        t = np.arange(T + 1)
+        exact = (np.abs(psi) ** 2) @ np.exp(-1j * np.outer(e, t) * tau)
+        for tiles in (None, np.array([0, n], dtype=np.int64)):
+            H = tbpm.Hamiltonian(n, ip, ix, e.astype(np.complex128), precision=prec,
+                                 tiles=tiles)
+            c = tbpm.corr_dos(H, bes, psi, T)
+            assert np.max(np.abs(c - exact)) < tol
+            H.close()
+
+
+def test_state_count_limits_and_one_step(gpu):
+    s = graphene_sample(8)
+    ip, ix, dt = _csr(s)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp32")
+    psi = _states(s.num_orb, 64, seed=40)
+    many = tbpm.corr_dos(H, bes, psi, 3)
+    for k in (0, 31, 63):
+        one = tbpm.corr_dos(H, bes, psi[k:k + 1], 3)[0]
+        assert np.max(np.abs(one - many[k])) < 1e-6
+    c1 = tbpm.corr_dos(H, bes, psi[:2], 1)
+    assert c1.shape == (2, 2)
+    assert np.max(np.abs(c1 - many[:2, :2])) < 1e-6
+    with pytest.raises(tb._lib.TBGError, match=r"\[1, 64\]"):
+        tbpm.corr_dos(H, bes, _states(s.num_orb, 65, seed=41), 3)
+    with pytest.raises(tb._lib.TBGError, match="time steps"):
+        tbpm.corr_dos(H, bes, psi[:1], 0)
+    H.close()
