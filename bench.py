@@ -221,8 +221,11 @@ def dc_workload(args):
     ts = 2 * math.pi / s.energy_range
     bes = tbpm.bessel_series(ts * s.rescale, 1e-14)
     h = _lib.default_handle(0)
-    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32", handle=h)
-    tune.apply(H, 2 * K, bes, not args.no_tune)
+    order, tiles = (s.tile_order(1.5, return_tiles=True) if args.order == "tiles"
+                    else (None, None))
+    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32", handle=h,
+                         order=order, tiles=tiles)
+    shape = tune.apply(H, 2 * K, bes, not args.no_tune)
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
     import torch
     torch.cuda.synchronize()
@@ -237,7 +240,10 @@ def dc_workload(args):
            "higher_is_better": True, "dtype": "fp32", "seconds": round(dt, 3),
            "config": {"workload": f"graphene {L}x{L}, 1% vacancies, Anderson W=1 eV, B=20 T, "
                                   f"DC at 5 energies, {K} states, {T} steps",
-                      "n_orbitals": n, "nnz": int(ix.numel()), "energies": en[inds].tolist()},
+                      "n_orbitals": n, "nnz": int(ix.numel()), "energies": en[inds].tolist(),
+                      "layout": ("caller order" if order is None else "tiles of 1.5 nm")
+                      + (", three terms per pass" if shape[2] else ", one term per pass"),
+                      "launch_shape": shape_json(shape)},
            "dc_corr_t1_mean": [float(np.mean(c[:, a, :, 0]).real) for a in range(2)]}
     print(json.dumps(out), flush=True)
 
@@ -266,8 +272,12 @@ def ac_workload(args):
     cO = tbpm.fermi_cheb_coef(2 ** 15, beta, 0.0, True, 1e-12)
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
+    # 1.5 nm tiles: 24 vectors (32 slots) per row leave room for ~100-row
+    # workgroup tiles in the LDS; the tuner keeps one term per pass if faster
+    order, tiles = (s.tile_order(1.5, return_tiles=True) if args.order == "tiles"
+                    else (None, None))
     H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32",
-                         handle=h)
+                         handle=h, order=order, tiles=tiles)
     shape = tune.apply(H, 3 * K, bes, not args.no_tune)
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
     # Fermi operators alone (T = 1 also forms one correlation step)
@@ -290,6 +300,8 @@ def ac_workload(args):
            "config": {"workload": f"graphene {L}x{L} AC, {K} states (24 vectors), {T} steps, "
                                   f"{len(bes)} Bessel terms, Fermi terms {len(cF)}",
                       "n_orbitals": n, "nnz": int(ix.numel()),
+                      "layout": ("caller order" if order is None else "tiles of 1.5 nm")
+                      + (", three terms per pass" if shape[2] else ", one term per pass"),
                       "launch_shape": shape_json(shape)},
            "corr_xx_t2_mean": complex(np.mean(c[:, 0, 1])).real}
     H.close()

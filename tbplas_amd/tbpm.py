@@ -66,6 +66,7 @@ class Hamiltonian:
         self.ordered = od is not None
         tp = None if tiles is None else np.ascontiguousarray(tiles, dtype=np.int64)
         self.tiled = tp is not None
+        self.ntiles = 0 if tp is None else len(tp) - 1
         out = ctypes.c_void_p()
         _lib.check(_lib.lib().tbg_ham_create_ordered(
             self.handle.ptr, self.n, _lib.ptr(ip), _lib.ptr(ix), _lib.ptr(hv),

@@ -4,8 +4,9 @@ For a prepared Hamiltonian, a state count and a precision it times a few
 Chebyshev steps (HIP events on the library stream, after warm-up steps) for
 every candidate (persistent blocks per CU, LDS staging of the slice rows)
 and keeps the fastest; for a Hamiltonian with spatial tiles it then times
-the best shape with three terms per pass and keeps that if it is faster.  Results are cached per (device name, library
-build, instances, rows, stored elements, widest slice, value layout, states,
+the best shape with three terms per pass and keeps that if it is faster.
+Results are cached per (device name, library build, instances, rows, stored
+elements, widest slice, value layout, row order and tile count, states,
 precision) in memory and in a JSON file
 (``$TBPLAS_AMD_TUNE_CACHE``, default ``~/.cache/tbplas_amd/tune.json``).
 """
@@ -63,8 +64,11 @@ def _key(hams, K):
     pad = sum(i["padded"] for i in infos)
     w = max(i["max_width"] for i in infos)
     real = all(i["real_values"] for i in infos)
+    # the row order and the tiles change which shape wins
+    order = "|".join(f"{int(getattr(H, 'ordered', False))}.{getattr(H, 'ntiles', 0)}"
+                     for H in hams)
     return (f"{dev}|lib={lib}|inst={len(hams)}|n={n}|pad={pad}|w={w}|real={int(real)}"
-            f"|K={K}|p={H0.precision}")
+            f"|order={order}|K={K}|p={H0.precision}")
 
 
 def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
