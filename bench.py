@@ -221,10 +221,10 @@ def dc_workload(args):
     ts = 2 * math.pi / s.energy_range
     bes = tbpm.bessel_series(ts * s.rescale, 1e-14)
     h = _lib.default_handle(0)
-    order, tiles = (s.tile_order(1.5, return_tiles=True) if args.order == "tiles"
-                    else (None, None))
-    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32", handle=h,
-                         order=order, tiles=tiles)
+    # caller order: with 32 state slots per row, 1.5 nm tiles measured no
+    # gain (the tuner kept one term per pass)
+    order = None
+    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32", handle=h)
     shape = tune.apply(H, 2 * K, bes, not args.no_tune)
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
     import torch
@@ -272,12 +272,11 @@ def ac_workload(args):
     cO = tbpm.fermi_cheb_coef(2 ** 15, beta, 0.0, True, 1e-12)
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
-    # 1.5 nm tiles: 24 vectors (32 slots) per row leave room for ~100-row
-    # workgroup tiles in the LDS; the tuner keeps one term per pass if faster
-    order, tiles = (s.tile_order(1.5, return_tiles=True) if args.order == "tiles"
-                    else (None, None))
+    # caller order: with 32 state slots per row, 1.5 nm tiles measured no
+    # gain (the tuner kept one term per pass, 302 vs 312 state-steps/s)
+    order = None
     H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32",
-                         handle=h, order=order, tiles=tiles)
+                         handle=h)
     shape = tune.apply(H, 3 * K, bes, not args.no_tune)
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
     # Fermi operators alone (T = 1 also forms one correlation step)

@@ -103,6 +103,20 @@ int tbg_get_rescale(tbg_handle *h, int64_t n, int64_t m,
                     const int64_t *hop_j, const double *hop_v,
                     double *rescale);
 
+/* Graphene supercell arrays (HOST): replaces building make_graphene_diamond
+ * (tbplas/materials/graphene.py:16-31) into a SuperCell and reading its
+ * orbitals and hops (tbplas/builder/super.py:703-809,
+ * tbplas/cython/super.pyx:383-986): same order and bit-identical values.
+ * L1 x L2 cells, lattice constant a (nm), hopping t, pbc per direction,
+ * vacancies (orbital ids before removal, may be NULL with nvac = 0).
+ * The caller sizes orb_pos for 2*L1*L2 orbitals (x, y, z), hop_i/hop_j/dr
+ * for 3*L1*L2 hops, hop_v as complex128; *n_out / *m_out get the counts. */
+int tbg_graphene_hops(int64_t L1, int64_t L2, double t, int pbc1, int pbc2,
+                      double a, const int64_t *vacancies, int64_t nvac,
+                      double *orb_pos, int64_t *hop_i, int64_t *hop_j,
+                      double *hop_v, double *dr, int64_t *n_out,
+                      int64_t *m_out);
+
 /* Replaces set_mag_field (tbplas/cython/sample.pyx:18-72): Peierls phases
  * for a field along -z, applied in place to hop_v.  HOST pointers (this is
  * O(m) host work done once per sample, kept on the host so the phases use

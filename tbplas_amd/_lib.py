@@ -54,6 +54,8 @@ _SIGS = {
     "tbg_corr_dos_batch": [_P, _I, _P, _P, _I, _P, _I, _I, _I, _D, _P,
                            ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "tbg_peierls_host": [_I64, _P, _P, _P, _P, _P, _D, _I],
+    "tbg_graphene_hops": [_I64, _I64, _D, _I, _I, _D, _P, _I64, _P, _P, _P, _P, _P,
+                          ctypes.POINTER(_I64), ctypes.POINTER(_I64)],
     "tbg_dos_begin": [_P, _I, _P, _P, _I, _P, _I, _I, ctypes.POINTER(_P)],
     "tbg_run_advance": [_P, _I, ctypes.POINTER(_D)],
     "tbg_dos_end": [_P, _I, _D, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)],

@@ -327,6 +327,88 @@ int tbg_get_rescale(tbg_handle* h, int64_t n, int64_t m, const double* orb_eng,
     });
 }
 
+// Host code: the graphene supercell of make_graphene_diamond
+// (tbplas/materials/graphene.py:16-31) in the order SuperCell builds it:
+// orbitals (ia, ib, io) (super.pyx:383-443), per cell the A->B hop then the
+// two B->A hops (check_conj, builder/base.py:76-92; build_hop_pbc,
+// super.pyx:860-986), hops crossing an open direction after all periodic
+// ones (split_pc_hop / build_hop_gen, super.pyx:643-857), positions and
+// wrapped-cell shifts as build_orb_pos / _fast_div (super.pyx:496-612),
+// same operation order, no contraction into FMA.  Vacancies drop their
+// hops and renumber the remaining orbitals.
+int tbg_graphene_hops(int64_t L1, int64_t L2, double t, int pbc1, int pbc2, double a,
+                      const int64_t* vacancies, int64_t nvac, double* orb_pos, int64_t* hop_i,
+                      int64_t* hop_j, double* hop_v, double* dr, int64_t* n_out,
+                      int64_t* m_out) {
+#pragma clang fp contract(off)
+    return guard([&] {
+        TBG_REQUIRE(L1 >= 1 && L2 >= 1 && L1 * L2 < (int64_t(1) << 40), "bad supercell size");
+        TBG_REQUIRE(orb_pos && hop_i && hop_j && hop_v && dr && n_out && m_out, "null argument");
+        TBG_REQUIRE(nvac == 0 || vacancies != nullptr, "null vacancy list");
+        const int64_t n0 = 2 * L1 * L2;
+        const double gam = 60.0 / 180 * 3.141592653589793;
+        const double lat[3][3] = {{a, 0.0, 0.0},
+                                  {a * std::cos(gam), a * std::sin(gam), 0.0},
+                                  {0.0, 0.0, 1.0}};
+        double sc1[3], sc2[3];
+        for (int k = 0; k < 3; ++k) {
+            sc1[k] = lat[0][k] * (double)L1;
+            sc2[k] = lat[1][k] * (double)L2;
+        }
+        auto pos = [&](int64_t ia, int64_t ib, int io, double* out) {
+            const double f = io ? 1 / 3. : 0.0;
+            const double ra = f + (double)ia, rb = f + (double)ib, rc = 0.0;
+            for (int k = 0; k < 3; ++k) out[k] = ra * lat[0][k] + rb * lat[1][k] + rc * lat[2][k];
+        };
+        std::vector<char> vac((size_t)n0, 0);
+        for (int64_t v = 0; v < nvac; ++v) {
+            TBG_REQUIRE(vacancies[v] >= 0 && vacancies[v] < n0, "vacancy index out of range");
+            vac[(size_t)vacancies[v]] = 1;
+        }
+        std::vector<int64_t> new_id((size_t)n0);
+        int64_t n = 0;
+        for (int64_t o = 0; o < n0; ++o) {
+            new_id[(size_t)o] = vac[(size_t)o] ? -1 : n;
+            if (!vac[(size_t)o]) {
+                pos(o / (2 * L2), (o / 2) % L2, (int)(o % 2), orb_pos + 3 * n);
+                ++n;
+            }
+        }
+        // reduced hops (ra, rb, orb_i, orb_j)
+        const int hops[3][4] = {{0, 0, 0, 1}, {1, 0, 1, 0}, {0, 1, 1, 0}};
+        const bool per[2] = {pbc1 != 0, pbc2 != 0};
+        int64_t m = 0;
+        for (int pass = 0; pass < 2; ++pass) {          // periodic hops, then free ones
+            for (int64_t ia = 0; ia < L1; ++ia)
+                for (int64_t ib = 0; ib < L2; ++ib)
+                    for (const auto& hp : hops) {
+                        const bool free = (!per[0] && hp[0] != 0) || (!per[1] && hp[1] != 0);
+                        if (free != (pass == 1)) continue;
+                        const int64_t ja0 = ia + hp[0], jb0 = ib + hp[1];
+                        if ((!per[0] && ja0 >= L1) || (!per[1] && jb0 >= L2)) continue;
+                        const int64_t na = ja0 / L1, nb = jb0 / L2;
+                        const int64_t ja = ja0 - na * L1, jb = jb0 - nb * L2;
+                        const int64_t i = (ia * L2 + ib) * 2 + hp[2];
+                        const int64_t j = (ja * L2 + jb) * 2 + hp[3];
+                        if (vac[(size_t)i] || vac[(size_t)j]) continue;
+                        double pi[3], pj[3];
+                        pos(ia, ib, hp[2], pi);
+                        pos(ja, jb, hp[3], pj);
+                        for (int k = 0; k < 3; ++k)
+                            dr[3 * m + k] = ((pj[k] - pi[k]) + (double)na * sc1[k]) +
+                                            (double)nb * sc2[k];
+                        hop_i[m] = new_id[(size_t)i];
+                        hop_j[m] = new_id[(size_t)j];
+                        hop_v[2 * m] = t;
+                        hop_v[2 * m + 1] = 0.0;
+                        ++m;
+                    }
+        }
+        *n_out = n;
+        *m_out = m;
+    });
+}
+
 // Host code: same operation order and libm cos/sin as set_mag_field
 // (tbplas/cython/sample.pyx:18-72), no contraction into FMA.
 int tbg_peierls_host(int64_t m, const int64_t* hop_i, const int64_t* hop_j, double* hop_v,

@@ -32,7 +32,8 @@ GRAPHENE_T = -2.7       # eV
 
 def graphene_hops(L1, L2, t=GRAPHENE_T, pbc=(True, True), vacancies=None,
                   a=GRAPHENE_A):
-    """Graphene supercell arrays in the reference's order, vectorised.
+    """Graphene supercell arrays in the reference's order (library host
+    code, ``tbg_graphene_hops``).
 
     Primitive cell of make_graphene_diamond (materials/graphene.py:16-31):
     lattice a at 60 deg, orbitals at (0,0) and (1/3,1/3), reduced hops
@@ -45,77 +46,23 @@ def graphene_hops(L1, L2, t=GRAPHENE_T, pbc=(True, True), vacancies=None,
     (split_pc_hop, super.pyx:786-857).
     Returns orb_pos (n,3), hop_i, hop_j, hop_v, dr (m,3).
     """
+    import ctypes
     L1, L2 = int(L1), int(L2)
-    a1 = np.array([a, 0.0, 0.0])
-    gam = 60.0 / 180 * math.pi     # gen_lattice_vectors (base/lattice.py:36-41)
-    a2 = np.array([a * math.cos(gam), a * math.sin(gam), 0.0])
-    lat = np.array([a1, a2, [0.0, 0.0, 1.0]])
-    n0 = 2 * L1 * L2
-    ia = np.repeat(np.arange(L1), L2)
-    ib = np.tile(np.arange(L2), L1)
-    # positions of all orbitals (before removing vacancies), build_orb_pos
+    n0, m0 = 2 * L1 * L2, 3 * L1 * L2
+    vac = (np.zeros(0, dtype=np.int64) if vacancies is None
+           else np.ascontiguousarray(vacancies, dtype=np.int64))
     pos = np.empty((n0, 3))
-    for io, (fa, fb) in enumerate(((0.0, 0.0), (1 / 3., 1 / 3.))):
-        ra = fa + ia
-        rb = fb + ib
-        rc = 0.0
-        pos[io::2] = (ra[:, None] * lat[0] + rb[:, None] * lat[1] + rc * lat[2])
-    sc1, sc2 = lat[0] * L1, lat[1] * L2
-    cell = ia * L2 + ib
-    pc = [(0, 0, 0, 1), (1, 0, 1, 0), (0, 1, 1, 0)]
-    per = (bool(pbc[0]), bool(pbc[1]))
-    periodic = [h for h in pc if (per[0] or h[0] == 0) and (per[1] or h[1] == 0)]
-    free = [h for h in pc if h not in periodic]
-
-    def emit(hops):
-        if not hops:
-            z = np.zeros(0, dtype=np.int64)
-            return z, z, np.zeros((0, 3)), np.zeros(0, dtype=bool)
-        ii, jj, dd, ok = [], [], [], []
-        for (ra, rb, oi, oj) in hops:
-            ja0, jb0 = ia + ra, ib + rb
-            valid = np.ones(L1 * L2, dtype=bool)
-            if not per[0]:
-                valid &= (ja0 >= 0) & (ja0 < L1)
-            if not per[1]:
-                valid &= (jb0 >= 0) & (jb0 < L2)
-            na, nb = np.floor_divide(ja0, L1), np.floor_divide(jb0, L2)
-            ja, jb = ja0 - na * L1, jb0 - nb * L2
-            i = cell * 2 + oi
-            j = (ja * L2 + jb) * 2 + oj
-            d = pos[j] - pos[i] + na[:, None] * sc1 + nb[:, None] * sc2
-            ii.append(i); jj.append(j); dd.append(d); ok.append(valid)
-        # interleave per cell: hop order inside a cell follows the list
-        k = len(hops)
-        I = np.stack(ii, axis=1).reshape(-1)
-        J = np.stack(jj, axis=1).reshape(-1)
-        D = np.stack(dd, axis=1).reshape(-1, 3)
-        V = np.stack(ok, axis=1).reshape(-1)
-        return I, J, D, V
-
-    I1, J1, D1, V1 = emit(periodic)
-    if free:
-        # build_hop_gen loops orbitals then hops with matching orb_i; all
-        # free hops of this cell start on orbital B, so cell order is kept.
-        I2, J2, D2, V2 = emit(free)
-        I = np.concatenate([I1, I2]); J = np.concatenate([J1, J2])
-        D = np.concatenate([D1, D2]); V = np.concatenate([V1, V2])
-    else:
-        I, J, D, V = I1, J1, D1, V1
-    if vacancies is not None and len(vacancies):
-        vac = np.zeros(n0, dtype=bool)
-        vac[np.asarray(vacancies, dtype=np.int64)] = True
-        V &= ~vac[I] & ~vac[J]
-        new_id = np.cumsum(~vac) - 1
-        keep_orb = ~vac
-    else:
-        new_id = np.arange(n0)
-        keep_orb = np.ones(n0, dtype=bool)
-    I, J, D = I[V], J[V], D[V]
-    hop_i = new_id[I].astype(np.int64)
-    hop_j = new_id[J].astype(np.int64)
-    hop_v = np.full(len(hop_i), t, dtype=np.complex128)
-    return pos[keep_orb].copy(), hop_i, hop_j, hop_v, np.ascontiguousarray(D)
+    hi = np.empty(m0, dtype=np.int64)
+    hj = np.empty(m0, dtype=np.int64)
+    hv = np.empty(m0, dtype=np.complex128)
+    dr = np.empty((m0, 3))
+    n, m = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(_lib.lib().tbg_graphene_hops(
+        L1, L2, float(t), int(bool(pbc[0])), int(bool(pbc[1])), float(a),
+        vac.ctypes.data if len(vac) else None, len(vac), pos.ctypes.data, hi.ctypes.data,
+        hj.ctypes.data, hv.ctypes.data, dr.ctypes.data, ctypes.byref(n), ctypes.byref(m)))
+    n, m = n.value, m.value
+    return pos[:n].copy(), hi[:m].copy(), hj[:m].copy(), hv[:m].copy(), dr[:m].copy()
 
 
 def random_vacancies(n_orb, fraction, seed):

@@ -212,6 +212,26 @@ def test_graphene_generator_matches_loop():
         np.testing.assert_array_equal(pos, a[4])
 
 
+def test_graphene_generator_vacancies_match_loop():
+    """Vacancies drop the hops that touch them and renumber the orbitals
+    (SuperCell(vacancies=...), super.py:485-510); checked against the loop
+    restatement with the same removal done afterwards."""
+    from tbplas_amd.sample import graphene_hops, random_vacancies
+    for L1, L2, pbc in ((7, 5, (True, True)), (6, 6, (False, True))):
+        vac = random_vacancies(2 * L1 * L2, 0.1, L1)
+        pos, hi, hj, hv, dr = graphene_hops(L1, L2, pbc=pbc, vacancies=vac)
+        a_hi, a_hj, a_hv, a_dr, a_pos = oracle.graphene_hops_loop(L1, L2, pbc=pbc)
+        gone = np.zeros(2 * L1 * L2, dtype=bool)
+        gone[vac] = True
+        keep = ~gone[a_hi] & ~gone[a_hj]
+        new_id = np.cumsum(~gone) - 1
+        np.testing.assert_array_equal(hi, new_id[a_hi[keep]])
+        np.testing.assert_array_equal(hj, new_id[a_hj[keep]])
+        np.testing.assert_array_equal(hv, a_hv[keep])
+        np.testing.assert_array_equal(dr, a_dr[keep])
+        np.testing.assert_array_equal(pos, a_pos[~gone])
+
+
 def test_rescale_and_peierls_loops_closed_form():
     orb, hi, hj, hv, dr, pos = _graphene(4, rescale=1.0)
     # pristine graphene: every row sums 3 |t|
