@@ -493,8 +493,13 @@ def main():
     if os.path.exists(traffic_file) and args.L == 2048 and K == 8:
         with open(traffic_file) as f:
             tr = json.load(f)
-        if tr.get("layout") == layout:
-            out["roofline"]["traffic"] = tr.get("bytes_per_term")
+        if tr.get("layout") == layout and tr.get("bytes_per_term"):
+            moved = float(tr["bytes_per_term"])
+            out["roofline"]["traffic"] = moved
+            # the bytes the counters saw per term over the same time per term
+            rate = moved / (t_step / nterm) / 1e9
+            out["roofline"]["moved_GBs"] = round(rate, 1)
+            out["roofline"]["moved_frac"] = round(rate / HBM_PEAK_GBS, 4)
 
     if not args.only_fp32:
         steps64 = args.fp64_steps or args.steps

@@ -218,6 +218,7 @@ int tbg_create(tbg_handle** out, int device) {
         TBG_HIP(hipGetDeviceProperties(&prop, device));
         h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         h->max_lds = prop.sharedMemPerBlock > 0 ? (int)prop.sharedMemPerBlock : 65536;
+        prepare_term3(h->max_lds);
         TBG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         *out = h.release();
     });

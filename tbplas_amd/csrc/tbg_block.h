@@ -67,4 +67,8 @@ template <typename R>
 void launch_term3(bool first, bool real_vals, const Term3Args& a, const Blocking& b,
                   hipStream_t s);
 
+// Lets the three-term kernels use max_lds bytes of dynamic LDS (per device,
+// once, before any launch).
+void prepare_term3(int max_lds);
+
 }  // namespace tbg

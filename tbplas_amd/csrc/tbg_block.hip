@@ -301,12 +301,36 @@ __global__ void __launch_bounds__(term3_threads<V>()) k_term3(Term3Args a) {
 template <typename R, int V, bool RV, bool FIRST>
 static void launch_f(const Term3Args& a, const Blocking& b, hipStream_t s) {
     const size_t lds = (size_t)(b.nA_max + b.nB_max) * a.K * sizeof(typename CplxT<R>::T);
-    if (lds > 65536)
-        TBG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_term3<R, V, RV, FIRST>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((k_term3<R, V, RV, FIRST>), dim3(b.nblocks), dim3(term3_threads<V>()),
                        lds, s, a);
     TBG_HIP(hipGetLastError());
+}
+
+// Allow every instance of the three-term kernel the LDS of one workgroup
+// (tables are sized up to it).  Called once per handle, outside any stream
+// capture.
+template <typename R, int V>
+static void allow_lds(int bytes) {
+    const void* f[4] = {reinterpret_cast<const void*>(&k_term3<R, V, true, true>),
+                        reinterpret_cast<const void*>(&k_term3<R, V, true, false>),
+                        reinterpret_cast<const void*>(&k_term3<R, V, false, true>),
+                        reinterpret_cast<const void*>(&k_term3<R, V, false, false>)};
+    for (const void* fn : f) {
+        // a refusal only limits the tiles: a launch beyond the default LDS
+        // then fails with an error status
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+            hipSuccess)
+            (void)hipGetLastError();
+    }
+}
+
+void prepare_term3(int max_lds) {
+    allow_lds<float, 1>(max_lds);
+    allow_lds<float, 2>(max_lds);
+    allow_lds<float, 4>(max_lds);
+    allow_lds<double, 1>(max_lds);
+    allow_lds<double, 2>(max_lds);
+    allow_lds<double, 4>(max_lds);
 }
 
 template <typename R, int V, bool RV>
