@@ -101,11 +101,12 @@ def log(*a):
 # CPU baseline: the reference Fortran, in a child process (no GPU there)
 # ---------------------------------------------------------------------------
 def cpu_child(args):
+    # numpy and the oracle only: no torch or HIP runtime (and their OpenMP
+    # runtime) in the process that times the reference's OpenMP code
     import numpy as np
     import oracle
-    from tbplas_amd.sample import graphene_hops
     L = args.L
-    pos, hi, hj, hv, dr = graphene_hops(L, L)
+    hi, hj, hv, dr, pos = oracle.graphene_hops_vec(L, L)
     n = len(pos)
     hv = hv * (1.0 / 9.0)
     ip, ix, dt = oracle.build_ham_csr(np.zeros(n), hi, hj, hv)

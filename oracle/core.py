@@ -13,7 +13,7 @@ __all__ = [
     "ref_random_state", "ref_tbpm_dos", "ref_tbpm_accond", "ref_tbpm_dccond",
     "ref_fermi_coef",
     "bessel_series", "build_ham_csr", "build_ham_dxy_loop",
-    "build_ham_dxy_stable", "graphene_hops_loop", "get_rescale_loop",
+    "build_ham_dxy_stable", "graphene_hops_loop", "graphene_hops_vec", "get_rescale_loop",
     "set_mag_field_loop", "kspace_graphene_corr", "dense_corr",
     "analyzer_calc_dos", "analyzer_calc_ac_cond", "analyzer_calc_dc_cond",
     "window_hanning", "window_exp",
@@ -497,6 +497,62 @@ def graphene_hops_loop(L1, L2, t=-2.7, pbc=(True, True), a=0.246):
     return (np.array(hi, dtype=np.int64), np.array(hj, dtype=np.int64),
             np.full(m, t, dtype=np.complex128), np.array(dr).reshape(m, 3),
             orb_pos)
+
+
+def graphene_hops_vec(L1, L2, t=-2.7, pbc=(True, True), a=0.246):
+    """graphene_hops_loop vectorised with numpy (same order and bits, pinned
+    against the loop), for sizes the loop cannot reach (the CPU baseline's
+    8.4M-orbital sample).  Returns hop_i, hop_j, hop_v, dr, orb_pos."""
+    L1, L2 = int(L1), int(L2)
+    a1 = np.array([a, 0.0, 0.0])
+    gam = 60.0 / 180 * math.pi
+    a2 = np.array([a * math.cos(gam), a * math.sin(gam), 0.0])
+    lat = np.array([a1, a2, [0.0, 0.0, 1.0]])
+    n = 2 * L1 * L2
+    ia = np.repeat(np.arange(L1), L2)
+    ib = np.tile(np.arange(L2), L1)
+    pos = np.empty((n, 3))
+    for io, (fa, fb) in enumerate(((0.0, 0.0), (1 / 3., 1 / 3.))):
+        pos[io::2] = ((fa + ia)[:, None] * lat[0] + (fb + ib)[:, None] * lat[1]
+                      + 0.0 * lat[2])
+    sc1, sc2 = lat[0] * L1, lat[1] * L2
+    cell = ia * L2 + ib
+    pc = [(0, 0, 0, 1), (1, 0, 1, 0), (0, 1, 1, 0)]
+    per = (bool(pbc[0]), bool(pbc[1]))
+    periodic = [h for h in pc if (per[0] or h[0] == 0) and (per[1] or h[1] == 0)]
+    free = [h for h in pc if h not in periodic]
+
+    def emit(hops):
+        ii, jj, dd, ok = [], [], [], []
+        for (ra, rb, oi, oj) in hops:
+            ja0, jb0 = ia + ra, ib + rb
+            valid = np.ones(L1 * L2, dtype=bool)
+            if not per[0]:
+                valid &= ja0 < L1
+            if not per[1]:
+                valid &= jb0 < L2
+            na, nb = ja0 // L1, jb0 // L2
+            ja, jb = ja0 - na * L1, jb0 - nb * L2
+            i = cell * 2 + oi
+            j = (ja * L2 + jb) * 2 + oj
+            ii.append(i)
+            jj.append(j)
+            dd.append(pos[j] - pos[i] + na[:, None] * sc1 + nb[:, None] * sc2)
+            ok.append(valid)
+        if not hops:
+            z = np.zeros(0, dtype=np.int64)
+            return z, z, np.zeros((0, 3)), np.zeros(0, dtype=bool)
+        return (np.stack(ii, 1).reshape(-1), np.stack(jj, 1).reshape(-1),
+                np.stack(dd, 1).reshape(-1, 3), np.stack(ok, 1).reshape(-1))
+
+    parts = [emit(periodic), emit(free)]
+    I = np.concatenate([p[0] for p in parts])
+    J = np.concatenate([p[1] for p in parts])
+    D = np.concatenate([p[2] for p in parts])
+    V = np.concatenate([p[3] for p in parts])
+    I, J, D = I[V], J[V], D[V]
+    return (I.astype(np.int64), J.astype(np.int64), np.full(len(I), t, dtype=np.complex128),
+            np.ascontiguousarray(D), pos)
 
 
 def get_rescale_loop(orb_eng, hop_i, hop_j, hop_v):

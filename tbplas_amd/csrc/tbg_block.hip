@@ -327,9 +327,11 @@ static void allow_lds(int bytes) {
 void prepare_term3(int max_lds) {
     allow_lds<float, 1>(max_lds);
     allow_lds<float, 2>(max_lds);
+    allow_lds<float, 3>(max_lds);
     allow_lds<float, 4>(max_lds);
     allow_lds<double, 1>(max_lds);
     allow_lds<double, 2>(max_lds);
+    allow_lds<double, 3>(max_lds);
     allow_lds<double, 4>(max_lds);
 }
 
@@ -343,7 +345,7 @@ template <typename R>
 void launch_term3(bool first, bool real_vals, const Term3Args& a, const Blocking& b,
                   hipStream_t s) {
     const int V = vectors_per_lane<R>(a.K);
-    if (a.K > 64 || (a.K & (a.K - 1)) || (1 << a.lgG) * V != a.K)
+    if (a.K > 64 || a.lgG < 0 || a.lgG > 6 || (1 << a.lgG) * V != a.K)
         throw Error{TBG_EINVAL, "internal: bad vector slot count"};
     if (b.row_bytes != a.K * (int)sizeof(typename CplxT<R>::T))
         throw Error{TBG_EINVAL, "internal: blocking built for another row size"};
@@ -355,6 +357,7 @@ void launch_term3(bool first, bool real_vals, const Term3Args& a, const Blocking
     switch (V) {
         case 1: TBG_T3(1); break;
         case 2: TBG_T3(2); break;
+        case 3: TBG_T3(3); break;
         case 4: TBG_T3(4); break;
         default: throw Error{TBG_EINVAL, "internal: unsupported vectors per lane"};
     }

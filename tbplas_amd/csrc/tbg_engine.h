@@ -38,12 +38,16 @@ inline int chunk_target(const tbg_handle* h, int64_t nslices) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(by_cu, (nslices + 3) / 4));
 }
 
-// Vector slots: a power of two >= the vectors needed (lanes per row must
-// divide the wavefront).
+// Vector slots: V * G >= the vectors needed with G (lanes per row) a power
+// of two that divides the wavefront: the smaller of the next power of two
+// and the next 3 * 2^j (three states per lane), e.g. 24 slots for the 24
+// vectors of an 8-sample AC run instead of 32.
 inline int slots_for(int k_needed) {
-    int kp = 1;
-    while (kp < k_needed) kp <<= 1;
-    return kp;
+    int p2 = 1;
+    while (p2 < k_needed) p2 <<= 1;
+    int p3 = 3;
+    while (p3 < k_needed) p3 <<= 1;
+    return p3 < p2 ? p3 : p2;
 }
 
 // Polynomial of one step: terms v_0..v_{N-1} with v_1 = alpha1 H v_0 and

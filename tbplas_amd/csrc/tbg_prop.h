@@ -50,9 +50,10 @@ struct TermArgs {
 // cost registers.  Chosen from sweeps on 8.4M-orbital graphene (best launch
 // shape each): complex64 K=8 runs 2% faster with V=4 than V=2, K=4 and K=32
 // 1-2% faster with V=2, V=8 loses 10% (occupancy 2-4 waves per SIMD);
-// complex128 uses up to 4.
+// complex128 uses up to 4.  Slot counts 3 * 2^j (slots_for) use V = 3.
 template <typename R>
 inline int vectors_per_lane(int Kp) {
+    if (Kp % 3 == 0) return 3;
     if (sizeof(R) == 8) return Kp < 4 ? Kp : 4;
     if (Kp == 8) return 4;
     return Kp < 2 ? Kp : 2;

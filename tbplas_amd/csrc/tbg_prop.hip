@@ -326,7 +326,7 @@ static void launch_term_v(int flags, const TermArgs& a, int nchunks, hipStream_t
 template <typename R>
 void launch_term(int flags, const TermArgs& a, int nchunks, hipStream_t s) {
     const int V = vectors_per_lane<R>(a.K);
-    if (a.K > 64 || (a.K & (a.K - 1)) || (1 << a.lgG) * V != a.K)
+    if (a.K > 64 || a.lgG < 0 || a.lgG > 6 || (1 << a.lgG) * V != a.K)
         throw Error{TBG_EINVAL, "internal: bad vector slot count"};
     // Stage the slice in LDS while one wave's area stays within 4 KB
     // (complex64 K <= 8, complex128 K <= 4): 16 KB per block.
@@ -345,6 +345,7 @@ void launch_term(int flags, const TermArgs& a, int nchunks, hipStream_t s) {
     switch (V) {
         case 1: TBG_TERM_V(1); break;
         case 2: TBG_TERM_V(2); break;
+        case 3: TBG_TERM_V(3); break;
         case 4: TBG_TERM_V(4); break;
         default:
             throw Error{TBG_EINVAL, "internal: unsupported vectors per lane"};

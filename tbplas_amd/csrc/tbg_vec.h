@@ -35,8 +35,11 @@ __device__ __forceinline__ Vec<R, V> vld(const void* base, int64_t idx) {
             o.re[2 * q] = t.x; o.im[2 * q] = t.y; o.re[2 * q + 1] = t.z; o.im[2 * q + 1] = t.w;
         }
     } else if constexpr (std::is_same<R, float>::value) {
-        const float2 t = reinterpret_cast<const float2*>(base)[idx];
-        o.re[0] = t.x; o.im[0] = t.y;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const float2 t = reinterpret_cast<const float2*>(base)[idx + v];
+            o.re[v] = t.x; o.im[v] = t.y;
+        }
     } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -55,7 +58,9 @@ __device__ __forceinline__ void vst(void* base, int64_t idx, const Vec<R, V>& o)
             reinterpret_cast<float4*>(base)[(idx >> 1) + q] =
                 make_float4(o.re[2 * q], o.im[2 * q], o.re[2 * q + 1], o.im[2 * q + 1]);
     } else if constexpr (std::is_same<R, float>::value) {
-        reinterpret_cast<float2*>(base)[idx] = make_float2(o.re[0], o.im[0]);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            reinterpret_cast<float2*>(base)[idx + v] = make_float2(o.re[v], o.im[v]);
     } else {
 #pragma unroll
         for (int v = 0; v < V; ++v)
@@ -84,8 +89,12 @@ __device__ __forceinline__ Vec<R, V> vld_nt(const void* base, int64_t idx) {
             o.re[2 * q] = t.x; o.im[2 * q] = t.y; o.re[2 * q + 1] = t.z; o.im[2 * q + 1] = t.w;
         }
     } else if constexpr (std::is_same<R, float>::value) {
-        const tbg_f2 t = __builtin_nontemporal_load(reinterpret_cast<const tbg_f2*>(base) + idx);
-        o.re[0] = t.x; o.im[0] = t.y;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const tbg_f2 t =
+                __builtin_nontemporal_load(reinterpret_cast<const tbg_f2*>(base) + idx + v);
+            o.re[v] = t.x; o.im[v] = t.y;
+        }
     } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -106,8 +115,11 @@ __device__ __forceinline__ void vst_nt(void* base, int64_t idx, const Vec<R, V>&
             __builtin_nontemporal_store(t, reinterpret_cast<tbg_f4*>(base) + (idx >> 1) + q);
         }
     } else if constexpr (std::is_same<R, float>::value) {
-        const tbg_f2 t = {o.re[0], o.im[0]};
-        __builtin_nontemporal_store(t, reinterpret_cast<tbg_f2*>(base) + idx);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const tbg_f2 t = {o.re[v], o.im[v]};
+            __builtin_nontemporal_store(t, reinterpret_cast<tbg_f2*>(base) + idx + v);
+        }
     } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) {

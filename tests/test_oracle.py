@@ -212,6 +212,15 @@ def test_graphene_generator_matches_loop():
         np.testing.assert_array_equal(pos, a[4])
 
 
+def test_vectorised_graphene_oracle_matches_loop():
+    for L1, L2, pbc in ((6, 6, (True, True)), (5, 7, (True, True)), (4, 5, (False, False)),
+                        (5, 4, (True, False)), (4, 6, (False, True))):
+        a = oracle.graphene_hops_loop(L1, L2, pbc=pbc)
+        b = oracle.graphene_hops_vec(L1, L2, pbc=pbc)
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+
+
 def test_graphene_generator_vacancies_match_loop():
     """Vacancies drop the hops that touch them and renumber the orbitals
     (SuperCell(vacancies=...), super.py:485-510); checked against the loop
