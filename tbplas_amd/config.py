@@ -16,6 +16,10 @@ look at:
                        (the reference's Exp(1) amplitudes)    default 'phase'
     gpu['fp32_norm_thr'] norm-check threshold used on the fp32 path, which
                        drifts ~1e-6 per hundred steps         default 1e-4
+    gpu['tile_nm']     DOS runs: store the rows in square tiles of this size
+                       (nm, from the orbital positions) so the tuner may run
+                       three Chebyshev terms per pass; same results, 0 keeps
+                       the sample's order                     default 3.5
 """
 import pickle
 
@@ -59,7 +63,8 @@ class Config:
                     'autotune': True,
                     'max_vectors': 16,
                     'state_kind': 'phase',
-                    'fp32_norm_thr': 1.0e-4}
+                    'fp32_norm_thr': 1.0e-4,
+                    'tile_nm': 3.5}
         self._legal_params = dict()
         self.set_legal_params()
 

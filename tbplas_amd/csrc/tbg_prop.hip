@@ -8,9 +8,9 @@
 //     column/value loads are fully coalesced.
 //   * K state vectors interleaved per row: element (row, k) at row*K + k,
 //     complex<R> with R = float or double.  A lane owns V consecutive vectors
-//     of one row (V = 2 for float: one 16-byte load; V = 1 for double), and
-//     G = K/V lanes share a row, so each gather x[col] is one contiguous
-//     K*sizeof(complex) segment read by G lanes.
+//     of one row (vectors_per_lane: 1 to 4), and G = K/V lanes share a row,
+//     so each gather x[col] is one contiguous K*sizeof(complex) segment read
+//     by G lanes; a wave covers its slice in G passes.
 //
 // The fused term kernel computes v_m = alpha H v_{m-1} + beta v_{m-2} for all
 // K vectors of a block of slices, writes v_m over v_{m-2} in place, and every

@@ -144,8 +144,12 @@ class Solver:
         indptr, indices, data = self._sample.build_ham_csr_device()
         num_sample = self._dist_sample()
         self._echo_time_step_fs(time_step)
+        order = tiles = None
+        tile = float(gpu.get("tile_nm", 0) or 0)
+        if tile > 0 and self._sample.orb_pos is not None:
+            order, tiles = self._sample.tile_order(tile, return_tiles=True)
         H = Hamiltonian(self._sample.num_orb, indptr, indices, data,
-                        precision=gpu["precision"], handle=h)
+                        precision=gpu["precision"], handle=h, order=order, tiles=tiles)
         psi0 = self._states(num_sample)
         chk, thr = self._chk()
         T = self._config.generic["nr_time_steps"]

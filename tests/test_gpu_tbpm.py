@@ -214,6 +214,37 @@ def test_three_term_pass_bitwise(gpu, prec, K, nb_prec):
         h.close()
 
 
+@pytest.mark.parametrize("prec,K", [("fp64", 4), ("fp32", 6)])
+def test_three_term_pass_scattered_tiles(gpu, prec, K):
+    """Tiles that are not spatial (a random row order cut every 16 rows):
+    the halo rings then reach far across the sample, which the tables must
+    still get right; the states stay bit-identical to one term per pass."""
+    s = tb.make_graphene_sample(12, vacancies=tb.random_vacancies(2 * 12 * 12, 0.05, 6))
+    tb.add_anderson_disorder(s, 0.9, 7)
+    s.set_magnetic_field(25.0, 0)
+    s.rescale_ham(9.0)
+    ip, ix, dt = _csr(s)
+    n = s.num_orb
+    bes = tbpm.bessel_series(tau_of(s), 1e-14)
+    psi = _states(n, K, seed=23)
+    h = tb._lib.Handle(0)
+    try:
+        h.set_blocking(True, 16)
+        H = tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h,
+                             order=_random_order(n, 12), tiles=_row_tiles(n, 16))
+        nblk, halo = H.blocking_info(K)
+        assert nblk == (n + 15) // 16 and halo > 2.0
+        a = tbpm.corr_dos(H, bes, psi, 25, chk_thr=1e-4)
+        h.set_blocking(False)
+        b = tbpm.corr_dos(H, bes, psi, 25, chk_thr=1e-4)
+        np.testing.assert_array_equal(a, b)
+        ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 25)
+        assert np.max(np.abs(a - ref)) < (1e-12 if prec == "fp64" else 2e-5)
+        H.close()
+    finally:
+        h.close()
+
+
 def test_three_term_pass_graph_and_step(gpu):
     """hipGraph replay of blocked steps equals launched steps; one blocked
     step (cheb_step) equals the oracle's step."""
