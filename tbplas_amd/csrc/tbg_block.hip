@@ -26,6 +26,7 @@
 // by the one-term kernel of the step's last term).
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "tbg_block.h"
 #include "tbg_vec.h"
@@ -206,6 +207,61 @@ std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_
 // ---------------------------------------------------------------------------
 // Kernel
 // ---------------------------------------------------------------------------
+// One lane's rows l = l0, l0 + step, ... < n of a phase when every row has
+// NJ <= 4 entries: the row number and the NJ table entries (index idx(j, l),
+// value at vals[j * nA + l]) of the next row are loaded before the current
+// row's work, so their latency hides behind it.  body(l, t) does the row's
+// work from its prefetched entries t (sums via tab_sum, in j order as
+// row_sum).
+template <typename R, bool RV, int NJ>
+struct RowTab {
+    int gr;
+    int col[NJ];
+    HVal<R, RV> h[NJ];
+};
+
+template <typename R, bool RV, int NJ, typename IDX, typename BODY>
+__device__ __forceinline__ void rows_prefetched(int l0, int n, int step, const int* rows,
+                                                const unsigned char* vals, int nA, IDX idx,
+                                                BODY body) {
+    auto load = [&](int l) {
+        RowTab<R, RV, NJ> t;
+        t.gr = rows[l];
+#pragma unroll
+        for (int u = 0; u < NJ; ++u) {
+            t.col[u] = idx(u, l);
+            t.h[u].load(vals, u * nA + l);
+        }
+        return t;
+    };
+    int l = l0;
+    if (l >= n) return;
+    RowTab<R, RV, NJ> cur = load(l);
+    while (true) {
+        const int ln = l + step;
+        const bool more = ln < n;
+        RowTab<R, RV, NJ> nxt;
+        if (more) nxt = load(ln);
+        body(l, cur);
+        if (!more) break;
+        cur = nxt;
+        l = ln;
+    }
+}
+
+// s = sum_j h_j x[col_j] over a prefetched row (gathers first, then the
+// products in j order).
+template <typename R, int V, bool RV, int NJ, typename GATHER>
+__device__ __forceinline__ Vec<R, V> tab_sum(const RowTab<R, RV, NJ>& t, GATHER gather) {
+    Vec<R, V> x[NJ];
+#pragma unroll
+    for (int u = 0; u < NJ; ++u) x[u] = gather(t.col[u]);
+    Vec<R, V> s = vzero<R, V>();
+#pragma unroll
+    for (int u = 0; u < NJ; ++u) mac<R, V, RV>(s, t.h[u], x[u]);
+    return s;
+}
+
 // Threads per workgroup: the tile's LDS limits a CU to about two
 // workgroups, so they are large (16 waves for one state per lane, 8
 // otherwise) to keep enough gathers in flight.  G = K/V lanes share a row,
@@ -249,6 +305,87 @@ __global__ void __launch_bounds__(term3_threads<V>()) k_term3(Term3Args a) {
     // phase A: v_m on R + E1 + E2.  The rows of B also park their v_{m-1}
     // in the (still unused) B area of the LDS for phase B: read now, next
     // to this block's gathers of the same rows, it mostly hits L2.
+    auto rowA = [&](int l, int gr, const Vec<R, V>& s) {
+        const Vec<R, V> prev = vld<R, V>(a.p0, (int64_t)gr * K + k0);
+        vst<R, V>(ldsA, (int64_t)l * K + k0, recur<R, V, true>(ar, ai, beta, s, prev));
+    };
+    auto parkA = [&](int l, int gr) {
+        if (l < nB) vst<R, V>(ldsB, (int64_t)l * K + k0, vld<R, V>(a.p1, (int64_t)gr * K + k0));
+    };
+    auto gatherA = [&](int c) { return vld<R, V>(a.p1, (int64_t)c * K + k0); };
+    // phase B: v_{m+1} on R + E1 (each lane reads its own row's parked
+    // v_{m-1} before overwriting it with v_{m+1})
+    auto rowB = [&](int l, int gr, const Vec<R, V>& s) {
+        const Vec<R, V> prev = vld<R, V>(ldsB, (int64_t)l * K + k0);
+        const Vec<R, V> v1 = recur<R, V, true>(ar, ai, beta, s, prev);
+        vst<R, V>(ldsB, (int64_t)l * K + k0, v1);
+        if (l < m.nR) vst<R, V>(a.q1, (int64_t)gr * K + k0, v1);
+    };
+    auto gatherB = [&](int c) { return vld<R, V>(ldsA, (int64_t)c * K + k0); };
+    // phase C: v_{m+2} on R, and the step output
+    auto rowC = [&](int l, int gr, const Vec<R, V>& s, const Vec<R, V>& acc0) {
+        const int64_t vidx = (int64_t)gr * K + k0;
+        const Vec<R, V> vm = vld<R, V>(ldsA, (int64_t)l * K + k0);
+        const Vec<R, V> v2 = recur<R, V, true>(ar, ai, beta, s, vm);
+        vst<R, V>(a.q2, vidx, v2);
+        const Vec<R, V> v1 = vld<R, V>(ldsB, (int64_t)l * K + k0);
+        Vec<R, V> o = acc0;
+        axpy<R, V>(o, cf[2], vm);
+        axpy<R, V>(o, cf[3], v1);
+        axpy<R, V>(o, cf[4], v2);
+        vst<R, V>(a.acc, vidx, o);
+    };
+    auto acc0 = [&](int gr) {
+        const int64_t vidx = (int64_t)gr * K + k0;
+        Vec<R, V> o;
+        if constexpr (FIRST) {
+            o = vzero<R, V>();
+            axpy<R, V>(o, cf[0], vld<R, V>(a.p0, vidx));
+            axpy<R, V>(o, cf[1], vld<R, V>(a.p1, vidx));
+        } else {
+            o = vld<R, V>(a.acc, vidx);
+        }
+        return o;
+    };
+    auto gatherC = [&](int c) { return vld<R, V>(ldsB, (int64_t)c * K + k0); };
+    auto idxG = [&](int j, int l) { return colG[j * nA + l]; };
+    auto idxL = [&](int j, int l) { return colL[j * nB + l]; };
+
+    // With two or more states per lane, rows of up to four entries (lattice
+    // models) prefetch the next row's table entries before the current
+    // row's work, so each row waits one memory round trip (+4.5% on
+    // 8.4M-orbital graphene, K=8); with one state per lane the workgroup has
+    // 16 waves to hide latency and the prefetch's registers cost more than
+    // they gain (-2%).
+    auto pipelined = [&](auto nj) {
+        constexpr int NJ = decltype(nj)::value;
+        rows_prefetched<R, RV, NJ>(rsub, nA, rpp, rows, vals, nA, idxG,
+                                   [&](int l, const RowTab<R, RV, NJ>& t) {
+            parkA(l, t.gr);
+            rowA(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherA));
+        });
+        __syncthreads();
+        rows_prefetched<R, RV, NJ>(rsub, nB, rpp, rows, vals, nA, idxL,
+                                   [&](int l, const RowTab<R, RV, NJ>& t) {
+            rowB(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherB));
+        });
+        __syncthreads();
+        rows_prefetched<R, RV, NJ>(rsub, m.nR, rpp, rows, vals, nA, idxL,
+                                   [&](int l, const RowTab<R, RV, NJ>& t) {
+            const Vec<R, V> o = acc0(t.gr);
+            rowC(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherC), o);
+        });
+    };
+    if constexpr (V >= 2) {
+        switch (w) {
+            case 1: pipelined(std::integral_constant<int, 1>{}); return;
+            case 2: pipelined(std::integral_constant<int, 2>{}); return;
+            case 3: pipelined(std::integral_constant<int, 3>{}); return;
+            case 4: pipelined(std::integral_constant<int, 4>{}); return;
+            default: break;
+        }
+    }
+    // otherwise one row at a time (phase A, B, C as above)
     for (int l = rsub; l < nA; l += rpp) {
         const int gr = rows[l];
         const Vec<R, V> prev = vld<R, V>(a.p0, (int64_t)gr * K + k0);

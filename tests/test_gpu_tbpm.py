@@ -245,6 +245,39 @@ def test_three_term_pass_scattered_tiles(gpu, prec, K):
         h.close()
 
 
+def test_three_term_pass_wide_rows(gpu):
+    """Rows of ~9 entries (a random sparse Hermitian matrix, wider than the
+    lattice rows the pipelined phase is specialised for)."""
+    rng = np.random.default_rng(31)
+    n = 400
+    m = 4 * n
+    hi = rng.integers(0, n, m)
+    hj = rng.integers(0, n, m)
+    keep = hi != hj
+    hi, hj = hi[keep], hj[keep]
+    hv = (rng.normal(size=len(hi)) + 1j * rng.normal(size=len(hi))) * 0.05
+    orb = rng.uniform(-0.2, 0.2, n)
+    ip, ix, dt = oracle.build_ham_csr(orb, hi, hj, hv)
+    assert np.max(np.diff(ip)) > 6
+    bes = tbpm.bessel_series(math.pi, 1e-14)
+    psi = _states(n, 2, seed=24)
+    h = tb._lib.Handle(0)
+    try:
+        h.set_blocking(True, 64)
+        H = tbpm.Hamiltonian(n, ip, ix, dt, precision="fp64", handle=h,
+                             tiles=_row_tiles(n, 32))
+        assert H.blocking_info(2)[0] > 1
+        a = tbpm.corr_dos(H, bes, psi, 20)
+        h.set_blocking(False)
+        b = tbpm.corr_dos(H, bes, psi, 20)
+        np.testing.assert_array_equal(a, b)
+        ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 20)
+        assert np.max(np.abs(a - ref)) < 1e-12
+        H.close()
+    finally:
+        h.close()
+
+
 def test_three_term_pass_graph_and_step(gpu):
     """hipGraph replay of blocked steps equals launched steps; one blocked
     step (cheb_step) equals the oracle's step."""
