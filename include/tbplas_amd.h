@@ -169,6 +169,12 @@ int tbg_ham_is_real(const tbg_ham *H, int *real);
  * of the three terms.  No reference counterpart. */
 int tbg_ham_blocking_info(tbg_handle *h, const tbg_ham *H, int states,
                           int precision, int *nblocks, double *halo);
+/* Launch shape of the three-term kernel for the same arguments: *max_rows =
+ * rows of the widest workgroup tile (halo included), *threads = threads
+ * per workgroup (both 0 when H runs one term per pass).  No reference
+ * counterpart. */
+int tbg_ham_blocking_launch(tbg_handle *h, const tbg_ham *H, int states,
+                            int precision, int *max_rows, int *threads);
 
 /* ---- correlation functions ----------------------------------------------
  * psi0: K initial states, complex128 (K, n) row-major (state k contiguous),

@@ -41,6 +41,7 @@ _SIGS = {
                                ctypes.POINTER(_P)],
     "tbg_set_blocking": [_P, _I, _I],
     "tbg_ham_blocking_info": [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)],
+    "tbg_ham_blocking_launch": [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "tbg_ham_destroy": [_P],
     "tbg_ham_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                      ctypes.POINTER(_I64), ctypes.POINTER(_I)],

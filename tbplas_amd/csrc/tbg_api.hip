@@ -520,6 +520,22 @@ int tbg_ham_blocking_info(tbg_handle* h, const tbg_ham* H, int states, int preci
     });
 }
 
+int tbg_ham_blocking_launch(tbg_handle* h, const tbg_ham* H, int states, int precision,
+                            int* max_rows, int* threads) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(H != nullptr && max_rows != nullptr && threads != nullptr,
+                    "null argument");
+        TBG_REQUIRE(states >= 1 && states <= 64, "states must be in [1, 64]");
+        TBG_REQUIRE(precision == TBG_FP64 || precision == TBG_FP32, "unknown precision");
+        const int Kp = slots_for(states);
+        const int eb = precision == TBG_FP32 ? 4 : 8;
+        auto b = get_blocking(h, H, Kp * 2 * eb);
+        *max_rows = b ? b->nA_max : 0;
+        *threads = b ? term3_launch_threads(eb, H->real_vals != 0, Kp, b->nA_max) : 0;
+    });
+}
+
 static void check_batch(tbg_handle* h, int n_inst, const tbg_ham* const* hams,
                         const double* bes, int n_bes, const double* const* psi0, int K, int T) {
     check_handle(h);

@@ -63,6 +63,18 @@ struct Term3Args {
     int nA_max;
 };
 
+// States per lane of the three-term kernel for K state slots: 3 for
+// multiples of 3, else 2 (1 for one slot), independent of the element type.
+// More lanes per row than the one-term kernel uses (which carries up to 4)
+// keep more gathers in flight per tile: complex64 K=8 +5.7%, complex128
+// K=8 +11%, complex64 K=16 +41% (8.4M-orbital graphene, 3.5 nm tiles).
+inline int term3_vectors_per_lane(int K) { return K % 3 == 0 ? 3 : (K < 2 ? K : 2); }
+
+// Workgroup size of the three-term kernel for K slots of elem_bytes-byte
+// reals (4 or 8), real or complex values, and tiles of at most nA rows.
+int term3_launch_threads(int elem_bytes, bool real_vals, int K, int nA);
+
+// a.lgG = log2(K / term3_vectors_per_lane(K))
 template <typename R>
 void launch_term3(bool first, bool real_vals, const Term3Args& a, const Blocking& b,
                   hipStream_t s);

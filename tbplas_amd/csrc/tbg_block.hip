@@ -262,15 +262,12 @@ __device__ __forceinline__ Vec<R, V> tab_sum(const RowTab<R, RV, NJ>& t, GATHER 
     return s;
 }
 
-// Threads per workgroup: the tile's LDS limits a CU to about two
-// workgroups, so they are large (16 waves for one state per lane, 8
-// otherwise) to keep enough gathers in flight.  G = K/V lanes share a row,
+// NT threads per workgroup: the tile's LDS limits a CU to about two
+// workgroups, so they are large (16 waves, or 8 where term3_launch_threads
+// says so) to keep enough gathers in flight.  G = K/V lanes share a row,
 // NT/G rows per pass of a phase.
-template <int V> constexpr int term3_threads() { return V == 1 ? 1024 : 512; }
-
-template <typename R, int V, bool RV, bool FIRST>
-__global__ void __launch_bounds__(term3_threads<V>()) k_term3(Term3Args a) {
-    constexpr int NT = term3_threads<V>();
+template <typename R, int V, bool RV, bool FIRST, int NT>
+__global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
     using C = typename CplxT<R>::T;
     extern __shared__ __align__(16) unsigned char tbg_smem3[];
     const int b = remap_chunk(blockIdx.x, gridDim.x);
@@ -435,41 +432,77 @@ __global__ void __launch_bounds__(term3_threads<V>()) k_term3(Term3Args a) {
     }
 }
 
+// Threads per workgroup for K state slots (V states per lane, G = K/V
+// lanes per row) over tiles of at most nA rows: one state per lane always
+// takes 16 waves (its registers allow it, and its tiles are the largest),
+// three states per lane 8 (their registers would not fit 16 waves).  With
+// two states per lane, 16 waves when the widest tile has at least one row
+// per lane for them, else 8: 16 waves over tiles with fewer rows than lanes
+// idle part of the workgroup (complex128 K=2: 502 vs 565 step·vectors/s
+// with 8 waves), while over tiles with enough rows they win (complex64
+// K=8: 1316 vs 1269).  complex128 states with complex values need more
+// than the 128 registers a lane has at 16 waves, so that pair stays at 8.
+int term3_launch_threads(int elem_bytes, bool real_vals, int K, int nA) {
+    const int V = term3_vectors_per_lane(K);
+    if (V == 1) return 1024;
+    if (V != 2 || (elem_bytes == 8 && !real_vals)) return 512;
+    return (int64_t)nA * (K / V) >= 1024 ? 1024 : 512;
+}
+
 template <typename R, int V, bool RV, bool FIRST>
 static void launch_f(const Term3Args& a, const Blocking& b, hipStream_t s) {
     const size_t lds = (size_t)(b.nA_max + b.nB_max) * a.K * sizeof(typename CplxT<R>::T);
-    hipLaunchKernelGGL((k_term3<R, V, RV, FIRST>), dim3(b.nblocks), dim3(term3_threads<V>()),
-                       lds, s, a);
-    TBG_HIP(hipGetLastError());
+    const int nt = term3_launch_threads((int)sizeof(R), RV, a.K, b.nA_max);
+    // the instances term3_launch_threads can ask for
+    constexpr bool wide = V == 1 || (V == 2 && (sizeof(R) == 4 || RV));
+    if constexpr (wide) {
+        if (nt == 1024) {
+            hipLaunchKernelGGL((k_term3<R, V, RV, FIRST, 1024>), dim3(b.nblocks), dim3(1024),
+                               lds, s, a);
+            TBG_HIP(hipGetLastError());
+            return;
+        }
+    }
+    if constexpr (V >= 2) {
+        if (nt == 512) {
+            hipLaunchKernelGGL((k_term3<R, V, RV, FIRST, 512>), dim3(b.nblocks), dim3(512), lds,
+                               s, a);
+            TBG_HIP(hipGetLastError());
+            return;
+        }
+    }
+    throw Error{TBG_EINVAL, "internal: no three-term kernel for this workgroup size"};
 }
 
 // Allow every instance of the three-term kernel the LDS of one workgroup
 // (tables are sized up to it).  Called once per handle, outside any stream
 // capture.
-template <typename R, int V>
+static void allow_lds_fn(const void* fn, int bytes) {
+    // a refusal only limits the tiles: a launch beyond the default LDS
+    // then fails with an error status
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
+        (void)hipGetLastError();
+}
+
+template <typename R, int V, int NT>
 static void allow_lds(int bytes) {
-    const void* f[4] = {reinterpret_cast<const void*>(&k_term3<R, V, true, true>),
-                        reinterpret_cast<const void*>(&k_term3<R, V, true, false>),
-                        reinterpret_cast<const void*>(&k_term3<R, V, false, true>),
-                        reinterpret_cast<const void*>(&k_term3<R, V, false, false>)};
-    for (const void* fn : f) {
-        // a refusal only limits the tiles: a launch beyond the default LDS
-        // then fails with an error status
-        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
-            hipSuccess)
-            (void)hipGetLastError();
+    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, true, true, NT>), bytes);
+    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, true, false, NT>), bytes);
+    if constexpr (!(NT == 1024 && V == 2 && sizeof(R) == 8)) {   // else not instantiated
+        allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, false, true, NT>), bytes);
+        allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, false, false, NT>), bytes);
     }
 }
 
 void prepare_term3(int max_lds) {
-    allow_lds<float, 1>(max_lds);
-    allow_lds<float, 2>(max_lds);
-    allow_lds<float, 3>(max_lds);
-    allow_lds<float, 4>(max_lds);
-    allow_lds<double, 1>(max_lds);
-    allow_lds<double, 2>(max_lds);
-    allow_lds<double, 3>(max_lds);
-    allow_lds<double, 4>(max_lds);
+    allow_lds<float, 1, 1024>(max_lds);
+    allow_lds<float, 2, 1024>(max_lds);
+    allow_lds<float, 2, 512>(max_lds);
+    allow_lds<float, 3, 512>(max_lds);
+    allow_lds<double, 1, 1024>(max_lds);
+    allow_lds<double, 2, 1024>(max_lds);
+    allow_lds<double, 2, 512>(max_lds);
+    allow_lds<double, 3, 512>(max_lds);
 }
 
 template <typename R, int V, bool RV>
@@ -481,7 +514,7 @@ static void launch_v(bool first, const Term3Args& a, const Blocking& b, hipStrea
 template <typename R>
 void launch_term3(bool first, bool real_vals, const Term3Args& a, const Blocking& b,
                   hipStream_t s) {
-    const int V = vectors_per_lane<R>(a.K);
+    const int V = term3_vectors_per_lane(a.K);
     if (a.K > 64 || a.lgG < 0 || a.lgG > 6 || (1 << a.lgG) * V != a.K)
         throw Error{TBG_EINVAL, "internal: bad vector slot count"};
     if (b.row_bytes != a.K * (int)sizeof(typename CplxT<R>::T))
@@ -495,7 +528,6 @@ void launch_term3(bool first, bool real_vals, const Term3Args& a, const Blocking
         case 1: TBG_T3(1); break;
         case 2: TBG_T3(2); break;
         case 3: TBG_T3(3); break;
-        case 4: TBG_T3(4); break;
         default: throw Error{TBG_EINVAL, "internal: unsupported vectors per lane"};
     }
 #undef TBG_T3

@@ -87,7 +87,7 @@ struct Engine {
     tbg_handle* h = nullptr;
     MatView M;
     Plan plan;
-    int Kp = 1, lgG = 0;
+    int Kp = 1, lgG = 0, lgG3 = 0;   // lanes per row: log2, one-term and three-term kernels
     int64_t n_pad = 0;
     DevBuf X, P, Y, part;
     size_t vbytes = 0;
@@ -105,6 +105,8 @@ struct Engine {
         const int V = vectors_per_lane<R>(Kp);
         lgG = 0;
         while ((1 << lgG) * V < Kp) ++lgG;
+        lgG3 = 0;
+        while ((1 << lgG3) * term3_vectors_per_lane(Kp) < Kp) ++lgG3;
         make_plan(h, plan, inst_slices, chunk_target);
         n_pad = m.nslices * 64;
         vbytes = (size_t)n_pad * Kp * sizeof(typename CplxT<R>::T);
@@ -188,7 +190,7 @@ struct Engine {
             t.bwd_mask = bwd;
             t.cls_mask = cls;
             t.K = Kp;
-            t.lgG = lgG;
+            t.lgG = lgG3;
             t.nA_max = blk->nA_max;
             launch_term3<R>(i == 0, M.real_vals != 0, t, *blk, h->stream);
             std::swap(b0, b2);

@@ -101,6 +101,17 @@ class Hamiltonian:
                                                     ctypes.byref(nb), ctypes.byref(halo)))
         return nb.value, halo.value
 
+    def blocking_launch(self, states, precision=None):
+        """(rows of the widest workgroup tile, threads per workgroup) of the
+        three-term pass for ``states`` vectors; (0, 0) when H runs one term
+        per pass."""
+        rows, nt = ctypes.c_int(), ctypes.c_int()
+        prec = self.precision if precision is None else precision_code(precision)
+        _lib.check(_lib.lib().tbg_ham_blocking_launch(self.handle.ptr, self._h, int(states),
+                                                      prec, ctypes.byref(rows),
+                                                      ctypes.byref(nt)))
+        return rows.value, nt.value
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.lib().tbg_ham_destroy(self._h)

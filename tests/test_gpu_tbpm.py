@@ -278,6 +278,44 @@ def test_three_term_pass_wide_rows(gpu):
         h.close()
 
 
+@pytest.mark.parametrize("prec,K,field,threads", [("fp32", 8, 0.0, 1024), ("fp32", 16, 0.0, 1024),
+                                                  ("fp64", 8, 0.0, 1024), ("fp64", 2, 0.0, 512),
+                                                  ("fp64", 8, 30.0, 512)])
+def test_three_term_pass_workgroup_sizes(gpu, prec, K, field, threads):
+    """Tiles of the benchmark's size (3.5 nm on 131,072-orbital graphene):
+    the three-term kernel runs with 16 waves per workgroup where the widest
+    tile gives every lane a row (real values), with 8 for complex128 K=2
+    (too few rows) and for complex values in complex128 (registers); the
+    states stay bit-identical to one term per pass either way."""
+    s = tb.make_graphene_sample(256)
+    tb.add_anderson_disorder(s, 0.4, 15)
+    if field:
+        s.set_magnetic_field(field, 0)
+    s.rescale_ham(9.0)
+    ip, ix, dt = _csr(s)
+    n = s.num_orb
+    order, tiles = s.tile_order(3.5, return_tiles=True)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(n, K, seed=27)
+    h = tb._lib.Handle(0)
+    try:
+        H = tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h, order=order,
+                             tiles=tiles)
+        assert H.info()["real_values"] == (field == 0.0)
+        rows, nt = H.blocking_launch(K)
+        assert nt == threads and rows > 0
+        a = tbpm.corr_dos(H, bes, psi, 12)
+        h.set_blocking(False)
+        assert H.blocking_launch(K) == (0, 0)
+        b = tbpm.corr_dos(H, bes, psi, 12)
+        np.testing.assert_array_equal(a, b)
+        ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi[:1], 12)
+        assert np.max(np.abs(a[:1] - ref)) < (1e-12 if prec == "fp64" else 2e-5)
+        H.close()
+    finally:
+        h.close()
+
+
 def test_three_term_pass_graph_and_step(gpu):
     """hipGraph replay of blocked steps equals launched steps; one blocked
     step (cheb_step) equals the oracle's step."""
