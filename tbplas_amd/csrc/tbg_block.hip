@@ -436,17 +436,18 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
 // lanes per row) over tiles of at most nA rows: one state per lane always
 // takes 16 waves (its registers allow it, and its tiles are the largest),
 // three states per lane 8 (their registers would not fit 16 waves).  With
-// two states per lane, 16 waves when the widest tile has at least one row
-// per lane for them, else 8: 16 waves over tiles with fewer rows than lanes
-// idle part of the workgroup (complex128 K=2: 502 vs 565 step·vectors/s
-// with 8 waves), while over tiles with enough rows they win (complex64
-// K=8: 1316 vs 1269).  complex128 states with complex values need more
-// than the 128 registers a lane has at 16 waves, so that pair stays at 8.
+// two states per lane, 16 waves when the widest tile gives every lane two
+// rows (nA * G >= 2048), else 8.  Measured on 8.4M-orbital graphene, 3.5 nm
+// tiles, step·vectors/s with 8 / 16 waves (nA * G): complex64 K=2 989 / 999
+// (2088), K=4 1148 / 1142 (2118), K=8 1278 / 1327 (2492), K=16 959 / 1352
+// (4984); complex128 K=2 564 / 501 (1059), K=4 610 / 589 (1246), K=8
+// 632 / 643 (2492).  complex128 states with complex values need more than
+// the 128 registers a lane has at 16 waves, so that pair stays at 8.
 int term3_launch_threads(int elem_bytes, bool real_vals, int K, int nA) {
     const int V = term3_vectors_per_lane(K);
     if (V == 1) return 1024;
     if (V != 2 || (elem_bytes == 8 && !real_vals)) return 512;
-    return (int64_t)nA * (K / V) >= 1024 ? 1024 : 512;
+    return (int64_t)nA * (K / V) >= 2048 ? 1024 : 512;
 }
 
 template <typename R, int V, bool RV, bool FIRST>

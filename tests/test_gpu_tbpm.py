@@ -278,15 +278,14 @@ def test_three_term_pass_wide_rows(gpu):
         h.close()
 
 
-@pytest.mark.parametrize("prec,K,field,threads", [("fp32", 8, 0.0, 1024), ("fp32", 16, 0.0, 1024),
-                                                  ("fp64", 8, 0.0, 1024), ("fp64", 2, 0.0, 512),
-                                                  ("fp64", 8, 30.0, 512)])
-def test_three_term_pass_workgroup_sizes(gpu, prec, K, field, threads):
+@pytest.mark.parametrize("prec,K,field", [("fp32", 8, 0.0), ("fp32", 16, 0.0), ("fp64", 8, 0.0),
+                                          ("fp64", 2, 0.0), ("fp64", 8, 30.0)])
+def test_three_term_pass_workgroup_sizes(gpu, prec, K, field):
     """Tiles of the benchmark's size (3.5 nm on 131,072-orbital graphene):
     the three-term kernel runs with 16 waves per workgroup where the widest
-    tile gives every lane a row (real values), with 8 for complex128 K=2
-    (too few rows) and for complex values in complex128 (registers); the
-    states stay bit-identical to one term per pass either way."""
+    tile gives every lane (K/2 per row) two rows, else with 8, and always
+    with 8 for complex values in complex128 (registers); the states stay
+    bit-identical to one term per pass either way."""
     s = tb.make_graphene_sample(256)
     tb.add_anderson_disorder(s, 0.4, 15)
     if field:
@@ -303,7 +302,10 @@ def test_three_term_pass_workgroup_sizes(gpu, prec, K, field, threads):
                              tiles=tiles)
         assert H.info()["real_values"] == (field == 0.0)
         rows, nt = H.blocking_launch(K)
-        assert nt == threads and rows > 0
+        wide = prec == "fp32" or field == 0.0
+        assert rows > 0 and nt == (1024 if wide and rows * (K // 2) >= 2048 else 512)
+        if (prec, K) == ("fp32", 16) or (prec, K, field) == ("fp64", 2, 0.0):
+            assert nt == (1024 if K == 16 else 512)     # both sizes are exercised
         a = tbpm.corr_dos(H, bes, psi, 12)
         h.set_blocking(False)
         assert H.blocking_launch(K) == (0, 0)
