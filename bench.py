@@ -69,6 +69,9 @@ def parse():
                         "tiles of --tile nm (Sample.tile_order), which also lets the "
                         "tuner choose three Chebyshev terms per pass")
     p.add_argument("--tile", type=float, default=3.5)
+    p.add_argument("--cond-tile", type=float, default=1.5,
+                   help="dc and ac workloads: tile edge in nm of a spatial row order "
+                        "(0: the sample's own order)")
     p.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
 
@@ -202,6 +205,14 @@ def gpu_run(H, bes, psi, warmup, steps):
     return ms, corr
 
 
+def cond_order(s, args):
+    """(row order, tile offsets) for the dc and ac workloads: square tiles
+    of --cond-tile nm, or the sample's own order."""
+    if args.cond_tile > 0:
+        return s.tile_order(args.cond_tile, return_tiles=True)
+    return None, None
+
+
 def dc_workload(args):
     """Graphene 1024x1024, 1% vacancies, Anderson W = 1 eV (U[-0.5, 0.5]),
     Peierls phases for B = 20 T (gauge 0), rescale 9; DC correlation at the 5
@@ -222,10 +233,9 @@ def dc_workload(args):
     ts = 2 * math.pi / s.energy_range
     bes = tbpm.bessel_series(ts * s.rescale, 1e-14)
     h = _lib.default_handle(0)
-    # caller order: with 32 state slots per row, 1.5 nm tiles measured no
-    # gain (the tuner kept one term per pass)
-    order = None
-    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32", handle=h)
+    order, tiles = cond_order(s, args)
+    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32", handle=h,
+                         order=order, tiles=tiles)
     shape = tune.apply(H, 2 * K, bes, not args.no_tune)
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
     import torch
@@ -242,7 +252,8 @@ def dc_workload(args):
            "config": {"workload": f"graphene {L}x{L}, 1% vacancies, Anderson W=1 eV, B=20 T, "
                                   f"DC at 5 energies, {K} states, {T} steps",
                       "n_orbitals": n, "nnz": int(ix.numel()), "energies": en[inds].tolist(),
-                      "layout": ("caller order" if order is None else "tiles of 1.5 nm")
+                      "layout": ("caller order" if order is None
+                                 else f"tiles of {args.cond_tile:g} nm")
                       + (", three terms per pass" if shape[2] else ", one term per pass"),
                       "launch_shape": shape_json(shape)},
            "dc_corr_t1_mean": [float(np.mean(c[:, a, :, 0]).real) for a in range(2)]}
@@ -273,11 +284,9 @@ def ac_workload(args):
     cO = tbpm.fermi_cheb_coef(2 ** 15, beta, 0.0, True, 1e-12)
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
-    # caller order: with 32 state slots per row, 1.5 nm tiles measured no
-    # gain (the tuner kept one term per pass, 302 vs 312 state-steps/s)
-    order = None
+    order, tiles = cond_order(s, args)
     H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32",
-                         handle=h)
+                         handle=h, order=order, tiles=tiles)
     shape = tune.apply(H, 3 * K, bes, not args.no_tune)
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
     # Fermi operators alone (T = 1 also forms one correlation step)
@@ -300,7 +309,8 @@ def ac_workload(args):
            "config": {"workload": f"graphene {L}x{L} AC, {K} states (24 vectors), {T} steps, "
                                   f"{len(bes)} Bessel terms, Fermi terms {len(cF)}",
                       "n_orbitals": n, "nnz": int(ix.numel()),
-                      "layout": ("caller order" if order is None else "tiles of 1.5 nm")
+                      "layout": ("caller order" if order is None
+                                 else f"tiles of {args.cond_tile:g} nm")
                       + (", three terms per pass" if shape[2] else ", one term per pass"),
                       "launch_shape": shape_json(shape)},
            "corr_xx_t2_mean": complex(np.mean(c[:, 0, 1])).real}

@@ -20,6 +20,8 @@ look at:
                        (nm, from the orbital positions) so the tuner may run
                        three Chebyshev terms per pass; same results, 0 keeps
                        the sample's order                     default 3.5
+    gpu['cond_tile_nm'] the same for AC and DC runs (more vectors per row,
+                       so smaller tiles fit the LDS)          default 1.5
 """
 import pickle
 
@@ -64,7 +66,8 @@ class Config:
                     'max_vectors': 16,
                     'state_kind': 'phase',
                     'fp32_norm_thr': 1.0e-4,
-                    'tile_nm': 3.5}
+                    'tile_nm': 3.5,
+                    'cond_tile_nm': 1.5}
         self._legal_params = dict()
         self.set_legal_params()
 

@@ -133,6 +133,14 @@ class Solver:
             yield i, min(total, i + per)
             i += per
 
+    def _row_order(self, gpu, key):
+        """(row order, tile offsets) of square tiles of gpu[key] nm when the
+        sample has orbital positions, else (None, None)."""
+        tile = float(gpu.get(key, 0) or 0)
+        if tile > 0 and self._sample.orb_pos is not None:
+            return self._sample.tile_order(tile, return_tiles=True)
+        return None, None
+
     # -- correlation functions -------------------------------------------------
     def calc_corr_dos(self):
         """(nr_time_steps+1,) complex128 DOS correlation (solver.py:256-284)."""
@@ -144,10 +152,7 @@ class Solver:
         indptr, indices, data = self._sample.build_ham_csr_device()
         num_sample = self._dist_sample()
         self._echo_time_step_fs(time_step)
-        order = tiles = None
-        tile = float(gpu.get("tile_nm", 0) or 0)
-        if tile > 0 and self._sample.orb_pos is not None:
-            order, tiles = self._sample.tile_order(tile, return_tiles=True)
+        order, tiles = self._row_order(gpu, "tile_nm")
         H = Hamiltonian(self._sample.num_orb, indptr, indices, data,
                         precision=gpu["precision"], handle=h, order=order, tiles=tiles)
         psi0 = self._states(num_sample)
@@ -180,9 +185,10 @@ class Solver:
                                  g["Fermi_cheb_precision"])
         coef_omF = fermi_cheb_coef(g["nr_Fermi_fft_steps"], beta_re, mu_re, True,
                                    g["Fermi_cheb_precision"])
+        order, tiles = self._row_order(gpu, "cond_tile_nm")
         H = Hamiltonian(self._sample.num_orb, indptr, indices, hop, dx, dy,
                         cur_scale=self._sample.rescale, precision=gpu["precision"],
-                        handle=h)
+                        handle=h, order=order, tiles=tiles)
         psi0 = self._states(num_sample)
         chk, thr = self._chk()
         T = g["nr_time_steps"]
@@ -215,9 +221,10 @@ class Solver:
         indptr, indices, hop, dx, dy = self._sample.build_ham_dxy_device()
         num_sample = self._dist_sample()
         self._echo_time_step_fs(time_step)
+        order, tiles = self._row_order(gpu, "cond_tile_nm")
         H = Hamiltonian(self._sample.num_orb, indptr, indices, hop, dx, dy,
                         cur_scale=self._sample.rescale, precision=gpu["precision"],
-                        handle=h)
+                        handle=h, order=order, tiles=tiles)
         psi0 = self._states(num_sample)
         chk, thr = self._chk()
         per = max(1, min(32, int(gpu["max_vectors"])))
