@@ -102,19 +102,21 @@ struct DosRun : tbg_run {
         TBG_HIP(hipMemsetAsync(counter.p, 0, sizeof(int), h->stream));
         // t = 0: <psi0|psi0> (tbpm.f90:58-62)
         launch_dotnorm<R>(e.plan, psi.p, psi.p, Kp, Kp, e.part.template as<double>(), h->stream);
-        reduce();
+        reduce(e.plan.inst_chunk_dev.template as<int>());
         sp = time_poly(bes, nb);
         TBG_HIP(hipStreamSynchronize(h->stream));
     }
 
-    void reduce() {
-        launch_reduce(e.part.template as<double>(), nv, e.plan.inst_chunk_dev.template as<int>(),
-                      n_inst, cbuf.as<double>(), stride, counter.as<int>(), 0, h->stream);
+    // sums the partial rows of ranges[inst] .. ranges[inst + 1] per instance
+    void reduce(const int* ranges) {
+        launch_reduce(e.part.template as<double>(), nv, ranges, n_inst, cbuf.as<double>(), stride,
+                      counter.as<int>(), 0, h->stream);
     }
 
     void advance(int nsteps) override {
         TBG_REQUIRE(nsteps >= 0 && done + nsteps <= T, "advancing past the last time step");
-        e.run(nsteps, sp, 0, 0, F_DOT | F_NORM, psi.p, counter.as<int>(), [&]() { reduce(); });
+        e.run(nsteps, sp, 0, 0, F_DOT | F_NORM, psi.p, counter.as<int>(),
+              [&]() { reduce(e.epi_range()); });
         done += nsteps;
     }
 

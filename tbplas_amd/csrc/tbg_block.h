@@ -39,11 +39,23 @@ struct Blocking {
 // a block-tile would not fit the LDS of one workgroup.
 std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_bytes);
 
+// Kinds of pass over the block-tiles (k_term3):
+//   P3_NORMAL  terms m, m+1, m+2: acc += c_m v_m + c_{m+1} v_{m+1} + c_{m+2} v_{m+2}
+//   P3_FIRST   the same with acc = c_{m-2} v_{m-2} + c_{m-1} v_{m-1} + ... (no read)
+//   P3_M1      terms 1, 2, 3 from v_0 (p1): v_1 = alpha_1 H v_0 (ar1, ai1), no
+//              v_{-1}; acc = c_0 v_0 + c_1 v_1 + c_2 v_2 + c_3 v_3 (c[1..4]);
+//              bumps *counter as the one-term kernel's m = 1 launch does
+//   P2_*       the step's last terms m, m+1: acc += c_m v_m + c_{m+1} v_{m+1},
+//              nothing else written, with no epilogue (P2_LAST), the FP64
+//              norm (P2_NORM) or the FP64 dot with psi0 and the norm
+//              (P2_DOTNORM) summed per tile into part[tile][nv] (layout of
+//              the one-term kernel's F_DOT / F_NORM partial rows)
+enum PassMode : int { P3_NORMAL = 0, P3_FIRST, P3_M1, P2_LAST, P2_NORM, P2_DOTNORM };
+
 // Arguments of the three-term kernel: from v_{m-2} (p0) and v_{m-1} (p1) it
 // forms v_m, v_{m+1}, v_{m+2} on each block-tile (v_m and v_{m+1} on the
 // halo rings, in LDS), writes v_{m+1} to q1 and v_{m+2} to q2 for the owned
-// rows, and adds c_m v_m + c_{m+1} v_{m+1} + c_{m+2} v_{m+2} to acc (first:
-// acc = c_{m-2} v_{m-2} + c_{m-1} v_{m-1} + ..., no read).
+// rows, and updates the step output acc (see PassMode).
 struct Term3Args {
     const BlockMeta* meta;
     const int* rows;
@@ -56,11 +68,16 @@ struct Term3Args {
     void* q2;
     void* acc;
     double ar[2], ai[2];
+    double ar1[2], ai1[2];   // alpha of the first term (P3_M1)
     double beta;
     double c[5][2];          // c_{m-2} .. c_{m+2} per coefficient class
     uint64_t bwd_mask, cls_mask;
     int K, lgG;
     int nA_max;
+    int* counter;            // P3_M1
+    const void* psi0;        // P2_DOTNORM
+    double* part;            // P2_NORM, P2_DOTNORM
+    int nv;                  // values per partial row (3 K)
 };
 
 // States per lane of the three-term kernel for K state slots: 3 for
@@ -76,7 +93,7 @@ int term3_launch_threads(int elem_bytes, bool real_vals, int K, int nA);
 
 // a.lgG = log2(K / term3_vectors_per_lane(K))
 template <typename R>
-void launch_term3(bool first, bool real_vals, const Term3Args& a, const Blocking& b,
+void launch_term3(int mode, bool real_vals, const Term3Args& a, const Blocking& b,
                   hipStream_t s);
 
 // Lets the three-term kernels use max_lds bytes of dynamic LDS (per device,

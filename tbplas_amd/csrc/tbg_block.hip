@@ -266,11 +266,27 @@ __device__ __forceinline__ Vec<R, V> tab_sum(const RowTab<R, RV, NJ>& t, GATHER 
 // workgroups, so they are large (16 waves, or 8 where term3_launch_threads
 // says so) to keep enough gathers in flight.  G = K/V lanes share a row,
 // NT/G rows per pass of a phase.
-template <typename R, int V, bool RV, bool FIRST, int NT>
+//
+// MODE (PassMode): three terms m, m+1, m+2 (phase A: v_m on R+E1+E2 from
+// v_{m-1} gathers in HBM/L2 and v_{m-2}; phase B: v_{m+1} on R+E1 from
+// LDS; phase C: v_{m+2} on R, the step output) with the output read
+// (P3_NORMAL), formed from v_{m-2} and v_{m-1} (P3_FIRST), or, for
+// m = 1, formed from v_0 with the first term's coefficient and no v_{-1}
+// (P3_M1); or the step's last two terms (P2_*: v_m on R+E1, v_{m+1} on R,
+// only the step output written) with no epilogue, the FP64 norm, or the
+// FP64 dot with psi0 and the norm, summed per tile into part.
+template <typename R, int V, bool RV, int MODE, int NT>
 __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
     using C = typename CplxT<R>::T;
+    constexpr bool PAIR = MODE >= P2_LAST;
+    constexpr bool NORM = MODE == P2_NORM || MODE == P2_DOTNORM;
+    constexpr bool DOT = MODE == P2_DOTNORM;
     extern __shared__ __align__(16) unsigned char tbg_smem3[];
     const int b = remap_chunk(blockIdx.x, gridDim.x);
+    if constexpr (MODE == P3_M1) {
+        // the step counter the last term's epilogue and the reductions read
+        if (a.counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.counter += 1;
+    }
     const BlockMeta m = a.meta[b];
     const int* rows = a.rows + m.offA;
     const int* colL = a.colL + m.offL;
@@ -285,29 +301,46 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
     const int k0 = g * V;
 
     R ar[V], ai[V], cf[5][V];
+    R ar1[MODE == P3_M1 ? V : 1], ai1[MODE == P3_M1 ? V : 1];
 #pragma unroll
     for (int v = 0; v < V; ++v) {
         const int k = k0 + v;
         const int cl = (int)((a.cls_mask >> k) & 1ull);
+        const bool bwd = ((a.bwd_mask >> k) & 1ull) != 0;
         ar[v] = (R)a.ar[cl];
-        ai[v] = ((a.bwd_mask >> k) & 1ull) ? (R)(-a.ai[cl]) : (R)a.ai[cl];
+        ai[v] = bwd ? (R)(-a.ai[cl]) : (R)a.ai[cl];
+        if constexpr (MODE == P3_M1) {
+            ar1[v] = (R)a.ar1[cl];
+            ai1[v] = bwd ? (R)(-a.ai1[cl]) : (R)a.ai1[cl];
+        }
 #pragma unroll
         for (int q = 0; q < 5; ++q) cf[q][v] = (R)a.c[q][cl];
     }
     const R beta = (R)a.beta;
     constexpr size_t vsz = RV ? sizeof(R) : sizeof(C);
     const unsigned char* vals = reinterpret_cast<const unsigned char*>(a.valG) + m.offG * vsz;
-    const int w = m.w, nA = m.nA, nB = m.nB;
+    const int w = m.w, nA = m.nA, nB = m.nB, nR = m.nR;
+    // rows phase A computes, and rows whose v_{m-1} it parks for the next
+    // phase: the pair needs one ring less
+    const int nAx = PAIR ? nB : nA;
+    const int nPark = PAIR ? nR : nB;
 
-    // phase A: v_m on R + E1 + E2.  The rows of B also park their v_{m-1}
-    // in the (still unused) B area of the LDS for phase B: read now, next
-    // to this block's gathers of the same rows, it mostly hits L2.
+    // phase A: v_m from the gathered v_{m-1}.  The rows of the next phase
+    // also park their v_{m-1} in the (still unused) B area of the LDS: read
+    // now, next to this block's gathers of the same rows, it mostly hits L2.
     auto rowA = [&](int l, int gr, const Vec<R, V>& s) {
-        const Vec<R, V> prev = vld<R, V>(a.p0, (int64_t)gr * K + k0);
-        vst<R, V>(ldsA, (int64_t)l * K + k0, recur<R, V, true>(ar, ai, beta, s, prev));
+        Vec<R, V> v;
+        if constexpr (MODE == P3_M1) {
+            v = recur<R, V, false>(ar1, ai1, beta, s, s);
+        } else {
+            const Vec<R, V> prev = vld<R, V>(a.p0, (int64_t)gr * K + k0);
+            v = recur<R, V, true>(ar, ai, beta, s, prev);
+        }
+        vst<R, V>(ldsA, (int64_t)l * K + k0, v);
     };
     auto parkA = [&](int l, int gr) {
-        if (l < nB) vst<R, V>(ldsB, (int64_t)l * K + k0, vld<R, V>(a.p1, (int64_t)gr * K + k0));
+        if (l < nPark)
+            vst<R, V>(ldsB, (int64_t)l * K + k0, vld<R, V>(a.p1, (int64_t)gr * K + k0));
     };
     auto gatherA = [&](int c) { return vld<R, V>(a.p1, (int64_t)c * K + k0); };
     // phase B: v_{m+1} on R + E1 (each lane reads its own row's parked
@@ -316,7 +349,7 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
         const Vec<R, V> prev = vld<R, V>(ldsB, (int64_t)l * K + k0);
         const Vec<R, V> v1 = recur<R, V, true>(ar, ai, beta, s, prev);
         vst<R, V>(ldsB, (int64_t)l * K + k0, v1);
-        if (l < m.nR) vst<R, V>(a.q1, (int64_t)gr * K + k0, v1);
+        if (l < nR) vst<R, V>(a.q1, (int64_t)gr * K + k0, v1);
     };
     auto gatherB = [&](int c) { return vld<R, V>(ldsA, (int64_t)c * K + k0); };
     // phase C: v_{m+2} on R, and the step output
@@ -335,9 +368,12 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
     auto acc0 = [&](int gr) {
         const int64_t vidx = (int64_t)gr * K + k0;
         Vec<R, V> o;
-        if constexpr (FIRST) {
+        if constexpr (MODE == P3_FIRST) {
             o = vzero<R, V>();
             axpy<R, V>(o, cf[0], vld<R, V>(a.p0, vidx));
+            axpy<R, V>(o, cf[1], vld<R, V>(a.p1, vidx));
+        } else if constexpr (MODE == P3_M1) {
+            o = vzero<R, V>();
             axpy<R, V>(o, cf[1], vld<R, V>(a.p1, vidx));
         } else {
             o = vld<R, V>(a.acc, vidx);
@@ -345,6 +381,38 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
         return o;
     };
     auto gatherC = [&](int c) { return vld<R, V>(ldsB, (int64_t)c * K + k0); };
+    // last two terms: v_{m+1} on R from v_m (LDS A) and the parked v_{m-1};
+    // the step output and its FP64 sums
+    double dre[V], dim_[V], nrm[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) { dre[v] = 0.0; dim_[v] = 0.0; nrm[v] = 0.0; }
+    auto rowP = [&](int l, int gr, const Vec<R, V>& s, const Vec<R, V>& acc0) {
+        const int64_t vidx = (int64_t)gr * K + k0;
+        const Vec<R, V> prev = vld<R, V>(ldsB, (int64_t)l * K + k0);
+        const Vec<R, V> v1 = recur<R, V, true>(ar, ai, beta, s, prev);
+        const Vec<R, V> vm = vld<R, V>(ldsA, (int64_t)l * K + k0);
+        Vec<R, V> o = acc0;
+        axpy<R, V>(o, cf[2], vm);
+        axpy<R, V>(o, cf[3], v1);
+        vst<R, V>(a.acc, vidx, o);
+        if constexpr (DOT) {
+            const Vec<R, V> p0 = stream_ld<R, V>(a.psi0, vidx);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const double pr = p0.re[v], pi = p0.im[v];
+                const double orr = o.re[v], oi = o.im[v];
+                dre[v] = fma(pr, orr, fma(pi, oi, dre[v]));
+                dim_[v] = fma(pr, oi, fma(-pi, orr, dim_[v]));
+            }
+        }
+        if constexpr (NORM) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const double orr = o.re[v], oi = o.im[v];
+                nrm[v] = fma(orr, orr, fma(oi, oi, nrm[v]));
+            }
+        }
+    };
     auto idxG = [&](int j, int l) { return colG[j * nA + l]; };
     auto idxL = [&](int j, int l) { return colL[j * nB + l]; };
 
@@ -354,81 +422,112 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
     // 8.4M-orbital graphene, K=8); with one state per lane the workgroup has
     // 16 waves to hide latency and the prefetch's registers cost more than
     // they gain (-2%).
-    auto pipelined = [&](auto nj) {
-        constexpr int NJ = decltype(nj)::value;
-        rows_prefetched<R, RV, NJ>(rsub, nA, rpp, rows, vals, nA, idxG,
-                                   [&](int l, const RowTab<R, RV, NJ>& t) {
-            parkA(l, t.gr);
-            rowA(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherA));
-        });
-        __syncthreads();
-        rows_prefetched<R, RV, NJ>(rsub, nB, rpp, rows, vals, nA, idxL,
-                                   [&](int l, const RowTab<R, RV, NJ>& t) {
-            rowB(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherB));
-        });
-        __syncthreads();
-        rows_prefetched<R, RV, NJ>(rsub, m.nR, rpp, rows, vals, nA, idxL,
-                                   [&](int l, const RowTab<R, RV, NJ>& t) {
-            const Vec<R, V> o = acc0(t.gr);
-            rowC(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherC), o);
-        });
-    };
+    bool done = false;
     if constexpr (V >= 2) {
+        auto pipelined = [&](auto nj) {
+            constexpr int NJ = decltype(nj)::value;
+            rows_prefetched<R, RV, NJ>(rsub, nAx, rpp, rows, vals, nA, idxG,
+                                       [&](int l, const RowTab<R, RV, NJ>& t) {
+                parkA(l, t.gr);
+                rowA(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherA));
+            });
+            __syncthreads();
+            if constexpr (PAIR) {
+                rows_prefetched<R, RV, NJ>(rsub, nR, rpp, rows, vals, nA, idxL,
+                                           [&](int l, const RowTab<R, RV, NJ>& t) {
+                    const Vec<R, V> o = acc0(t.gr);
+                    rowP(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherB), o);
+                });
+            } else {
+                rows_prefetched<R, RV, NJ>(rsub, nB, rpp, rows, vals, nA, idxL,
+                                           [&](int l, const RowTab<R, RV, NJ>& t) {
+                    rowB(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherB));
+                });
+                __syncthreads();
+                rows_prefetched<R, RV, NJ>(rsub, nR, rpp, rows, vals, nA, idxL,
+                                           [&](int l, const RowTab<R, RV, NJ>& t) {
+                    const Vec<R, V> o = acc0(t.gr);
+                    rowC(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherC), o);
+                });
+            }
+            done = true;
+        };
         switch (w) {
-            case 1: pipelined(std::integral_constant<int, 1>{}); return;
-            case 2: pipelined(std::integral_constant<int, 2>{}); return;
-            case 3: pipelined(std::integral_constant<int, 3>{}); return;
-            case 4: pipelined(std::integral_constant<int, 4>{}); return;
+            case 1: pipelined(std::integral_constant<int, 1>{}); break;
+            case 2: pipelined(std::integral_constant<int, 2>{}); break;
+            case 3: pipelined(std::integral_constant<int, 3>{}); break;
+            case 4: pipelined(std::integral_constant<int, 4>{}); break;
             default: break;
         }
     }
-    // otherwise one row at a time (phase A, B, C as above)
-    for (int l = rsub; l < nA; l += rpp) {
-        const int gr = rows[l];
-        const Vec<R, V> prev = vld<R, V>(a.p0, (int64_t)gr * K + k0);
-        if (l < nB) vst<R, V>(ldsB, (int64_t)l * K + k0, vld<R, V>(a.p1, (int64_t)gr * K + k0));
-        const Vec<R, V> s = row_sum<R, V, RV>(
-            w, vals, [&](int j) { return colG[j * nA + l]; }, [&](int j) { return j * nA + l; },
-            [&](int c) { return vld<R, V>(a.p1, (int64_t)c * K + k0); });
-        vst<R, V>(ldsA, (int64_t)l * K + k0, recur<R, V, true>(ar, ai, beta, s, prev));
-    }
-    __syncthreads();
-    // phase B: v_{m+1} on R + E1 (each lane reads its own row's parked
-    // v_{m-1} before overwriting it with v_{m+1})
-    for (int l = rsub; l < nB; l += rpp) {
-        const int gr = rows[l];
-        const Vec<R, V> prev = vld<R, V>(ldsB, (int64_t)l * K + k0);
-        const Vec<R, V> s = row_sum<R, V, RV>(
-            w, vals, [&](int j) { return colL[j * nB + l]; }, [&](int j) { return j * nA + l; },
-            [&](int c) { return vld<R, V>(ldsA, (int64_t)c * K + k0); });
-        const Vec<R, V> v1 = recur<R, V, true>(ar, ai, beta, s, prev);
-        vst<R, V>(ldsB, (int64_t)l * K + k0, v1);
-        if (l < m.nR) vst<R, V>(a.q1, (int64_t)gr * K + k0, v1);
-    }
-    __syncthreads();
-    // phase C: v_{m+2} on R, and the step output
-    for (int l = rsub; l < m.nR; l += rpp) {
-        const int gr = rows[l];
-        const int64_t vidx = (int64_t)gr * K + k0;
-        const Vec<R, V> vm = vld<R, V>(ldsA, (int64_t)l * K + k0);
-        const Vec<R, V> s = row_sum<R, V, RV>(
-            w, vals, [&](int j) { return colL[j * nB + l]; }, [&](int j) { return j * nA + l; },
-            [&](int c) { return vld<R, V>(ldsB, (int64_t)c * K + k0); });
-        const Vec<R, V> v2 = recur<R, V, true>(ar, ai, beta, s, vm);
-        vst<R, V>(a.q2, vidx, v2);
-        const Vec<R, V> v1 = vld<R, V>(ldsB, (int64_t)l * K + k0);
-        Vec<R, V> o;
-        if constexpr (FIRST) {
-            o = vzero<R, V>();
-            axpy<R, V>(o, cf[0], vld<R, V>(a.p0, vidx));
-            axpy<R, V>(o, cf[1], vld<R, V>(a.p1, vidx));
-        } else {
-            o = vld<R, V>(a.acc, vidx);
+    if (!done) {
+        // one row at a time
+        for (int l = rsub; l < nAx; l += rpp) {
+            const int gr = rows[l];
+            parkA(l, gr);
+            rowA(l, gr, row_sum<R, V, RV>(
+                            w, vals, [&](int j) { return colG[j * nA + l]; },
+                            [&](int j) { return j * nA + l; }, gatherA));
         }
-        axpy<R, V>(o, cf[2], vm);
-        axpy<R, V>(o, cf[3], v1);
-        axpy<R, V>(o, cf[4], v2);
-        vst<R, V>(a.acc, vidx, o);
+        __syncthreads();
+        if constexpr (PAIR) {
+            for (int l = rsub; l < nR; l += rpp) {
+                const int gr = rows[l];
+                const Vec<R, V> o = acc0(gr);
+                rowP(l, gr, row_sum<R, V, RV>(
+                                w, vals, [&](int j) { return colL[j * nB + l]; },
+                                [&](int j) { return j * nA + l; }, gatherB), o);
+            }
+        } else {
+            for (int l = rsub; l < nB; l += rpp) {
+                const int gr = rows[l];
+                rowB(l, gr, row_sum<R, V, RV>(
+                                w, vals, [&](int j) { return colL[j * nB + l]; },
+                                [&](int j) { return j * nA + l; }, gatherB));
+            }
+            __syncthreads();
+            for (int l = rsub; l < nR; l += rpp) {
+                const int gr = rows[l];
+                const Vec<R, V> o = acc0(gr);
+                rowC(l, gr, row_sum<R, V, RV>(
+                                w, vals, [&](int j) { return colL[j * nB + l]; },
+                                [&](int j) { return j * nA + l; }, gatherC), o);
+            }
+        }
+    }
+
+    if constexpr (NORM) {
+        // Lanes of a wave with equal g hold the same states: butterfly over
+        // them, then the waves' sums in wave order (LDS reused: every lane
+        // is past its last LDS read at the barrier).
+        const int lane = threadIdx.x & 63;
+        for (int o = 1 << lgG; o < 64; o <<= 1) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                dre[v] += __shfl_xor(dre[v], o);
+                dim_[v] += __shfl_xor(dim_[v], o);
+                nrm[v] += __shfl_xor(nrm[v], o);
+            }
+        }
+        constexpr int NW = NT / 64;
+        double* red = reinterpret_cast<double*>(tbg_smem3);   // [NW][3K]
+        __syncthreads();
+        if ((lane >> lgG) == 0) {
+            const int wv = threadIdx.x >> 6;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const int k = k0 + v;
+                red[wv * 3 * K + k * 3 + 0] = dre[v];
+                red[wv * 3 * K + k * 3 + 1] = dim_[v];
+                red[wv * 3 * K + k * 3 + 2] = nrm[v];
+            }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < 3 * K; q += NT) {
+            double t = red[q];
+            for (int wv = 1; wv < NW; ++wv) t += red[wv * 3 * K + q];
+            a.part[(int64_t)b * a.nv + q] = t;
+        }
     }
 }
 
@@ -450,15 +549,17 @@ int term3_launch_threads(int elem_bytes, bool real_vals, int K, int nA) {
     return (int64_t)nA * (K / V) >= 2048 ? 1024 : 512;
 }
 
-template <typename R, int V, bool RV, bool FIRST>
+template <typename R, int V, bool RV, int MODE>
 static void launch_f(const Term3Args& a, const Blocking& b, hipStream_t s) {
-    const size_t lds = (size_t)(b.nA_max + b.nB_max) * a.K * sizeof(typename CplxT<R>::T);
     const int nt = term3_launch_threads((int)sizeof(R), RV, a.K, b.nA_max);
+    size_t lds = (size_t)(b.nA_max + b.nB_max) * a.K * sizeof(typename CplxT<R>::T);
+    if constexpr (MODE == P2_NORM || MODE == P2_DOTNORM)   // the waves' partial sums
+        lds = std::max(lds, (size_t)(nt / 64) * 3 * a.K * sizeof(double));
     // the instances term3_launch_threads can ask for
     constexpr bool wide = V == 1 || (V == 2 && (sizeof(R) == 4 || RV));
     if constexpr (wide) {
         if (nt == 1024) {
-            hipLaunchKernelGGL((k_term3<R, V, RV, FIRST, 1024>), dim3(b.nblocks), dim3(1024),
+            hipLaunchKernelGGL((k_term3<R, V, RV, MODE, 1024>), dim3(b.nblocks), dim3(1024),
                                lds, s, a);
             TBG_HIP(hipGetLastError());
             return;
@@ -466,7 +567,7 @@ static void launch_f(const Term3Args& a, const Blocking& b, hipStream_t s) {
     }
     if constexpr (V >= 2) {
         if (nt == 512) {
-            hipLaunchKernelGGL((k_term3<R, V, RV, FIRST, 512>), dim3(b.nblocks), dim3(512), lds,
+            hipLaunchKernelGGL((k_term3<R, V, RV, MODE, 512>), dim3(b.nblocks), dim3(512), lds,
                                s, a);
             TBG_HIP(hipGetLastError());
             return;
@@ -485,14 +586,21 @@ static void allow_lds_fn(const void* fn, int bytes) {
         (void)hipGetLastError();
 }
 
+template <typename R, int V, bool RV, int NT>
+static void allow_lds_modes(int bytes) {
+    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P3_NORMAL, NT>), bytes);
+    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P3_FIRST, NT>), bytes);
+    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P3_M1, NT>), bytes);
+    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P2_LAST, NT>), bytes);
+    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P2_NORM, NT>), bytes);
+    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P2_DOTNORM, NT>), bytes);
+}
+
 template <typename R, int V, int NT>
 static void allow_lds(int bytes) {
-    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, true, true, NT>), bytes);
-    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, true, false, NT>), bytes);
-    if constexpr (!(NT == 1024 && V == 2 && sizeof(R) == 8)) {   // else not instantiated
-        allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, false, true, NT>), bytes);
-        allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, false, false, NT>), bytes);
-    }
+    allow_lds_modes<R, V, true, NT>(bytes);
+    if constexpr (!(NT == 1024 && V == 2 && sizeof(R) == 8))   // else not instantiated
+        allow_lds_modes<R, V, false, NT>(bytes);
 }
 
 void prepare_term3(int max_lds) {
@@ -507,23 +615,34 @@ void prepare_term3(int max_lds) {
 }
 
 template <typename R, int V, bool RV>
-static void launch_v(bool first, const Term3Args& a, const Blocking& b, hipStream_t s) {
-    if (first) launch_f<R, V, RV, true>(a, b, s);
-    else launch_f<R, V, RV, false>(a, b, s);
+static void launch_v(int mode, const Term3Args& a, const Blocking& b, hipStream_t s) {
+    switch (mode) {
+        case P3_NORMAL: launch_f<R, V, RV, P3_NORMAL>(a, b, s); break;
+        case P3_FIRST: launch_f<R, V, RV, P3_FIRST>(a, b, s); break;
+        case P3_M1: launch_f<R, V, RV, P3_M1>(a, b, s); break;
+        case P2_LAST: launch_f<R, V, RV, P2_LAST>(a, b, s); break;
+        case P2_NORM: launch_f<R, V, RV, P2_NORM>(a, b, s); break;
+        case P2_DOTNORM: launch_f<R, V, RV, P2_DOTNORM>(a, b, s); break;
+        default: throw Error{TBG_EINVAL, "internal: unknown pass mode"};
+    }
 }
 
 template <typename R>
-void launch_term3(bool first, bool real_vals, const Term3Args& a, const Blocking& b,
+void launch_term3(int mode, bool real_vals, const Term3Args& a, const Blocking& b,
                   hipStream_t s) {
     const int V = term3_vectors_per_lane(a.K);
     if (a.K > 64 || a.lgG < 0 || a.lgG > 6 || (1 << a.lgG) * V != a.K)
         throw Error{TBG_EINVAL, "internal: bad vector slot count"};
     if (b.row_bytes != a.K * (int)sizeof(typename CplxT<R>::T))
         throw Error{TBG_EINVAL, "internal: blocking built for another row size"};
+    if ((mode == P2_NORM || mode == P2_DOTNORM) && (a.part == nullptr || a.nv < 3 * a.K))
+        throw Error{TBG_EINVAL, "internal: epilogue without a partial-sum buffer"};
+    if (mode == P2_DOTNORM && a.psi0 == nullptr)
+        throw Error{TBG_EINVAL, "internal: dot epilogue without psi0"};
 #define TBG_T3(VV)                                                             \
     do {                                                                       \
-        if (real_vals) launch_v<R, VV, true>(first, a, b, s);                  \
-        else launch_v<R, VV, false>(first, a, b, s);                           \
+        if (real_vals) launch_v<R, VV, true>(mode, a, b, s);                   \
+        else launch_v<R, VV, false>(mode, a, b, s);                            \
     } while (0)
     switch (V) {
         case 1: TBG_T3(1); break;
@@ -533,7 +652,7 @@ void launch_term3(bool first, bool real_vals, const Term3Args& a, const Blocking
     }
 #undef TBG_T3
 }
-template void launch_term3<float>(bool, bool, const Term3Args&, const Blocking&, hipStream_t);
-template void launch_term3<double>(bool, bool, const Term3Args&, const Blocking&, hipStream_t);
+template void launch_term3<float>(int, bool, const Term3Args&, const Blocking&, hipStream_t);
+template void launch_term3<double>(int, bool, const Term3Args&, const Blocking&, hipStream_t);
 
 }  // namespace tbg

@@ -397,8 +397,8 @@ void corr_ac_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
                   nbuf.as<double>(), 0, counter.as<int>(), 0, st);
     const StepPoly sp = time_poly(bes, nb);
     auto after = [&]() {
-        launch_reduce(e.part.template as<double>(), nvn, e.plan.inst_chunk_dev.template as<int>(),
-                      1, nbuf.as<double>(), 0, counter.as<int>(), 0, st);
+        launch_reduce(e.part.template as<double>(), nvn, e.epi_range(), 1, nbuf.as<double>(), 0,
+                      counter.as<int>(), 0, st);
         hipLaunchKernelGGL(k_accorr<R>, dim3(e.plan.nchunks), dim3(256), 0, st,
                            H->cols.as<int>(), H->jx.as<C>(), H->jy.as<C>(),
                            H->slice_off.as<int64_t>(), H->slice_w.as<int>(),
@@ -486,8 +486,8 @@ void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
                   nbuf.as<double>(), 0, counter.as<int>(), 0, st);
     const StepPoly sp = time_poly(bes, nb);
     auto after1 = [&]() {
-        launch_reduce(e.part.template as<double>(), nvn, e.plan.inst_chunk_dev.template as<int>(),
-                      1, nbuf.as<double>(), 0, counter.as<int>(), 0, st);
+        launch_reduce(e.part.template as<double>(), nvn, e.epi_range(), 1, nbuf.as<double>(), 0,
+                      counter.as<int>(), 0, st);
         hipLaunchKernelGGL(k_qe<R>, dim3(e.plan.nchunks), dim3(256), 0, st,
                            e.plan.chunk.template as<int64_t>(), e.X.template as<C>(), Kp, K,
                            psi.as<C>(), qe.as<double2>(), n_en, dptab.as<double2>(),

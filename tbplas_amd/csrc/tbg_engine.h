@@ -94,7 +94,8 @@ struct Engine {
     // Temporal blocking (three terms per pass) and its two extra recurrence
     // buffers, when the single Hamiltonian of the run has spatial tiles.
     std::shared_ptr<Blocking> blk;
-    DevBuf Q, S;
+    DevBuf Q, S, tile_range;
+    bool epi_tiles = false;
 
     void init(tbg_handle* hh, const MatView& m, const std::vector<int64_t>& inst_slices,
               int k_needed, int chunk_target, const tbg_ham* H = nullptr) {
@@ -113,18 +114,34 @@ struct Engine {
         X.alloc(vbytes);
         P.alloc(vbytes);
         Y.alloc(vbytes);
-        part.alloc((size_t)plan.nchunks * 3 * Kp * sizeof(double) + 64);
         launch_fill_zero(X.p, vbytes, h->stream);
         launch_fill_zero(P.p, vbytes, h->stream);
         launch_fill_zero(Y.p, vbytes, h->stream);
         blk.reset();
         if (H != nullptr) blk = get_blocking(h, H, Kp * (int)sizeof(typename CplxT<R>::T));
+        int nparts = plan.nchunks;
         if (blk) {
             Q.alloc(vbytes);
             S.alloc(vbytes);
             launch_fill_zero(Q.p, vbytes, h->stream);
             launch_fill_zero(S.p, vbytes, h->stream);
+            // partial sums of a pair pass: one row per tile, tiles [0, nblocks)
+            const int range[2] = {0, blk->nblocks};
+            tile_range.alloc(sizeof(range));
+            TBG_HIP(hipMemcpyAsync(tile_range.p, range, sizeof(range), hipMemcpyHostToDevice,
+                                   h->stream));
+            TBG_HIP(hipStreamSynchronize(h->stream));
+            nparts = std::max(nparts, blk->nblocks);
         }
+        part.alloc((size_t)nparts * 3 * Kp * sizeof(double) + 64);
+        epi_tiles = false;
+    }
+
+    // The partial-row range (per instance) that the last step's F_DOT /
+    // F_NORM epilogue wrote into part: the chunks of the one-term kernel, or
+    // the tiles [0, nblocks) when a pair pass ended the step.
+    const int* epi_range() const {
+        return epi_tiles ? tile_range.as<int>() : plan.inst_chunk_dev.template as<int>();
     }
 
     TermArgs base_args(uint64_t bwd, uint64_t cls) const {
@@ -145,59 +162,69 @@ struct Engine {
         return a;
     }
 
-    // Three terms per pass: m = 1 alone, then triples (m, m+1, m+2) from
-    // m = 2 while at least one term is left for the last (single) launch,
-    // which also forms the correlations.  The step output receives the terms
-    // in the order m = 0, 1, 2, ... exactly as in the one-term schedule.
+    // Three terms per pass: terms 1, 2, 3 (P3_M1), then triples; the last
+    // one or two terms end the step: with two left, one pair pass that also
+    // forms the epilogue's sums (per tile); with one left, the one-term
+    // kernel with the epilogue; with three left, one one-term launch, then
+    // the pair.  The step output receives the terms in the order m = 0, 1,
+    // 2, ... as in the one-term schedule, so the states are bit-identical to
+    // it; the epilogue's sums of a pair pass are grouped by tile instead of
+    // by chunk.  21 Bessel terms: 7 launches (m = 1-3, five triples, 19-20).
     void step_blocked(const StepPoly& sp, uint64_t bwd, uint64_t cls, int epi,
                       const void* psi0, int* counter) {
-        const int N = sp.N;
-        {
-            TermArgs a = base_args(bwd, cls);
-            a.acc = Y.p;
-            a.pin = X.p;
-            a.pout = P.p;
-            a.ar[0] = a.ar[1] = sp.ar1;
-            a.ai[0] = a.ai[1] = sp.ai1;
-            a.counter = counter;
-            launch_term<R>(F_STORE, a, plan.nchunks, h->stream);
-        }
-        void* b0 = X.p;   // v_{m-2}
-        void* b1 = P.p;   // v_{m-1}
-        void* b2 = Q.p;
-        void* b3 = S.p;
-        const int ntri = (N - 3) / 3;
-        int m = 2;
-        for (int i = 0; i < ntri; ++i, m += 3) {
-            Term3Args t{};
-            t.meta = blk->meta.as<BlockMeta>();
-            t.rows = blk->rows.as<int>();
-            t.colL = blk->colL.as<int>();
-            t.colG = blk->colG.as<int>();
-            t.valG = blk->valG.p;
-            t.p0 = b0;
-            t.p1 = b1;
-            t.q1 = b2;
-            t.q2 = b3;
-            t.acc = Y.p;
-            t.ar[0] = t.ar[1] = sp.ar;
-            t.ai[0] = t.ai[1] = sp.ai;
-            t.beta = sp.beta;
+        const int N = sp.N;   // terms m = 1 .. N-1
+        Term3Args base{};
+        base.meta = blk->meta.as<BlockMeta>();
+        base.rows = blk->rows.as<int>();
+        base.colL = blk->colL.as<int>();
+        base.colG = blk->colG.as<int>();
+        base.valG = blk->valG.p;
+        base.acc = Y.p;
+        base.ar[0] = base.ar[1] = sp.ar;
+        base.ai[0] = base.ai[1] = sp.ai;
+        base.ar1[0] = base.ar1[1] = sp.ar1;
+        base.ai1[0] = base.ai1[1] = sp.ai1;
+        base.beta = sp.beta;
+        base.bwd_mask = bwd;
+        base.cls_mask = cls;
+        base.K = Kp;
+        base.lgG = lgG3;
+        base.nA_max = blk->nA_max;
+        base.nv = 3 * Kp;
+        auto pass = [&](int mode, int m, void* p0, void* p1, void* q1, void* q2) {
+            Term3Args t = base;
+            t.p0 = p0;
+            t.p1 = p1;
+            t.q1 = q1;
+            t.q2 = q2;
             for (int q = 0; q < 5; ++q) {
-                t.c[q][0] = sp.c[0][m - 2 + q];
-                t.c[q][1] = sp.c[1][m - 2 + q];
+                const int j = m - 2 + q;   // c_{m-2} .. c_{m+2}, those in the series
+                t.c[q][0] = (j >= 0 && j < N) ? sp.c[0][j] : 0.0;
+                t.c[q][1] = (j >= 0 && j < N) ? sp.c[1][j] : 0.0;
             }
-            t.bwd_mask = bwd;
-            t.cls_mask = cls;
-            t.K = Kp;
-            t.lgG = lgG3;
-            t.nA_max = blk->nA_max;
-            launch_term3<R>(i == 0, M.real_vals != 0, t, *blk, h->stream);
+            if (mode == P3_M1) t.counter = counter;
+            if (mode == P2_NORM || mode == P2_DOTNORM) {
+                t.psi0 = psi0;
+                t.part = part.as<double>();
+            }
+            launch_term3<R>(mode, M.real_vals != 0, t, *blk, h->stream);
+        };
+        // v_{m-2}, v_{m-1} and two free buffers; X (v_0) is free after the
+        // first pass and Y receives the output (X and Y swap at the end)
+        void* b0 = P.p;
+        void* b1 = Q.p;
+        void* b2 = S.p;
+        void* b3 = X.p;
+        pass(P3_M1, 1, nullptr, X.p, b0, b1);
+        int m = 4;
+        const int rem = N - 4;   // terms m = 4 .. N-1
+        const int tail = rem % 3 == 0 ? 3 : rem % 3;
+        for (int i = 0; i < (rem - tail) / 3; ++i, m += 3) {
+            pass(P3_NORMAL, m, b0, b1, b2, b3);
             std::swap(b0, b2);
             std::swap(b1, b3);
         }
-        for (; m < N; ++m) {
-            const bool last = (m == N - 1);
+        if (tail == 3) {   // one term alone, leaving the pair
             TermArgs a = base_args(bwd, cls);
             a.acc = Y.p;
             a.pin = b1;
@@ -208,16 +235,29 @@ struct Engine {
             a.beta = sp.beta;
             a.cc[0] = sp.c[0][m];
             a.cc[1] = sp.c[1][m];
-            int fl = F_PREV | F_ACC | F_ACCRMW;
-            if (last) {
-                fl |= epi;
-                a.psi0 = psi0;
-                a.part = part.as<double>();
-            } else {
-                fl |= F_STORE;
-            }
-            launch_term<R>(fl, a, plan.nchunks, h->stream);
+            launch_term<R>(F_PREV | F_ACC | F_ACCRMW | F_STORE, a, plan.nchunks, h->stream);
             std::swap(b0, b1);
+            ++m;
+        }
+        if (tail == 1) {
+            TermArgs a = base_args(bwd, cls);
+            a.acc = Y.p;
+            a.pin = b1;
+            a.pprev = b0;
+            a.pout = b0;
+            a.ar[0] = a.ar[1] = sp.ar;
+            a.ai[0] = a.ai[1] = sp.ai;
+            a.beta = sp.beta;
+            a.cc[0] = sp.c[0][m];
+            a.cc[1] = sp.c[1][m];
+            a.psi0 = psi0;
+            a.part = part.as<double>();
+            launch_term<R>(F_PREV | F_ACC | F_ACCRMW | epi, a, plan.nchunks, h->stream);
+            epi_tiles = false;
+        } else {
+            const int mode = (epi & F_DOT) ? P2_DOTNORM : (epi & F_NORM) ? P2_NORM : P2_LAST;
+            pass(mode, m, b0, b1, nullptr, nullptr);
+            epi_tiles = mode != P2_LAST;
         }
         std::swap(X, Y);
         parity ^= 1;
