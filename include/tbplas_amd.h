@@ -193,6 +193,25 @@ int tbg_corr_dos(tbg_handle *h, const tbg_ham *H, const double *bes,
                  int n_bes, const double *psi0, int K, int T, int chk_step,
                  double chk_thr, double *corr, int *fail_step);
 
+/* Replaces f2py tbpm_ldos (tbplas/fortran/tbpm.f90:97-189; called from
+ * tbplas/tbpm/solver.py:286-315).  psi0: the LDOS states (random states
+ * kept on the selected sites and normalised, tbpm.f90:151-156, formed by
+ * the caller).  corr: complex128 (K, T+1) = <psi0|psi(t)>, t = 0..T.  The
+ * norm check compares with the initial norm (tbpm.f90:170), chk_step in
+ * 1..T. */
+int tbg_corr_ldos(tbg_handle *h, const tbg_ham *H, const double *bes,
+                  int n_bes, const double *psi0, int K, int T, int chk_step,
+                  double chk_thr, double *corr, int *fail_step);
+
+/* Replaces f2py tbpm_psi_t (tbplas/fortran/tbpm.f90:1029-1093; called from
+ * tbplas/tbpm/solver.py:543-600).  psi0: device complex128 (n,); time_log:
+ * host, n_log strictly increasing steps in [0, T]; psi_t: device
+ * complex128 (n_log, n), row i = the state after time_log[i] steps.  The
+ * norm of the state entering step chk_step is checked against psi0's. */
+int tbg_psi_t(tbg_handle *h, const tbg_ham *H, const double *bes, int n_bes,
+              const double *psi0, const int64_t *time_log, int n_log, int T,
+              int chk_step, double chk_thr, double *psi_t, int *fail_step);
+
 /* Replaces f2py tbpm_accond (tbplas/fortran/tbpm.f90:193-351; called from
  * tbplas/tbpm/solver.py:335-349).  H must carry current operators built with
  * cur_scale = rescale.  coef_F / coef_omF: host Chebyshev coefficients of the
@@ -213,6 +232,18 @@ int tbg_corr_dc(tbg_handle *h, const tbg_ham *H, const double *bes, int n_bes,
                 const double *energies, const int64_t *en_inds, int n_en,
                 int chk_step, double chk_thr, double *dos_corr,
                 double *dc_corr, int *fail_step);
+
+/* Replaces f2py tbpm_eigenstates (tbplas/fortran/tbpm.f90:721-830; called
+ * from tbplas/tbpm/solver.py:472-505).  energies: host, n_en energies (eV,
+ * with t_step in hbar/eV as for tbg_corr_dc).  qe: device float64
+ * (K, n_en, n) = |QE|^2 per state after normalisation (tbpm.f90:812-825);
+ * the reference's wf_QE is the average over states of these rows.  At most
+ * 32 states per call (each propagates forward and backward). */
+int tbg_quasi_eigenstates(tbg_handle *h, const tbg_ham *H, const double *bes,
+                          int n_bes, const double *psi0, int K, int T,
+                          double t_step, const double *energies, int n_en,
+                          int chk_step, double chk_thr, double *qe,
+                          int *fail_step);
 
 /* One propagation step psi <- exp(-+ i H~ tau) psi of K states in place
  * (cheb_wf_timestep, tbplas/fortran/propagation.f90:12-49).  fwd=1: forward.

@@ -9,9 +9,9 @@ import numpy as np
 __all__ = [
     "oracle_lib", "ref_lib", "ref_available",
     "orc_tbpm_dos", "orc_tbpm_accond", "orc_tbpm_dccond", "orc_step",
-    "orc_fermi_coef", "orc_csr_mv",
+    "orc_fermi_coef", "orc_csr_mv", "orc_tbpm_ldos", "orc_psi_t", "orc_quasi_eigenstates",
     "ref_random_state", "ref_tbpm_dos", "ref_tbpm_accond", "ref_tbpm_dccond",
-    "ref_fermi_coef",
+    "ref_fermi_coef", "ref_tbpm_ldos", "ref_tbpm_psi_t", "ref_tbpm_eigenstates",
     "bessel_series", "build_ham_csr", "build_ham_dxy_loop",
     "build_ham_dxy_stable", "graphene_hops_loop", "graphene_hops_vec", "get_rescale_loop",
     "set_mag_field_loop", "kspace_graphene_corr", "dense_corr",
@@ -174,6 +174,65 @@ def orc_step(indptr, indices, hop, bes, psi, fwd=True):
     return out
 
 
+def orc_tbpm_ldos(indptr, indices, hop, bes, psi0, n_timestep):
+    """tbpm_ldos (tbpm.f90:97-189) on caller-supplied LDOS states psi0
+    (S, n) (already kept on the sites and normalised, tbpm.f90:151-156):
+    per-state <psi0|psi(t)>, (S, T+1).  Built from orc_step."""
+    psi0 = np.atleast_2d(np.asarray(psi0, dtype=np.complex128))
+    corr = np.zeros((psi0.shape[0], n_timestep + 1), dtype=np.complex128)
+    for s, w0 in enumerate(psi0):
+        wt = w0.copy()
+        corr[s, 0] = np.vdot(w0, wt)
+        for k in range(1, n_timestep + 1):
+            wt = orc_step(indptr, indices, hop, bes, wt, True)
+            corr[s, k] = np.vdot(w0, wt)
+    return corr
+
+
+def orc_psi_t(indptr, indices, hop, bes, psi0, time_log, n_timestep):
+    """tbpm_psi_t (tbpm.f90:1029-1093): psi0 propagated, logged after the
+    steps in time_log (sorted; the log pointer stays on the last entry):
+    (n_log, n).  Built from orc_step."""
+    tl = [int(t) for t in time_log]
+    out = np.zeros((len(tl), len(psi0)), dtype=np.complex128)
+    wt = np.asarray(psi0, dtype=np.complex128).copy()
+    ptr = 0
+    for t in range(1, n_timestep + 1):
+        if tl[ptr] == t - 1:
+            out[ptr] = wt
+            if ptr < len(tl) - 1:
+                ptr += 1
+        wt = orc_step(indptr, indices, hop, bes, wt, True)
+    if tl[ptr] == n_timestep:
+        out[ptr] = wt
+    return out
+
+
+def orc_quasi_eigenstates(indptr, indices, hop, bes, psi0, n_timestep, t_step, energies):
+    """tbpm_eigenstates (tbpm.f90:721-830) on caller-supplied states psi0
+    (S, n): QE_i = psi0 + sum_k W(k) (e^{i E_i k dt} psi(+k) + e^{-i E_i k
+    dt} psi(-k)), W = Hanning 0.5 (1 + cos(pi k / T)), normalised; returns
+    the per-state |QE|^2, (S, n_E, n) (the reference averages them)."""
+    psi0 = np.atleast_2d(np.asarray(psi0, dtype=np.complex128))
+    en = np.asarray(energies, dtype=np.float64)
+    out = np.zeros((psi0.shape[0], len(en), psi0.shape[1]))
+    for s, w0 in enumerate(psi0):
+        pos, neg = w0.copy(), w0.copy()
+        wfq = np.tile(w0, (len(en), 1))
+        for k in range(1, n_timestep + 1):
+            pos = orc_step(indptr, indices, hop, bes, pos, True)
+            neg = orc_step(indptr, indices, hop, bes, neg, False)
+            W = 0.5 * (1 + math.cos(math.pi * k / n_timestep))
+            for i, e in enumerate(en):
+                P = complex(math.cos(e * k * t_step), math.sin(e * k * t_step))
+                wfq[i] = wfq[i] + P * pos * W
+                wfq[i] = wfq[i] + P.conjugate() * neg * W
+        for i in range(len(en)):
+            wfq[i] = wfq[i] / np.sqrt(np.vdot(wfq[i], wfq[i]).real)
+        out[s] = np.abs(wfq) ** 2
+    return out
+
+
 def orc_csr_mv(indptr, indices, val, x):
     """csr_mv (csr.F90:71-101)."""
     lib = oracle_lib()
@@ -297,6 +356,63 @@ def ref_tbpm_dccond(bes, beta_re, mu_re, indptr, indices, hop, rescale, dx, dy,
                      r(I(len(en))), f1, f2, _ptr(dos), _ptr(dc), r(I(0)),
                      r(I(chk_step)), r(D(chk_thr)), l1, l2)
     return dos, np.ascontiguousarray(dc)
+
+
+def ref_tbpm_ldos(site_indices, bes, indptr, indices, hop, seed, n_timestep, n_samples,
+                  chk_step=128, chk_thr=1e-9):
+    """The reference tbpm_ldos (tbpm.f90:97-189), sample-averaged (T+1,)."""
+    lib = ref_lib()
+    si = _i32(site_indices)
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    ip, ix = _i32(indptr), _i32(indices)
+    hop = np.ascontiguousarray(hop, dtype=np.complex128)
+    corr = np.zeros(n_timestep + 1, dtype=np.complex128)
+    fn, fl = _fstr("unused")
+    I, D = ctypes.c_int, ctypes.c_double
+    r = ctypes.byref
+    lib.tbpm_ldos_(_ptr(si, I), r(I(len(si))), _ptr(bes), r(I(len(bes))), _ptr(ip, I),
+                   r(I(len(ip))), _ptr(ix, I), r(I(len(ix))), _ptr(hop), r(I(len(hop))),
+                   r(I(seed)), r(I(n_timestep)), r(I(n_samples)), fn, _ptr(corr), r(I(0)),
+                   r(I(chk_step)), r(D(chk_thr)), fl)
+    return corr
+
+
+def ref_tbpm_psi_t(bes, indptr, indices, hop, psi0, time_log, n_timestep, chk_step=128,
+                   chk_thr=1e-9):
+    """The reference tbpm_psi_t (tbpm.f90:1029-1093): (n_log, n)."""
+    lib = ref_lib()
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    ip, ix = _i32(indptr), _i32(indices)
+    hop = np.ascontiguousarray(hop, dtype=np.complex128)
+    psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
+    tl = _i32(time_log)
+    out = np.zeros((len(tl), len(psi0)), dtype=np.complex128)   # Fortran (n, n_log)
+    I, D = ctypes.c_int, ctypes.c_double
+    r = ctypes.byref
+    lib.tbpm_psi_t_(_ptr(bes), r(I(len(bes))), _ptr(ip, I), r(I(len(ip))), _ptr(ix, I),
+                    r(I(len(ix))), _ptr(hop), r(I(len(hop))), _ptr(psi0), r(I(len(psi0))),
+                    _ptr(tl, I), r(I(len(tl))), r(I(n_timestep)), r(I(chk_step)),
+                    r(D(chk_thr)), _ptr(out))
+    return out
+
+
+def ref_tbpm_eigenstates(bes, indptr, indices, hop, seed, n_timestep, n_samples, t_step,
+                         energies, chk_step=128, chk_thr=1e-9):
+    """The reference tbpm_eigenstates (tbpm.f90:721-830): (n_E, n)."""
+    lib = ref_lib()
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    ip, ix = _i32(indptr), _i32(indices)
+    hop = np.ascontiguousarray(hop, dtype=np.complex128)
+    en = np.ascontiguousarray(energies, dtype=np.float64)
+    n = len(ip) - 1
+    out = np.zeros((n, len(en)))                                  # Fortran (n_E, n)
+    I, D = ctypes.c_int, ctypes.c_double
+    r = ctypes.byref
+    lib.tbpm_eigenstates_(_ptr(bes), r(I(len(bes))), _ptr(ip, I), r(I(len(ip))), _ptr(ix, I),
+                          r(I(len(ix))), _ptr(hop), r(I(len(hop))), r(I(seed)),
+                          r(I(n_timestep)), r(I(n_samples)), r(D(t_step)), _ptr(en),
+                          r(I(len(en))), _ptr(out), r(I(0)), r(I(chk_step)), r(D(chk_thr)))
+    return np.ascontiguousarray(out.T)
 
 
 def ref_fermi_coef(nr_fermi, beta_re, mu_re, one_minus, eps):

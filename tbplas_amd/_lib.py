@@ -46,6 +46,10 @@ _SIGS = {
     "tbg_ham_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                      ctypes.POINTER(_I64), ctypes.POINTER(_I)],
     "tbg_corr_dos": [_P, _P, _P, _I, _P, _I, _I, _I, _D, _P, ctypes.POINTER(_I)],
+    "tbg_corr_ldos": [_P, _P, _P, _I, _P, _I, _I, _I, _D, _P, ctypes.POINTER(_I)],
+    "tbg_psi_t": [_P, _P, _P, _I, _P, _P, _I, _I, _I, _D, _P, ctypes.POINTER(_I)],
+    "tbg_quasi_eigenstates": [_P, _P, _P, _I, _P, _I, _I, _D, _P, _I, _I, _D, _P,
+                              ctypes.POINTER(_I)],
     "tbg_corr_ac": [_P, _P, _P, _I, _P, _I, _P, _I, _P, _I, _I, _I, _D, _P,
                     ctypes.POINTER(_I)],
     "tbg_corr_dc": [_P, _P, _P, _I, _P, _I, _I, _D, _P, _P, _I, _I, _D, _P, _P,

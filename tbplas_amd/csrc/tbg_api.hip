@@ -55,6 +55,9 @@ struct tbg_run {
                      int* fail_step) = 0;
     tbg_handle* h = nullptr;
     int T = 0, done = 0;
+    // time index of the norm reference of the wave-function check: 1 for
+    // DOS (tbpm.f90:73), 0 for LDOS (tbpm.f90:170)
+    int norm_ref_t = 1;
 };
 
 namespace tbg {
@@ -140,10 +143,12 @@ struct DosRun : tbg_run {
                                    hipMemcpyHostToDevice, h->stream));
             TBG_HIP(hipStreamSynchronize(h->stream));
             // tbpm.f90:73-78: norm_ref after step 1, checked on the state of
-            // step chk_step-1 at loop index chk_step (2..T).
-            if (rc == TBG_OK && chk_step >= 2 && chk_step <= done) {
+            // step chk_step-1 at loop index chk_step (2..T); LDOS
+            // (tbpm.f90:170-176): norm_ref of the initial state, loop index
+            // chk_step in 1..T.
+            if (rc == TBG_OK && chk_step >= norm_ref_t + 1 && chk_step <= done) {
                 for (int k = 0; k < K; ++k) {
-                    const double ref = b[(size_t)1 * nv + 3 * k + 2];
+                    const double ref = b[(size_t)norm_ref_t * nv + 3 * k + 2];
                     const double nn = b[(size_t)(chk_step - 1) * nv + 3 * k + 2];
                     if (norm_broken(nn, ref, chk_thr)) {
                         rc = TBG_ENORM;
@@ -173,8 +178,9 @@ static tbg_run* dos_begin(tbg_handle* h, int n_inst, const tbg_ham* const* hams,
 static void corr_dos_impl(tbg_handle* h, int n_inst, const tbg_ham* const* hams,
                           const double* bes, int nb, const double* const* psi0, int K, int T,
                           int chk_step, double chk_thr, double* const* corr, int* fail_inst,
-                          int* fail_step) {
+                          int* fail_step, int norm_ref_t = 1) {
     std::unique_ptr<tbg_run> r(dos_begin(h, n_inst, hams, bes, nb, psi0, K, T));
+    r->norm_ref_t = norm_ref_t;
     r->advance(T);
     r->end(chk_step, chk_thr, corr, fail_inst, fail_step);
 }
@@ -188,6 +194,64 @@ static void step_impl(tbg_handle* h, const tbg_ham* H, const StepPoly& sp, doubl
     e.step(sp, bwd, 0, 0, nullptr, nullptr);
     launch_unpack<R>(e.X.p, e.Kp, 0, H->n, psi, K, H->perm_ptr(), h->stream);
     TBG_HIP(hipStreamSynchronize(h->stream));
+}
+
+// tbpm_psi_t (tbpm.f90:1029-1093): psi0 propagated forward T steps; the
+// state after t steps is written to psi_t[i] when time_log[i] == t (a
+// sorted list; the pointer stays on the last entry once it is reached,
+// as in the reference's loop).  The norm of the state entering step
+// chk_step is checked against the initial norm.
+template <typename R>
+static void psi_t_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb,
+                       const double* psi0, const int64_t* tlog, int n_log, int T, int chk_step,
+                       double chk_thr, double* psi_t, int* fail_step) {
+    const hipStream_t st = h->stream;
+    Engine<R> e;
+    e.init(h, view_of(H), {0, H->nslices}, 1, chunk_target(h, H->nslices), H);
+    const int Kp = e.Kp, nv = 3 * Kp;
+    const int64_t n = H->n;
+    launch_pack<R>(psi0, 1, n, e.X.p, Kp, 0, H->perm_ptr(), st);
+    DevBuf nbuf((size_t)nv * sizeof(double));
+    auto norm_of_state = [&]() {
+        launch_dotnorm<R>(e.plan, e.X.p, e.X.p, Kp, Kp, e.part.template as<double>(), st);
+        launch_reduce(e.part.template as<double>(), nv, e.plan.inst_chunk_dev.template as<int>(),
+                      1, nbuf.as<double>(), 0, nullptr, 0, st);
+        double v[3];
+        TBG_HIP(hipMemcpyAsync(v, nbuf.p, sizeof(v), hipMemcpyDeviceToHost, st));
+        TBG_HIP(hipStreamSynchronize(st));
+        return v[2];
+    };
+    const double norm_ref = norm_of_state();
+    const StepPoly sp = time_poly(bes, nb);
+    auto log_frame = [&](int i) {
+        launch_unpack<R>(e.X.p, Kp, 0, n, psi_t + (size_t)i * n * 2, 1, H->perm_ptr(), st);
+    };
+    int ptr = 0;
+    int t = 1;   // loop index of the reference: the state holds t-1 steps
+    while (t <= T) {
+        if (tlog[ptr] == t - 1) {
+            log_frame(ptr);
+            if (ptr < n_log - 1) ++ptr;
+        }
+        if (t == chk_step) {
+            const double nn = norm_of_state();
+            if (norm_broken(nn, norm_ref, chk_thr)) {
+                if (fail_step) *fail_step = chk_step;
+                throw Error{TBG_ENORM, "wave function norm broken at step " +
+                                           std::to_string(chk_step) +
+                                           "; check the Hermiticity of the Hamiltonian or "
+                                           "increase the rescale factor"};
+            }
+        }
+        // run up to the next loop index that logs or checks
+        int next = T + 1;
+        if (tlog[ptr] + 1 > t && tlog[ptr] + 1 <= T) next = (int)(tlog[ptr] + 1);
+        if (chk_step > t && chk_step < next) next = chk_step;
+        e.run(next - t, sp, 0, 0, 0, nullptr, nullptr, [] {});
+        t = next;
+    }
+    if (tlog[ptr] == T) log_frame(ptr);
+    TBG_HIP(hipStreamSynchronize(st));
 }
 
 static void check_psi_args(const tbg_ham* H, const double* psi0, int K, int T) {
@@ -563,6 +627,42 @@ int tbg_corr_dos(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
     });
 }
 
+int tbg_corr_ldos(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
+                  const double* psi0, int K, int T, int chk_step, double chk_thr, double* corr,
+                  int* fail_step) {
+    return guard([&] {
+        const tbg_ham* hs[1] = {H};
+        const double* ps[1] = {psi0};
+        double* cs[1] = {corr};
+        check_batch(h, 1, hs, bes, n_bes, ps, K, T);
+        TBG_REQUIRE(corr != nullptr, "null output");
+        corr_dos_impl(h, 1, hs, bes, n_bes, ps, K, T, chk_step, chk_thr, cs, nullptr, fail_step,
+                      0);
+    });
+}
+
+int tbg_psi_t(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes, const double* psi0,
+              const int64_t* time_log, int n_log, int T, int chk_step, double chk_thr,
+              double* psi_t, int* fail_step) {
+    return guard([&] {
+        check_handle(h);
+        check_psi_args(H, psi0, 1, T);
+        TBG_REQUIRE(bes && n_bes >= 3, "need >= 3 Bessel terms");
+        TBG_REQUIRE(psi_t != nullptr && time_log != nullptr && n_log >= 1, "null output or log");
+        for (int i = 0; i < n_log; ++i) {
+            TBG_REQUIRE(time_log[i] >= 0 && time_log[i] <= T, "time_log entry out of [0, T]");
+            TBG_REQUIRE(i == 0 || time_log[i] > time_log[i - 1],
+                        "time_log must be strictly increasing");
+        }
+        if (H->precision == TBG_FP32)
+            psi_t_impl<float>(h, H, bes, n_bes, psi0, time_log, n_log, T, chk_step, chk_thr,
+                              psi_t, fail_step);
+        else
+            psi_t_impl<double>(h, H, bes, n_bes, psi0, time_log, n_log, T, chk_step, chk_thr,
+                               psi_t, fail_step);
+    });
+}
+
 int tbg_corr_dos_batch(tbg_handle* h, int n_inst, const tbg_ham* const* hams, const double* bes,
                        int n_bes, const double* const* psi0, int K, int T, int chk_step,
                        double chk_thr, double* const* corr, int* fail_inst, int* fail_step) {
@@ -684,6 +784,31 @@ int tbg_corr_dc(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
         else
             corr_dc_impl<double>(h, H, bes, n_bes, psi0, K, T, t_step, energies, en_inds, n_en,
                                  chk_step, chk_thr, dos_corr, dc_corr, fail_step);
+    });
+}
+
+int tbg_quasi_eigenstates(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
+                          const double* psi0, int K, int T, double t_step,
+                          const double* energies, int n_en, int chk_step, double chk_thr,
+                          double* qe, int* fail_step) {
+    return guard([&] {
+        check_handle(h);
+        check_psi_args(H, psi0, K, T);
+        TBG_REQUIRE(bes && n_bes >= 3 && qe, "need >= 3 Bessel terms and an output");
+        TBG_REQUIRE(n_en >= 1 && energies, "need at least one energy");
+        TBG_REQUIRE(2 * K <= 64, "at most 32 states per call");
+        // the energies themselves (tbpm.f90:786) through the DC run's index list
+        std::vector<int64_t> idx(n_en);
+        for (int i = 0; i < n_en; ++i) idx[i] = i;
+        std::vector<double> dos((size_t)K * T * 2);
+        DevBuf ddos(dos.size() * sizeof(double));
+        if (H->precision == TBG_FP32)
+            corr_dc_impl<float>(h, H, bes, n_bes, psi0, K, T, t_step, energies, idx.data(), n_en,
+                                chk_step, chk_thr, ddos.as<double>(), nullptr, fail_step, qe);
+        else
+            corr_dc_impl<double>(h, H, bes, n_bes, psi0, K, T, t_step, energies, idx.data(),
+                                 n_en, chk_step, chk_thr, ddos.as<double>(), nullptr, fail_step,
+                                 qe);
     });
 }
 

@@ -32,6 +32,6 @@ template <typename R>
 void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, const double* psi0,
                   int K, int T, double t_step, const double* energies, const int64_t* en_inds,
                   int n_en, int chk_step, double chk_thr, double* dos_corr, double* dc_corr,
-                  int* fail_step);
+                  int* fail_step, double* qe_abs2 = nullptr);
 
 }  // namespace tbg

@@ -272,6 +272,21 @@ __global__ void k_sum_rows(const double* __restrict__ part, int nchunks, int64_t
     out[q] = acc;
 }
 
+// |QE(r, e, s)|^2 in the caller's row order: out[(s * n_en + e) * n + perm[r]]
+// (tbpm.f90:820-825, one sample per state s).
+__global__ void k_qe_abs2(const double2* __restrict__ qe, int64_t n, int K, int n_en,
+                          const int* __restrict__ perm, double* __restrict__ out) {
+    const int64_t nq = (int64_t)n_en * K;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * nq) return;
+    const int64_t r = i / nq;
+    const int q = (int)(i - r * nq);
+    const int e = q / K, s = q - e * K;
+    const double2 v = qe[i];
+    const int64_t c = perm ? (int64_t)perm[r] : r;
+    out[((int64_t)s * n_en + e) * n + c] = v.x * v.x + v.y * v.y;
+}
+
 __global__ void k_qe_scale(double2* __restrict__ qe, int64_t total, int64_t nq,
                            const double* __restrict__ div) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -442,7 +457,7 @@ template <typename R>
 void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, const double* psi0,
                   int K, int T, double t_step, const double* energies, const int64_t* en_inds,
                   int n_en, int chk_step, double chk_thr, double* dos_corr, double* dc_corr,
-                  int* fail_step) {
+                  int* fail_step, double* qe_abs2) {
     using C = typename CplxT<R>::T;
     const hipStream_t st = h->stream;
     Engine<R> e;
@@ -485,6 +500,26 @@ void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
     launch_reduce(e.part.template as<double>(), nvn, e.plan.inst_chunk_dev.template as<int>(), 1,
                   nbuf.as<double>(), 0, counter.as<int>(), 0, st);
     const StepPoly sp = time_poly(bes, nb);
+    // Check at t == chk_step on psi(+-) after chk_step-1 steps vs <psi0|psi0>
+    // (tbpm.f90:608-611, 772-775).
+    auto check_norms = [&]() {
+        if (chk_step < 1 || chk_step > T) return;
+        std::vector<double> hn((size_t)(T + 1) * nvn);
+        TBG_HIP(hipMemcpyAsync(hn.data(), nbuf.p, hn.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        TBG_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k < 2 * K; ++k) {
+            const double ref = hn[k * 3 + 2];
+            const double nn = hn[(size_t)(chk_step - 1) * nvn + k * 3 + 2];
+            if (broken(nn, ref, chk_thr)) {
+                if (fail_step) *fail_step = chk_step;
+                throw Error{TBG_ENORM, "wave function norm broken at step " +
+                                           std::to_string(chk_step) +
+                                           "; check the Hermiticity of the Hamiltonian or "
+                                           "increase the rescale factor"};
+            }
+        }
+    };
     auto after1 = [&]() {
         launch_reduce(e.part.template as<double>(), nvn, e.epi_range(), 1, nbuf.as<double>(), 0,
                       counter.as<int>(), 0, st);
@@ -524,6 +559,14 @@ void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
                 hipLaunchKernelGGL(k_qe_scale, dim3(nblk(n_pad * nq)), dim3(256), 0, st,
                                    qe.as<double2>(), n_pad * nq, nq, dadiv.as<double>());
                 TBG_HIP(hipGetLastError());
+                if (qe_abs2 != nullptr) {   // quasi-eigenstates: done after normalising
+                    hipLaunchKernelGGL(k_qe_abs2, dim3(nblk(n * nq)), dim3(256), 0, st,
+                                       qe.as<double2>(), n, K, n_en, H->perm_ptr(), qe_abs2);
+                    TBG_HIP(hipGetLastError());
+                    TBG_HIP(hipStreamSynchronize(st));
+                    check_norms();
+                    return;
+                }
             } else {
                 for (int64_t q = 0; q < nq; ++q) adiv[q] = std::hypot(hs[q * 3 + 0], hs[q * 3 + 1]);
                 TBG_HIP(hipMemcpyAsync(dadiv.p, adiv.data(), nq * sizeof(double),
@@ -561,11 +604,10 @@ void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
     };
     after2();  // t = 1, unpropagated
     if (T >= 2) e.run(T - 1, sp, 0, 0, 0, nullptr, counter.as<int>(), after2);
-    std::vector<double> hd((size_t)T * 2 * K), hc((size_t)T * nq2 * 2), hn((size_t)(T + 1) * nvn);
+    std::vector<double> hd((size_t)T * 2 * K), hc((size_t)T * nq2 * 2);
     TBG_HIP(hipMemcpyAsync(hd.data(), dbuf.p, hd.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     if (nq2 > 0)
         TBG_HIP(hipMemcpyAsync(hc.data(), cbuf.p, hc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    TBG_HIP(hipMemcpyAsync(hn.data(), nbuf.p, hn.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     TBG_HIP(hipStreamSynchronize(st));
     std::vector<double> od((size_t)K * T * 2), oc((size_t)K * 2 * std::max(n_en, 0) * T * 2);
     for (int s = 0; s < K; ++s)
@@ -583,20 +625,7 @@ void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
     if (!oc.empty())
         TBG_HIP(hipMemcpyAsync(dc_corr, oc.data(), oc.size() * sizeof(double), hipMemcpyHostToDevice, st));
     TBG_HIP(hipStreamSynchronize(st));
-    // Check at t == chk_step on psi(+-) after chk_step-1 steps vs <psi0|psi0>.
-    if (chk_step >= 1 && chk_step <= T) {
-        for (int k = 0; k < 2 * K; ++k) {
-            const double ref = hn[k * 3 + 2];
-            const double nn = hn[(size_t)(chk_step - 1) * nvn + k * 3 + 2];
-            if (broken(nn, ref, chk_thr)) {
-                if (fail_step) *fail_step = chk_step;
-                throw Error{TBG_ENORM, "wave function norm broken at step " +
-                                           std::to_string(chk_step) +
-                                           "; check the Hermiticity of the Hamiltonian or "
-                                           "increase the rescale factor"};
-            }
-        }
-    }
+    check_norms();
 }
 
 template void corr_ac_impl<float>(tbg_handle*, const tbg_ham*, const double*, int, const double*,
@@ -607,9 +636,9 @@ template void corr_ac_impl<double>(tbg_handle*, const tbg_ham*, const double*, i
                                    double*, int*);
 template void corr_dc_impl<float>(tbg_handle*, const tbg_ham*, const double*, int, const double*,
                                   int, int, double, const double*, const int64_t*, int, int, double,
-                                  double*, double*, int*);
+                                  double*, double*, int*, double*);
 template void corr_dc_impl<double>(tbg_handle*, const tbg_ham*, const double*, int, const double*,
                                    int, int, double, const double*, const int64_t*, int, int,
-                                   double, double*, double*, int*);
+                                   double, double*, double*, int*, double*);
 
 }  // namespace tbg

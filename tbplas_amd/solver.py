@@ -1,9 +1,10 @@
 """Drop-in ``Solver`` for the TBPM correlation functions.
 
 Mirrors tbplas/tbpm/solver.py: the constructor's output/config side effects
-(solver.py:84-153), ``calc_corr_dos`` (256-284), ``calc_corr_ac_cond``
-(317-352) and ``calc_corr_dc_cond`` (393-441) keep their signatures and
-return shapes.  The f2py Fortran calls are replaced by the HIP library;
+(solver.py:84-153), ``calc_corr_dos`` (256-284), ``calc_corr_ldos``
+(286-315), ``calc_corr_ac_cond`` (317-352), ``calc_corr_dc_cond``
+(393-441), ``calc_quasi_eigenstates`` (472-505) and ``calc_psi_t``
+(543-600) keep their signatures and return shapes.  The f2py Fortran calls are replaced by the HIP library;
 initial states come from the seeded host generator (tbplas_amd.states) and
 are averaged over samples exactly as the reference accumulates
 ``corr += corrval / n_ran_samples``.  A failed norm check raises
@@ -19,8 +20,8 @@ import numpy as np
 from . import _lib, states, tune
 from .config import Config
 from .sample import Sample
-from .tbpm import (Hamiltonian, bessel_series, corr_ac, corr_dc, corr_dos,
-                   fermi_cheb_coef)
+from .tbpm import (Hamiltonian, bessel_series, corr_ac, corr_dc, corr_dos, corr_ldos,
+                   fermi_cheb_coef, ldos_states, psi_t, quasi_eigenstates)
 
 __all__ = ["Solver", "H_BAR_EV"]
 
@@ -168,6 +169,95 @@ class Solver:
         corr = self.all_average(corr)
         self._save_data(corr, self._output["corr_DOS"])
         return corr
+
+    def calc_corr_ldos(self):
+        """(nr_time_steps+1,) complex128 LDOS correlation (solver.py:286-315):
+        random states kept on config.LDOS['site_indices'] and normalised
+        (tbpm.f90:151-156), propagated as for the DOS."""
+        time_step = self._get_time_step()
+        bes = self._get_bessel_series(time_step)
+        gpu = self._gpu()
+        h = self._handle()
+        self._sample.device = gpu["device"]
+        indptr, indices, data = self._sample.build_ham_csr_device()
+        num_sample = self._dist_sample()
+        self._echo_time_step_fs(time_step)
+        order, tiles = self._row_order(gpu, "tile_nm")
+        H = Hamiltonian(self._sample.num_orb, indptr, indices, data,
+                        precision=gpu["precision"], handle=h, order=order, tiles=tiles)
+        psi0 = ldos_states(self._states(num_sample), self._config.LDOS["site_indices"])
+        chk, thr = self._chk()
+        T = self._config.generic["nr_time_steps"]
+        per = max(1, min(64, int(gpu["max_vectors"])))
+        tune.apply(H, min(per, num_sample), bes, gpu["autotune"])
+        corr = np.zeros(T + 1, dtype=np.complex128)
+        for a, b in self._batches(num_sample, per):
+            c = corr_ldos(H, bes, psi0[a:b], T, chk, thr)
+            for s in range(b - a):
+                corr = corr + c[s] / num_sample
+        corr = self.all_average(corr)
+        self._save_data(corr, self._output["corr_LDOS"])
+        return corr
+
+    def calc_quasi_eigenstates(self, save_data=False):
+        """(n_energies, num_orb) float64 quasi-eigenstates at
+        config.quasi_eigenstates['energies'] (solver.py:472-505)."""
+        time_step = self._get_time_step()
+        bes = self._get_bessel_series(time_step)
+        gpu = self._gpu()
+        h = self._handle()
+        self._sample.device = gpu["device"]
+        indptr, indices, data = self._sample.build_ham_csr_device()
+        num_sample = self._dist_sample()
+        self._echo_time_step_fs(time_step)
+        order, tiles = self._row_order(gpu, "tile_nm")
+        H = Hamiltonian(self._sample.num_orb, indptr, indices, data,
+                        precision=gpu["precision"], handle=h, order=order, tiles=tiles)
+        energies = np.asarray(self._config.quasi_eigenstates["energies"], dtype=np.float64)
+        psi0 = self._states(num_sample)
+        chk, thr = self._chk()
+        T = self._config.generic["nr_time_steps"]
+        per = max(1, min(32, int(gpu["max_vectors"])))
+        tune.apply(H, 2 * min(per, num_sample), bes, gpu["autotune"])
+        out = np.zeros((len(energies), self._sample.num_orb), dtype=np.float64)
+        for a, b in self._batches(num_sample, per):
+            q = quasi_eigenstates(H, bes, psi0[a:b], T, time_step, energies, chk, thr)
+            for s in range(b - a):
+                out = out + q[s] / num_sample
+        out = self.all_average(out)
+        if save_data:
+            self._save_data(out, self._output["qe"])
+        return out
+
+    def calc_psi_t(self, psi_0, time_log, dt_scale=1.0, save_data=False):
+        """(num_time, num_orb) complex128 wave function psi_0 propagated and
+        logged at the steps of time_log (solver.py:543-600): psi_0 is
+        normalised, time_log de-duplicated and sorted, every entry must lie
+        in [0, nr_time_steps]."""
+        if self.size != 1:
+            raise RuntimeError("Using more than 1 mpi process is not allowed"
+                               " for wave function propagation")
+        psi_0 = np.array(psi_0, dtype=np.complex128)
+        psi_0 /= np.linalg.norm(psi_0)
+        time_log = np.array(sorted(set(int(t) for t in time_log)), dtype=np.int64)
+        T = self._config.generic["nr_time_steps"]
+        for it in time_log:
+            if it not in range(T + 1):
+                raise ValueError(f"time {it} out of range")
+        time_step = self._get_time_step() * dt_scale
+        bes = self._get_bessel_series(time_step)
+        gpu = self._gpu()
+        h = self._handle()
+        self._sample.device = gpu["device"]
+        indptr, indices, data = self._sample.build_ham_csr_device()
+        self._echo_time_step_fs(time_step)
+        H = Hamiltonian(self._sample.num_orb, indptr, indices, data,
+                        precision=gpu["precision"], handle=h)
+        chk, thr = self._chk()
+        out = psi_t(H, bes, psi_0, time_log, T, chk, thr)
+        if save_data:
+            self._save_data(out, self._output["psi_t"])
+        return out
 
     def calc_corr_ac_cond(self):
         """(4, nr_time_steps) complex128 AC correlation (solver.py:317-352)."""

@@ -12,7 +12,8 @@ import numpy as np
 from . import _lib
 
 __all__ = ["Hamiltonian", "DosRun", "bessel_series", "fermi_cheb_coef", "corr_dos",
-           "corr_dos_batch", "corr_ac", "corr_dc", "cheb_step", "cheb_poly",
+           "corr_dos_batch", "corr_ac", "corr_dc", "corr_ldos", "ldos_states", "psi_t",
+           "quasi_eigenstates", "cheb_step", "cheb_poly",
            "precision_code"]
 
 
@@ -191,6 +192,76 @@ def corr_dos(H, bes, psi0, n_timestep, chk_step=128, chk_thr=1e-9):
     rc = _lib.lib().tbg_corr_dos(H.handle.ptr, H.ptr, bes.ctypes.data, len(bes), _lib.ptr(p),
                                  K, int(n_timestep), int(chk_step), float(chk_thr),
                                  _lib.ptr(out), ctypes.byref(fs))
+    _lib.check(rc, fs)
+    return out.cpu().numpy()
+
+
+def ldos_states(psi, site_indices):
+    """LDOS initial states: each random state kept on the selected sites and
+    normalised (tbpm.f90:151-156).  psi (K, n) -> (K, n) complex128."""
+    psi = np.atleast_2d(np.asarray(psi, dtype=np.complex128))
+    out = np.zeros_like(psi)
+    idx = np.asarray(site_indices, dtype=np.int64)
+    out[:, idx] = psi[:, idx]
+    norm = np.sqrt(np.einsum("kj,kj->k", out.conj(), out).real)
+    return out / norm[:, None]
+
+
+def corr_ldos(H, bes, psi0, n_timestep, chk_step=128, chk_thr=1e-9):
+    """Per-state LDOS correlation (K, T+1) <psi0|psi(t)> (tbpm_ldos,
+    tbpm.f90:97-189); psi0 are LDOS states (``ldos_states``)."""
+    torch = _lib.torch_cuda()
+    dv = H.handle.device
+    p, (K, n) = _states_dev(psi0, dv)
+    if n != H.n:
+        raise ValueError("state length does not match the Hamiltonian")
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    out = torch.empty((K, n_timestep + 1), dtype=torch.complex128, device=f"cuda:{dv}")
+    fs = ctypes.c_int(0)
+    rc = _lib.lib().tbg_corr_ldos(H.handle.ptr, H.ptr, bes.ctypes.data, len(bes), _lib.ptr(p),
+                                  K, int(n_timestep), int(chk_step), float(chk_thr),
+                                  _lib.ptr(out), ctypes.byref(fs))
+    _lib.check(rc, fs)
+    return out.cpu().numpy()
+
+
+def psi_t(H, bes, psi0, time_log, n_timestep, chk_step=128, chk_thr=1e-9):
+    """Wave function psi0 (n,) propagated n_timestep steps, logged after the
+    steps in time_log (strictly increasing, in [0, T]): (n_log, n)
+    complex128 (tbpm_psi_t, tbpm.f90:1029-1093)."""
+    torch = _lib.torch_cuda()
+    dv = H.handle.device
+    p, (K, n) = _states_dev(np.atleast_2d(psi0), dv)
+    if K != 1 or n != H.n:
+        raise ValueError("psi0 must be one state of the Hamiltonian's size")
+    tl = np.ascontiguousarray(time_log, dtype=np.int64)
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    out = torch.zeros((len(tl), n), dtype=torch.complex128, device=f"cuda:{dv}")
+    fs = ctypes.c_int(0)
+    rc = _lib.lib().tbg_psi_t(H.handle.ptr, H.ptr, bes.ctypes.data, len(bes), _lib.ptr(p),
+                              tl.ctypes.data, len(tl), int(n_timestep), int(chk_step),
+                              float(chk_thr), _lib.ptr(out), ctypes.byref(fs))
+    _lib.check(rc, fs)
+    return out.cpu().numpy()
+
+
+def quasi_eigenstates(H, bes, psi0, n_timestep, t_step, energies, chk_step=128, chk_thr=1e-9):
+    """Per-state |QE|^2 (K, n_en, n) of the quasi-eigenstates at the given
+    energies (tbpm_eigenstates, tbpm.f90:721-830); the reference's output
+    is their average over states."""
+    torch = _lib.torch_cuda()
+    dv = H.handle.device
+    p, (K, n) = _states_dev(psi0, dv)
+    if n != H.n:
+        raise ValueError("state length does not match the Hamiltonian")
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    en = np.ascontiguousarray(energies, dtype=np.float64)
+    out = torch.empty((K, len(en), n), dtype=torch.float64, device=f"cuda:{dv}")
+    fs = ctypes.c_int(0)
+    rc = _lib.lib().tbg_quasi_eigenstates(H.handle.ptr, H.ptr, bes.ctypes.data, len(bes),
+                                          _lib.ptr(p), K, int(n_timestep), float(t_step),
+                                          en.ctypes.data, len(en), int(chk_step),
+                                          float(chk_thr), _lib.ptr(out), ctypes.byref(fs))
     _lib.check(rc, fs)
     return out.cpu().numpy()
 

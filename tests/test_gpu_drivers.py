@@ -1,0 +1,171 @@
+"""The remaining Chebyshev drivers on the same engine: LDOS correlation,
+wave-function propagation psi(t) and quasi-eigenstates, against the oracle
+(pinned in test_oracle.py against the reference's own subroutines and
+closed forms) on identical states, in both precisions, in the caller's row
+order and with spatial tiles (three terms per pass); and the Solver
+methods that drive them."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import tbplas_amd as tb
+from tbplas_amd import states, tbpm
+
+pytestmark = pytest.mark.gpu
+needs_ref = pytest.mark.skipif(not oracle.ref_available(),
+                               reason="oracle/_ref not built (reference sources absent)")
+
+
+def _sample(L=14, W=0.6, field=0.0, seed=3):
+    s = tb.make_graphene_sample(L, vacancies=tb.random_vacancies(2 * L * L, 0.02, seed))
+    tb.add_anderson_disorder(s, W, seed + 1)
+    if field:
+        s.set_magnetic_field(field, 0)
+    s.rescale_ham(9.0)
+    return s
+
+
+def _ham(s, prec, layout, handle=None):
+    ip, ix, dt = oracle.build_ham_csr(s.orb_eng, s.hop_i, s.hop_j, s.hop_v)
+    kw = {}
+    if layout == "tiles":
+        kw["order"], kw["tiles"] = s.tile_order(0.8, return_tiles=True)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec, handle=handle, **kw)
+    if layout == "tiles":
+        assert H.blocking_info(4)[0] > 1
+    return H, (ip, ix, dt)
+
+
+def _tol(prec):
+    return 1e-12 if prec == "fp64" else 3e-5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("layout", ["caller", "tiles"])
+def test_ldos_vs_oracle(gpu, prec, layout):
+    s = _sample(field=20.0 if layout == "caller" else 0.0)
+    H, (ip, ix, dt) = _ham(s, prec, layout)
+    bes = tbpm.bessel_series(2 * math.pi / s.energy_range * s.rescale, 1e-15)
+    psi = tbpm.ldos_states(states.random_states(s.num_orb, states.sample_seeds(11, 4)),
+                           [0, 5, 17, 40, 41])
+    assert np.allclose(np.linalg.norm(psi, axis=1), 1.0)
+    T = 40
+    g = tbpm.corr_ldos(H, bes, psi, T, chk_thr=1e-4)
+    r = oracle.orc_tbpm_ldos(ip, ix, dt, bes, psi, T)
+    assert g.shape == (4, T + 1)
+    assert np.max(np.abs(g - r)) < _tol(prec)
+    H.close()
+
+
+@needs_ref
+def test_ldos_vs_reference_live(gpu):
+    """Averaged over the reference's own random states: the reference's
+    tbpm_ldos output."""
+    s = _sample(L=8)
+    H, (ip, ix, dt) = _ham(s, "fp64", "caller")
+    bes = tbpm.bessel_series(2 * math.pi / s.energy_range * s.rescale, 1e-14)
+    S, T, seed, sites = 3, 30, 7, [2, 9, 33]
+    ref = oracle.ref_tbpm_ldos(sites, bes, ip, ix, dt, seed, T, S)
+    psi = np.array([oracle.ref_random_state(s.num_orb, seed * (i + 1)) for i in range(S)])
+    g = tbpm.corr_ldos(H, bes, tbpm.ldos_states(psi, sites), T)
+    avg = np.zeros(T + 1, dtype=complex)
+    for i in range(S):
+        avg = avg + g[i] / S
+    assert np.max(np.abs(avg - ref)) < 1e-12
+    H.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("layout", ["caller", "tiles"])
+@pytest.mark.parametrize("tl", [[0, 1, 2, 25, 40], [40], [0], [3, 17, 39]])
+def test_psi_t_vs_oracle(gpu, prec, layout, tl):
+    s = _sample()
+    H, (ip, ix, dt) = _ham(s, prec, layout)
+    bes = tbpm.bessel_series(2 * math.pi / s.energy_range * s.rescale, 1e-15)
+    p0 = states.random_states(s.num_orb, states.sample_seeds(5, 1))[0]
+    T = 40
+    g = tbpm.psi_t(H, bes, p0, tl, T, chk_step=20, chk_thr=1e-4)
+    r = oracle.orc_psi_t(ip, ix, dt, bes, p0, tl, T)
+    assert g.shape == (len(tl), s.num_orb)
+    assert np.max(np.abs(g - r)) < _tol(prec)
+    if 0 in tl:   # the initial state as stored (complex64 rounds it)
+        stored = p0 if prec == "fp64" else p0.astype(np.complex64).astype(np.complex128)
+        np.testing.assert_array_equal(g[tl.index(0)], stored)
+    H.close()
+
+
+def test_psi_t_rejects_bad_time_log(gpu):
+    s = _sample(L=6)
+    H, _ = _ham(s, "fp64", "caller")
+    bes = tbpm.bessel_series(math.pi, 1e-14)
+    p0 = states.random_states(s.num_orb, states.sample_seeds(5, 1))[0]
+    for bad in ([3, 1], [0, 0], [-1], [11]):
+        with pytest.raises(tb._lib.TBGError):
+            tbpm.psi_t(H, bes, p0, bad, 10)
+    H.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("layout", ["caller", "tiles"])
+def test_quasi_eigenstates_vs_oracle(gpu, prec, layout):
+    s = _sample(L=10)
+    H, (ip, ix, dt) = _ham(s, prec, layout)
+    t_step = 2 * math.pi / s.energy_range
+    bes = tbpm.bessel_series(t_step * s.rescale, 1e-14)
+    psi = states.random_states(s.num_orb, states.sample_seeds(9, 3))
+    T, en = 32, [-1.0, 0.0, 0.35, 2.0]
+    g = tbpm.quasi_eigenstates(H, bes, psi, T, t_step, en, chk_thr=1e-4)
+    r = oracle.orc_quasi_eigenstates(ip, ix, dt, bes, psi, T, t_step, en)
+    assert g.shape == (3, len(en), s.num_orb)
+    assert np.max(np.abs(g - r)) < _tol(prec) * max(1.0, np.max(r)) * 10
+    assert np.allclose(g.sum(axis=2), 1.0, atol=1e-12 if prec == "fp64" else 1e-5)
+    H.close()
+
+
+def _config(T, S, precision="fp64"):
+    c = tb.Config()
+    c.generic["nr_time_steps"] = T
+    c.generic["nr_random_samples"] = S
+    c.gpu["precision"] = precision
+    c.gpu["autotune"] = False
+    return c
+
+
+def test_solver_ldos_qe_psi_t(gpu, tmp_path):
+    s = _sample(L=10, W=0.0)
+    T, S = 24, 3
+    cfg = _config(T, S)
+    cfg.LDOS["site_indices"] = [0, 1, 2, 50]
+    cfg.quasi_eigenstates["energies"] = [-0.5, 0.0, 0.5]
+    solver = tb.Solver(s, cfg, directory=str(tmp_path / "sim"), prefix="d", echo_details=False)
+    ip, ix, dt = oracle.build_ham_csr(s.orb_eng, s.hop_i, s.hop_j, s.hop_v)
+    t_step = 2 * math.pi / s.energy_range
+    bes = oracle.bessel_series(t_step * s.rescale, cfg.generic["Bessel_precision"])
+    psi = states.random_states(s.num_orb, states.sample_seeds(1337, S))
+
+    ldos = solver.calc_corr_ldos()
+    assert ldos.shape == (T + 1,) and (tmp_path / "sim" / "d.corr_LDOS.npy").exists()
+    r = oracle.orc_tbpm_ldos(ip, ix, dt, bes, tbpm.ldos_states(psi, [0, 1, 2, 50]), T)
+    ref = np.zeros(T + 1, dtype=complex)
+    for i in range(S):
+        ref = ref + r[i] / S
+    assert np.max(np.abs(ldos - ref)) < 1e-12
+
+    qe = solver.calc_quasi_eigenstates(save_data=True)
+    assert qe.shape == (3, s.num_orb) and (tmp_path / "sim" / "d.qe.npy").exists()
+    q = oracle.orc_quasi_eigenstates(ip, ix, dt, bes, psi, T, t_step, [-0.5, 0.0, 0.5])
+    qref = np.zeros_like(q[0])
+    for i in range(S):
+        qref = qref + q[i] / S
+    assert np.max(np.abs(qe - qref)) < 1e-12
+
+    p0 = np.zeros(s.num_orb, dtype=complex)
+    p0[7] = 2.0
+    frames = solver.calc_psi_t(p0, [5, 0, 5, 24], save_data=True)
+    assert frames.shape == (3, s.num_orb)
+    pr = oracle.orc_psi_t(ip, ix, dt, bes, p0 / 2.0, [0, 5, 24], T)
+    assert np.max(np.abs(frames - pr)) < 1e-12
+    with pytest.raises(ValueError):
+        solver.calc_psi_t(p0, [T + 1])

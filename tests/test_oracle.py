@@ -260,3 +260,110 @@ def test_analyzer_loops_textbook():
     e, d = oracle.analyzer_calc_dos(corr, T, en_range)
     assert abs(e[np.argmax(d)] - E0) <= 0.5 * en_range / T + 1e-12
     assert abs(np.sum(d) * 0.5 * en_range / T - 1.0) < 1e-12
+
+
+# -- remaining Chebyshev drivers: LDOS, psi(t), quasi-eigenstates ---------------
+def _dense_eig(ip, ix, dt):
+    n = len(ip) - 1
+    Hd = np.zeros((n, n), dtype=complex)
+    for i in range(n):
+        for p in range(ip[i], ip[i + 1]):
+            Hd[i, ix[p]] += dt[p]
+    return np.linalg.eigh(Hd)
+
+
+def _small_disordered(L=5, seed=13):
+    orb, hi, hj, hv, dr, pos = _graphene(L)
+    rng = np.random.default_rng(seed)
+    orb = rng.uniform(-0.05, 0.05, len(orb))
+    hv = hv * np.exp(1j * rng.uniform(-1, 1, len(hv)))
+    return oracle.build_ham_csr(orb, hi, hj, hv)
+
+
+def test_orc_ldos_psi_t_vs_dense_exact():
+    """LDOS correlation sum_n |<phi_n|psi0>|^2 e^{-i E_n t tau} and the
+    logged frames V e^{-i E t tau} V^H psi0 (closed forms)."""
+    ip, ix, dt = _small_disordered()
+    n = len(ip) - 1
+    tau = math.pi
+    bes = oracle.bessel_series(tau, 1e-15)
+    rng = np.random.default_rng(5)
+    psi = np.exp(2j * np.pi * rng.random(n))
+    sites = [0, 3, 7, 11]
+    w0 = np.zeros(n, complex)
+    w0[sites] = psi[sites]
+    w0 /= np.linalg.norm(w0)
+    E, V = _dense_eig(ip, ix, dt)
+    c = V.conj().T @ w0
+    T = 30
+    ldos = oracle.orc_tbpm_ldos(ip, ix, dt, bes, w0[None, :], T)[0]
+    exact = np.array([np.sum(np.abs(c) ** 2 * np.exp(-1j * E * t * tau)) for t in range(T + 1)])
+    assert np.max(np.abs(ldos - exact)) < 1e-12
+    tl = [0, 1, 7, 30]
+    fr = oracle.orc_psi_t(ip, ix, dt, bes, w0, tl, T)
+    for i, t in enumerate(tl):
+        assert np.max(np.abs(fr[i] - V @ (np.exp(-1j * E * t * tau) * c))) < 1e-12
+
+
+def test_orc_quasi_eigenstates_vs_dense_exact():
+    """QE(E) = sum_n phi_n <phi_n|psi0> (1 + 2 sum_k W_k cos((E - E_n) k tau)),
+    normalised (the spectral form of tbpm.f90:776-812)."""
+    ip, ix, dt = _small_disordered(seed=17)
+    n = len(ip) - 1
+    tau = math.pi
+    bes = oracle.bessel_series(tau, 1e-15)
+    rng = np.random.default_rng(9)
+    psi = np.exp(2j * np.pi * rng.random((2, n)))
+    psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+    E, V = _dense_eig(ip, ix, dt)
+    T, energies = 40, [-0.2, 0.0, 0.13]
+    q = oracle.orc_quasi_eigenstates(ip, ix, dt, bes, psi, T, tau, energies)
+    k = np.arange(1, T + 1)
+    W = 0.5 * (1 + np.cos(np.pi * k / T))
+    for s in range(2):
+        c = V.conj().T @ psi[s]
+        for i, e in enumerate(energies):
+            f = 1 + 2 * np.sum(W[None, :] * np.cos(np.outer(e - E, k) * tau), axis=1)
+            qe = V @ (c * f)
+            qe /= np.linalg.norm(qe)
+            assert np.max(np.abs(q[s, i] - np.abs(qe) ** 2)) < 1e-12
+    assert np.allclose(q.sum(axis=2), 1.0, atol=1e-13)
+
+
+@needs_ref
+def test_orc_ldos_psi_t_qe_match_reference_live():
+    orb, hi, hj, hv, dr, pos = _graphene(6)
+    rng = np.random.default_rng(21)
+    orb = rng.uniform(-0.3, 0.3, len(orb)) / 9.0
+    ip, ix, dt = oracle.build_ham_csr(orb, hi, hj, hv)
+    n = len(orb)
+    bes = oracle.bessel_series(math.pi, 1e-14)
+    S, T, seed = 2, 24, 5
+    sites = [1, 4, 9, 30]
+    # LDOS: the reference's own random states kept on the sites
+    ref = oracle.ref_tbpm_ldos(sites, bes, ip, ix, dt, seed, T, S)
+    psi = np.array([oracle.ref_random_state(n, seed * (s + 1)) for s in range(S)])
+    w0 = np.zeros_like(psi)
+    w0[:, sites] = psi[:, sites]
+    w0 /= np.linalg.norm(w0, axis=1, keepdims=True)
+    c = oracle.orc_tbpm_ldos(ip, ix, dt, bes, w0, T)
+    avg = np.zeros(T + 1, dtype=complex)
+    for s in range(S):
+        avg = avg + c[s] / S
+    assert np.max(np.abs(avg - ref)) < 1e-13
+    # psi(t), log pointer quirks included (0, T and a repeated last entry)
+    p0 = psi[0] / np.linalg.norm(psi[0])
+    for tl in ([0, 3, T], [2, 5], [T]):
+        r = oracle.ref_tbpm_psi_t(bes, ip, ix, dt, p0, tl, T)
+        o = oracle.orc_psi_t(ip, ix, dt, bes, p0, tl, T)
+        assert np.max(np.abs(r - o)) < 1e-13
+    # quasi-eigenstates averaged over the reference's states
+    t_step = math.pi / 9.0
+    en = [-0.5, 0.0, 0.7]
+    r = oracle.ref_tbpm_eigenstates(bes, ip, ix, dt, seed, T, S, t_step, en)
+    q = oracle.orc_quasi_eigenstates(ip, ix, dt, bes, psi, T, t_step, en)
+    avg = np.zeros_like(q[0])
+    for s in range(S):
+        avg = avg + q[s] / S
+    assert r.shape == (len(en), n)
+    assert np.max(np.abs(avg - r)) < 1e-13 * max(1.0, np.max(r))
