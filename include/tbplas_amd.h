@@ -233,6 +233,21 @@ int tbg_corr_dc(tbg_handle *h, const tbg_ham *H, const double *bes, int n_bes,
                 int chk_step, double chk_thr, double *dos_corr,
                 double *dc_corr, int *fail_step);
 
+/* Replaces f2py tbpm_dyn_pol (tbplas/fortran/tbpm.f90:355-493; called from
+ * tbplas/tbpm/solver.py:373-392).  coef_F / coef_omF: host Fermi and
+ * one-minus-Fermi Chebyshev coefficients (as for tbg_corr_ac); q_points:
+ * host (n_q, 3); orb_pos: host (n, 3) orbital positions in the caller's
+ * order (density e^{i q.r}, funcs.f90:125-149).  corr: device float64
+ * (K, n_q, T) = Im <F psi0 | e^{iq.r} U^{t-1} (1-F) e^{-iq.r} psi0> per state
+ * (U^{t-1} acting on both), the reference's corr is the mean over states.
+ * At most 32 states per call. */
+int tbg_corr_dyn_pol(tbg_handle *h, const tbg_ham *H, const double *bes,
+                     int n_bes, const double *coef_F, int n_F,
+                     const double *coef_omF, int n_omF, const double *psi0,
+                     int K, int T, const double *q_points, int n_q,
+                     const double *orb_pos, int chk_step, double chk_thr,
+                     double *corr, int *fail_step);
+
 /* Replaces f2py tbpm_eigenstates (tbplas/fortran/tbpm.f90:721-830; called
  * from tbplas/tbpm/solver.py:472-505).  energies: host, n_en energies (eV,
  * with t_step in hbar/eV as for tbg_corr_dc).  qe: device float64

@@ -10,6 +10,7 @@ __all__ = [
     "oracle_lib", "ref_lib", "ref_available",
     "orc_tbpm_dos", "orc_tbpm_accond", "orc_tbpm_dccond", "orc_step",
     "orc_fermi_coef", "orc_csr_mv", "orc_tbpm_ldos", "orc_psi_t", "orc_quasi_eigenstates",
+    "orc_cheb_apply", "orc_tbpm_dyn_pol", "ref_tbpm_dyn_pol",
     "ref_random_state", "ref_tbpm_dos", "ref_tbpm_accond", "ref_tbpm_dccond",
     "ref_fermi_coef", "ref_tbpm_ldos", "ref_tbpm_psi_t", "ref_tbpm_eigenstates",
     "bessel_series", "build_ham_csr", "build_ham_dxy_loop",
@@ -233,6 +234,48 @@ def orc_quasi_eigenstates(indptr, indices, hop, bes, psi0, n_timestep, t_step, e
     return out
 
 
+def orc_cheb_apply(indptr, indices, hop, coef, psi):
+    """Fermi (propagation.f90:52-82): sum_k c_k T_k(H) psi by the recurrence
+    T_0 = psi, T_1 = H psi, T_k = 2 H T_{k-1} - T_{k-2}, in that order of
+    accumulation; H products from orc_csr_mv."""
+    p0 = np.asarray(psi, dtype=np.complex128).copy()
+    out = coef[0] * p0
+    if len(coef) == 1:
+        return out
+    p1 = orc_csr_mv(indptr, indices, hop, p0)
+    out = out + coef[1] * p1
+    for k in range(2, len(coef)):
+        p2 = 2.0 * orc_csr_mv(indptr, indices, hop, p1) - p0
+        out = out + coef[k] * p2
+        p0, p1 = p1, p2
+    return out
+
+
+def orc_tbpm_dyn_pol(indptr, indices, hop, bes, coef_F, coef_omF, psi0, n_timestep,
+                     q_points, orb_pos):
+    """tbpm_dyn_pol (tbpm.f90:355-493) on caller states psi0 (S, n): per
+    q-point, psi1 = (1-F) (density(-q) psi0), psi2 = F psi0, corr(t) =
+    Im <psi2 | density(q) psi1> for t = 1 (no step) .. T, both vectors one
+    step further each time (density_coef, funcs.f90:125-149).  Returns
+    (S, n_q, T)."""
+    psi0 = np.atleast_2d(np.asarray(psi0, dtype=np.complex128))
+    q = np.atleast_2d(np.asarray(q_points, dtype=np.float64))
+    pos = np.asarray(orb_pos, dtype=np.float64)
+    out = np.zeros((psi0.shape[0], len(q), n_timestep))
+    for iq, qq in enumerate(q):
+        pw = qq[0] * pos[:, 0] + qq[1] * pos[:, 1] + qq[2] * pos[:, 2]
+        rho = np.cos(pw) + 1j * np.sin(pw)
+        for s, w0 in enumerate(psi0):
+            psi1 = orc_cheb_apply(indptr, indices, hop, coef_omF, np.conj(rho) * w0)
+            psi2 = orc_cheb_apply(indptr, indices, hop, coef_F, w0)
+            out[s, iq, 0] = np.vdot(psi2, rho * psi1).imag
+            for t in range(1, n_timestep):
+                psi1 = orc_step(indptr, indices, hop, bes, psi1, True)
+                psi2 = orc_step(indptr, indices, hop, bes, psi2, True)
+                out[s, iq, t] = np.vdot(psi2, rho * psi1).imag
+    return out
+
+
 def orc_csr_mv(indptr, indices, val, x):
     """csr_mv (csr.F90:71-101)."""
     lib = oracle_lib()
@@ -413,6 +456,34 @@ def ref_tbpm_eigenstates(bes, indptr, indices, hop, seed, n_timestep, n_samples,
                           r(I(n_timestep)), r(I(n_samples)), r(D(t_step)), _ptr(en),
                           r(I(len(en))), _ptr(out), r(I(0)), r(I(chk_step)), r(D(chk_thr)))
     return np.ascontiguousarray(out.T)
+
+
+def ref_tbpm_dyn_pol(bes, beta_re, mu_re, indptr, indices, hop, rescale, dx, dy, orb_pos,
+                     seed, n_timestep, n_samples, q_points, nr_fermi=2 ** 15, fermi_prec=1e-12,
+                     chk_step=128, chk_thr=1e-9):
+    """The reference tbpm_dyn_pol (tbpm.f90:355-493): (n_q, T)."""
+    lib = ref_lib()
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    ip, ix = _i32(indptr), _i32(indices)
+    hop = np.ascontiguousarray(hop, dtype=np.complex128)
+    dx = np.ascontiguousarray(dx, dtype=np.float64)
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    pos = np.asarray(orb_pos, dtype=np.float64)
+    sx, sy, sz = (np.ascontiguousarray(pos[:, i]) for i in range(3))
+    q = np.atleast_2d(np.asarray(q_points, dtype=np.float64))
+    qf = np.ascontiguousarray(q.T)                      # Fortran (n_q, 3)
+    corr = np.zeros((n_timestep, len(q)))               # Fortran (n_q, T)
+    fn, fl = _fstr("unused")
+    I, D = ctypes.c_int, ctypes.c_double
+    r = ctypes.byref
+    lib.tbpm_dyn_pol_(_ptr(bes), r(I(len(bes))), r(D(beta_re)), r(D(mu_re)), _ptr(ip, I),
+                      r(I(len(ip))), _ptr(ix, I), r(I(len(ix))), _ptr(hop), r(I(len(hop))),
+                      r(D(rescale)), _ptr(dx), r(I(len(dx))), _ptr(dy), r(I(len(dy))),
+                      _ptr(sx), r(I(len(sx))), _ptr(sy), r(I(len(sy))), _ptr(sz), r(I(len(sz))),
+                      r(I(seed)), r(I(n_timestep)), r(I(n_samples)), r(I(nr_fermi)),
+                      r(D(fermi_prec)), _ptr(qf), r(I(len(q))), fn, _ptr(corr), r(I(0)),
+                      r(I(chk_step)), r(D(chk_thr)), fl)
+    return np.ascontiguousarray(corr.T)
 
 
 def ref_fermi_coef(nr_fermi, beta_re, mu_re, one_minus, eps):

@@ -787,6 +787,27 @@ int tbg_corr_dc(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
     });
 }
 
+int tbg_corr_dyn_pol(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
+                     const double* coef_F, int n_F, const double* coef_omF, int n_omF,
+                     const double* psi0, int K, int T, const double* q_points, int n_q,
+                     const double* orb_pos, int chk_step, double chk_thr, double* corr,
+                     int* fail_step) {
+    return guard([&] {
+        check_handle(h);
+        check_psi_args(H, psi0, K, T);
+        TBG_REQUIRE(bes && n_bes >= 3 && corr, "need >= 3 Bessel terms and an output");
+        TBG_REQUIRE(coef_F && n_F >= 1 && coef_omF && n_omF >= 1, "need Fermi coefficients");
+        TBG_REQUIRE(q_points && n_q >= 1 && orb_pos, "need q-points and orbital positions");
+        TBG_REQUIRE(2 * K <= 64, "at most 32 states per call");
+        if (H->precision == TBG_FP32)
+            dyn_pol_impl<float>(h, H, bes, n_bes, coef_F, n_F, coef_omF, n_omF, psi0, K, T,
+                                q_points, n_q, orb_pos, chk_step, chk_thr, corr, fail_step);
+        else
+            dyn_pol_impl<double>(h, H, bes, n_bes, coef_F, n_F, coef_omF, n_omF, psi0, K, T,
+                                 q_points, n_q, orb_pos, chk_step, chk_thr, corr, fail_step);
+    });
+}
+
 int tbg_quasi_eigenstates(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
                           const double* psi0, int K, int T, double t_step,
                           const double* energies, int n_en, int chk_step, double chk_thr,

@@ -29,6 +29,12 @@ void corr_ac_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
                   int chk_step, double chk_thr, double* corr, int* fail_step);
 
 template <typename R>
+void dyn_pol_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, const double* cF,
+                  int nF, const double* cO, int nO, const double* psi0, int K, int T,
+                  const double* q_points, int n_q, const double* pos, int chk_step,
+                  double chk_thr, double* corr, int* fail_step);
+
+template <typename R>
 void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, const double* psi0,
                   int K, int T, double t_step, const double* energies, const int64_t* en_inds,
                   int n_en, int chk_step, double chk_thr, double* dos_corr, double* dc_corr,

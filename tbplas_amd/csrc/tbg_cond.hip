@@ -369,6 +369,54 @@ k_dcdot(const int64_t* __restrict__ chunk, const typename CplxT<R>::T* __restric
     }
 }
 
+// Dynamic polarisation, per sample s: slots s (psi1), K+s (psi2):
+// part[c][s] = Im sum_r conj(psi2_r) rho_r psi1_r, rho_r = e^{i q.r_r}
+// (inner_prod(psi2, density_q .pMul. psi1), tbpm.f90:455-478).
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_rhocorr(const int64_t* __restrict__ chunk, const typename CplxT<R>::T* __restrict__ X, int Kp,
+          int K, const double2* __restrict__ rho, double* __restrict__ part) {
+    __shared__ double sh[256];
+    const int c = remap_chunk(blockIdx.x, gridDim.x);
+    const int64_t r0 = chunk[c] * 64, r1 = chunk[c + 1] * 64;
+    const int ng = 256 / K;
+    const int s = threadIdx.x % K, grp = threadIdx.x / K;
+    double d = 0.0;
+    if (grp < ng) {
+        for (int64_t r = r0 + grp; r < r1; r += ng) {
+            const auto a = X[r * Kp + s];
+            const auto b = X[r * Kp + K + s];
+            const double2 q = rho[r];
+            const double ar = a.x, ai = a.y;
+            const double wr = q.x * ar - q.y * ai, wi = q.x * ai + q.y * ar;
+            d = fma((double)b.x, wi, fma(-(double)b.y, wr, d));
+        }
+    }
+    sh[threadIdx.x] = d;
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double acc = 0.0;
+        for (int g = 0; g < ng; ++g) acc += sh[g * K + threadIdx.x];
+        part[(int64_t)c * K + threadIdx.x] = acc;
+    }
+}
+
+// Slots [0, K) of X times conj(rho) per layout row (density(-q) .pMul. wf0).
+template <typename R>
+__global__ void k_rho_conj_mul(typename CplxT<R>::T* __restrict__ X, int Kp, int K,
+                               const double2* __restrict__ rho, int64_t n_pad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pad * K) return;
+    const int64_t r = i / K;
+    const int s = (int)(i - r * K);
+    auto v = X[r * Kp + s];
+    const double2 q = rho[r];
+    const double vr = v.x, vi = v.y;
+    v.x = (R)(q.x * vr + q.y * vi);
+    v.y = (R)(q.x * vi - q.y * vr);
+    X[r * Kp + s] = v;
+}
+
 // ---------------------------------------------------------------------------
 // Drivers
 // ---------------------------------------------------------------------------
@@ -451,6 +499,99 @@ void corr_ac_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
             }
         }
     }
+}
+
+// tbpm_dyn_pol (tbpm.f90:355-493) on caller states: per q-point, psi1 =
+// (1 - F)(H) (density(-q) .pMul. psi0) and psi2 = F(H) psi0, propagated
+// together; corr(s, q, t) = Im <psi2 | density(q) .pMul. psi1> at t = 1
+// (before any step) .. T.  Norm check at loop index chk_step (2..T) on
+// psi2 after chk_step-2 steps against its norm after the Fermi operator.
+template <typename R>
+void dyn_pol_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, const double* cF,
+                  int nF, const double* cO, int nO, const double* psi0, int K, int T,
+                  const double* q_points, int n_q, const double* pos, int chk_step,
+                  double chk_thr, double* corr, int* fail_step) {
+    using C = typename CplxT<R>::T;
+    const hipStream_t st = h->stream;
+    Engine<R> e;
+    e.init(h, view_of(H), {0, H->nslices}, 2 * K, chunks_for(h, H->nslices), H);
+    const int Kp = e.Kp;
+    const int64_t n = H->n, n_pad = e.n_pad;
+    uint64_t cls = 0;
+    for (int k = K; k < 2 * K; ++k) cls |= 1ull << k;
+    const StepPoly fp = cheb_poly(cO, nO, cF, nF);
+    const StepPoly sp = time_poly(bes, nb);
+    const int nvn = 3 * Kp;
+    DevBuf rho((size_t)n_pad * sizeof(double2)), rho_in((size_t)n * sizeof(double2));
+    DevBuf nbuf((size_t)T * nvn * sizeof(double));
+    DevBuf cbuf((size_t)T * K * sizeof(double));
+    DevBuf counter(sizeof(int));
+    std::vector<double> hr((size_t)n * 2), hc((size_t)T * K), hn((size_t)T * nvn);
+    std::vector<double> out((size_t)K * n_q * T);
+    auto corr_now = [&]() {
+        hipLaunchKernelGGL(k_rhocorr<R>, dim3(e.plan.nchunks), dim3(256), 0, st,
+                           e.plan.chunk.template as<int64_t>(), e.X.template as<C>(), Kp, K,
+                           rho.as<double2>(), e.part.template as<double>());
+        launch_reduce(e.part.template as<double>(), K, e.plan.inst_chunk_dev.template as<int>(),
+                      1, cbuf.as<double>(), 0, counter.as<int>(), 0, st);
+    };
+    auto after = [&]() {
+        launch_reduce(e.part.template as<double>(), nvn, e.epi_range(), 1, nbuf.as<double>(), 0,
+                      counter.as<int>(), 0, st);
+        corr_now();
+    };
+    for (int iq = 0; iq < n_q; ++iq) {
+        // density(q) = e^{i q.r} per orbital (funcs.f90:125-149), caller order
+        const double* q = q_points + (size_t)iq * 3;
+        for (int64_t j = 0; j < n; ++j) {
+            const double pw = q[0] * pos[j * 3 + 0] + q[1] * pos[j * 3 + 1] + q[2] * pos[j * 3 + 2];
+            hr[j * 2 + 0] = std::cos(pw);
+            hr[j * 2 + 1] = std::sin(pw);
+        }
+        TBG_HIP(hipMemcpyAsync(rho_in.p, hr.data(), hr.size() * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+        launch_fill_zero(rho.p, rho.bytes, st);
+        launch_pack<double>(rho_in.as<double>(), 1, n, rho.p, 1, 0, H->perm_ptr(), st);
+        // psi1 = (1 - F) density(-q) psi0 in slots [0, K), psi2 = F psi0 in [K, 2K)
+        launch_fill_zero(e.X.p, e.vbytes, st);
+        launch_pack<R>(psi0, K, n, e.X.p, Kp, 0, H->perm_ptr(), st);
+        launch_pack<R>(psi0, K, n, e.X.p, Kp, K, H->perm_ptr(), st);
+        hipLaunchKernelGGL(k_rho_conj_mul<R>, dim3(nblk(n_pad * K)), dim3(256), 0, st,
+                           e.X.template as<C>(), Kp, K, rho.as<double2>(), n_pad);
+        TBG_HIP(hipGetLastError());
+        e.step(fp, 0, cls, 0, nullptr, nullptr);
+        TBG_HIP(hipMemsetAsync(counter.p, 0, sizeof(int), st));
+        launch_dotnorm<R>(e.plan, e.X.p, e.X.p, Kp, Kp, e.part.template as<double>(), st);
+        launch_reduce(e.part.template as<double>(), nvn, e.plan.inst_chunk_dev.template as<int>(),
+                      1, nbuf.as<double>(), 0, counter.as<int>(), 0, st);
+        corr_now();   // t = 1 (tbpm.f90:459-466)
+        if (T >= 2) e.run(T - 1, sp, 0, 0, F_NORM, nullptr, counter.as<int>(), after);
+        TBG_HIP(hipMemcpyAsync(hc.data(), cbuf.p, hc.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        TBG_HIP(hipMemcpyAsync(hn.data(), nbuf.p, hn.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        TBG_HIP(hipStreamSynchronize(st));
+        for (int s = 0; s < K; ++s)
+            for (int t = 0; t < T; ++t) out[((size_t)s * n_q + iq) * T + t] = hc[(size_t)t * K + s];
+        if (chk_step >= 2 && chk_step <= T) {
+            for (int s = 0; s < K; ++s) {
+                const int slot = K + s;
+                const double ref = hn[slot * 3 + 2];
+                const double nn = hn[(size_t)(chk_step - 2) * nvn + slot * 3 + 2];
+                if (broken(nn, ref, chk_thr)) {
+                    if (fail_step) *fail_step = chk_step;
+                    throw Error{TBG_ENORM, "wave function norm broken at step " +
+                                               std::to_string(chk_step) + " (state " +
+                                               std::to_string(s) + ", q-point " +
+                                               std::to_string(iq) + "); check the Hermiticity "
+                                               "of the Hamiltonian or increase the rescale factor"};
+                }
+            }
+        }
+    }
+    TBG_HIP(hipMemcpyAsync(corr, out.data(), out.size() * sizeof(double), hipMemcpyHostToDevice,
+                           st));
+    TBG_HIP(hipStreamSynchronize(st));
 }
 
 template <typename R>
@@ -634,6 +775,12 @@ template void corr_ac_impl<float>(tbg_handle*, const tbg_ham*, const double*, in
 template void corr_ac_impl<double>(tbg_handle*, const tbg_ham*, const double*, int, const double*,
                                    int, const double*, int, const double*, int, int, int, double,
                                    double*, int*);
+template void dyn_pol_impl<float>(tbg_handle*, const tbg_ham*, const double*, int, const double*,
+                                  int, const double*, int, const double*, int, int, const double*,
+                                  int, const double*, int, double, double*, int*);
+template void dyn_pol_impl<double>(tbg_handle*, const tbg_ham*, const double*, int,
+                                   const double*, int, const double*, int, const double*, int, int,
+                                   const double*, int, const double*, int, double, double*, int*);
 template void corr_dc_impl<float>(tbg_handle*, const tbg_ham*, const double*, int, const double*,
                                   int, int, double, const double*, const int64_t*, int, int, double,
                                   double*, double*, int*, double*);

@@ -2,9 +2,10 @@
 
 Mirrors tbplas/tbpm/solver.py: the constructor's output/config side effects
 (solver.py:84-153), ``calc_corr_dos`` (256-284), ``calc_corr_ldos``
-(286-315), ``calc_corr_ac_cond`` (317-352), ``calc_corr_dc_cond``
-(393-441), ``calc_quasi_eigenstates`` (472-505) and ``calc_psi_t``
-(543-600) keep their signatures and return shapes.  The f2py Fortran calls are replaced by the HIP library;
+(286-315), ``calc_corr_ac_cond`` (317-352), ``calc_corr_dyn_pol``
+(354-391), ``calc_corr_dc_cond`` (393-441), ``calc_quasi_eigenstates``
+(472-505) and ``calc_psi_t`` (543-600) keep their signatures and return
+shapes.  The f2py Fortran calls are replaced by the HIP library;
 initial states come from the seeded host generator (tbplas_amd.states) and
 are averaged over samples exactly as the reference accumulates
 ``corr += corrval / n_ran_samples``.  A failed norm check raises
@@ -20,8 +21,8 @@ import numpy as np
 from . import _lib, states, tune
 from .config import Config
 from .sample import Sample
-from .tbpm import (Hamiltonian, bessel_series, corr_ac, corr_dc, corr_dos, corr_ldos,
-                   fermi_cheb_coef, ldos_states, psi_t, quasi_eigenstates)
+from .tbpm import (Hamiltonian, bessel_series, corr_ac, corr_dc, corr_dos, corr_dyn_pol,
+                   corr_ldos, fermi_cheb_coef, ldos_states, psi_t, quasi_eigenstates)
 
 __all__ = ["Solver", "H_BAR_EV"]
 
@@ -291,6 +292,42 @@ class Solver:
                 corr = corr + c[s] / num_sample
         corr = self.all_average(corr)
         self._save_data(corr, self._output["corr_AC"])
+        return corr
+
+    def calc_corr_dyn_pol(self):
+        """(n_q_points, nr_time_steps) float64 dynamic-polarisation
+        correlation at config.dyn_pol['q_points'] (solver.py:354-391)."""
+        time_step = 0.5 * self._get_time_step()
+        bes = self._get_bessel_series(time_step)
+        beta_re, mu_re = self._get_beta_mu_re()
+        gpu = self._gpu()
+        h = self._handle()
+        self._sample.device = gpu["device"]
+        indptr, indices, data = self._sample.build_ham_csr_device()
+        num_sample = self._dist_sample()
+        self._echo_time_step_fs(time_step)
+        g = self._config.generic
+        coef_F = fermi_cheb_coef(g["nr_Fermi_fft_steps"], beta_re, mu_re, False,
+                                 g["Fermi_cheb_precision"])
+        coef_omF = fermi_cheb_coef(g["nr_Fermi_fft_steps"], beta_re, mu_re, True,
+                                   g["Fermi_cheb_precision"])
+        order, tiles = self._row_order(gpu, "cond_tile_nm")
+        H = Hamiltonian(self._sample.num_orb, indptr, indices, data,
+                        precision=gpu["precision"], handle=h, order=order, tiles=tiles)
+        q_points = np.asarray(self._config.dyn_pol["q_points"], dtype=np.float64)
+        pos = np.asarray(self._sample.orb_pos, dtype=np.float64)
+        psi0 = self._states(num_sample)
+        chk, thr = self._chk()
+        T = g["nr_time_steps"]
+        per = max(1, min(32, int(gpu["max_vectors"])))
+        tune.apply(H, 2 * min(per, num_sample), bes, gpu["autotune"])
+        corr = np.zeros((len(q_points), T), dtype=np.float64)
+        for a, b in self._batches(num_sample, per):
+            c = corr_dyn_pol(H, bes, coef_F, coef_omF, psi0[a:b], T, q_points, pos, chk, thr)
+            for s in range(b - a):
+                corr = corr + c[s] / num_sample
+        corr = self.all_average(corr)
+        self._save_data(corr, self._output["corr_dyn_pol"])
         return corr
 
     def calc_corr_dc_cond(self):

@@ -13,7 +13,7 @@ from . import _lib
 
 __all__ = ["Hamiltonian", "DosRun", "bessel_series", "fermi_cheb_coef", "corr_dos",
            "corr_dos_batch", "corr_ac", "corr_dc", "corr_ldos", "ldos_states", "psi_t",
-           "quasi_eigenstates", "cheb_step", "cheb_poly",
+           "quasi_eigenstates", "corr_dyn_pol", "cheb_step", "cheb_poly",
            "precision_code"]
 
 
@@ -241,6 +241,34 @@ def psi_t(H, bes, psi0, time_log, n_timestep, chk_step=128, chk_thr=1e-9):
     rc = _lib.lib().tbg_psi_t(H.handle.ptr, H.ptr, bes.ctypes.data, len(bes), _lib.ptr(p),
                               tl.ctypes.data, len(tl), int(n_timestep), int(chk_step),
                               float(chk_thr), _lib.ptr(out), ctypes.byref(fs))
+    _lib.check(rc, fs)
+    return out.cpu().numpy()
+
+
+def corr_dyn_pol(H, bes, coef_F, coef_omF, psi0, n_timestep, q_points, orb_pos,
+                 chk_step=128, chk_thr=1e-9):
+    """Per-state dynamic-polarisation correlation (K, n_q, T) float64
+    (tbpm_dyn_pol, tbpm.f90:355-493); orb_pos (n, 3) in the caller's order,
+    q_points (n_q, 3) in the inverse units of orb_pos."""
+    torch = _lib.torch_cuda()
+    dv = H.handle.device
+    p, (K, n) = _states_dev(psi0, dv)
+    if n != H.n:
+        raise ValueError("state length does not match the Hamiltonian")
+    bes = np.ascontiguousarray(bes, dtype=np.float64)
+    cF = np.ascontiguousarray(coef_F, dtype=np.float64)
+    cO = np.ascontiguousarray(coef_omF, dtype=np.float64)
+    q = np.ascontiguousarray(np.atleast_2d(q_points), dtype=np.float64)
+    pos = np.ascontiguousarray(orb_pos, dtype=np.float64)
+    if q.shape[1] != 3 or pos.shape != (n, 3):
+        raise ValueError("q_points must be (n_q, 3) and orb_pos (n, 3)")
+    out = torch.empty((K, len(q), n_timestep), dtype=torch.float64, device=f"cuda:{dv}")
+    fs = ctypes.c_int(0)
+    rc = _lib.lib().tbg_corr_dyn_pol(H.handle.ptr, H.ptr, bes.ctypes.data, len(bes),
+                                     cF.ctypes.data, len(cF), cO.ctypes.data, len(cO),
+                                     _lib.ptr(p), K, int(n_timestep), q.ctypes.data, len(q),
+                                     pos.ctypes.data, int(chk_step), float(chk_thr),
+                                     _lib.ptr(out), ctypes.byref(fs))
     _lib.check(rc, fs)
     return out.cpu().numpy()
 

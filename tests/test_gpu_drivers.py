@@ -169,3 +169,70 @@ def test_solver_ldos_qe_psi_t(gpu, tmp_path):
     assert np.max(np.abs(frames - pr)) < 1e-12
     with pytest.raises(ValueError):
         solver.calc_psi_t(p0, [T + 1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("layout", ["caller", "tiles"])
+def test_dyn_pol_vs_oracle(gpu, prec, layout):
+    s = _sample(L=10, field=15.0 if layout == "caller" else 0.0)
+    H, (ip, ix, dt) = _ham(s, prec, layout)
+    tau = 0.5 * 2 * math.pi / s.energy_range * s.rescale
+    bes = tbpm.bessel_series(tau, 1e-14)
+    beta_re = 1.0 / (8.617333262e-5 * 300.0) * s.rescale
+    cF = tbpm.fermi_cheb_coef(2 ** 15, beta_re, 0.0, False, 1e-12)
+    cO = tbpm.fermi_cheb_coef(2 ** 15, beta_re, 0.0, True, 1e-12)
+    psi = states.random_states(s.num_orb, states.sample_seeds(13, 3))
+    qs = [[1.0, 0.5, 0.0], [0.0, 2.0, 0.0], [0.3, 0.0, 0.0]]
+    T = 20
+    g = tbpm.corr_dyn_pol(H, bes, cF, cO, psi, T, qs, s.orb_pos, chk_thr=1e-4)
+    r = oracle.orc_tbpm_dyn_pol(ip, ix, dt, bes, cF, cO, psi, T, qs, s.orb_pos)
+    assert g.shape == (3, 3, T)
+    scale = max(1e-3, np.max(np.abs(r)))
+    assert np.max(np.abs(g - r)) < (1e-12 if prec == "fp64" else 2e-5) * max(1.0, scale)
+    H.close()
+
+
+@needs_ref
+def test_dyn_pol_vs_reference_live(gpu):
+    s = _sample(L=6, W=0.0)
+    ip, ix, dt, dx, dy = oracle.build_ham_dxy_loop(s.orb_eng, s.hop_i, s.hop_j, s.hop_v, s.dr)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+    tau = 0.5 * 2 * math.pi / s.energy_range * s.rescale
+    bes = tbpm.bessel_series(tau, 1e-14)
+    beta_re, mu_re = 1.0 / (8.617333262e-5 * 300.0) * s.rescale, 0.0
+    S, T, seed, qs = 2, 12, 5, [[0.9, 0.1, 0.0]]
+    ref = oracle.ref_tbpm_dyn_pol(bes, beta_re, mu_re, ip, ix, dt, s.rescale, dx, dy, s.orb_pos,
+                                  seed, T, S, qs)
+    cF = tbpm.fermi_cheb_coef(2 ** 15, beta_re, mu_re, False, 1e-12)
+    cO = tbpm.fermi_cheb_coef(2 ** 15, beta_re, mu_re, True, 1e-12)
+    psi = np.array([oracle.ref_random_state(s.num_orb, seed * (i + 1)) for i in range(S)])
+    g = tbpm.corr_dyn_pol(H, bes, cF, cO, psi, T, qs, s.orb_pos)
+    avg = np.zeros_like(g[0])
+    for i in range(S):
+        avg = avg + g[i] / S
+    assert np.max(np.abs(avg - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+    H.close()
+
+
+def test_solver_dyn_pol(gpu, tmp_path):
+    s = _sample(L=8, W=0.0)
+    T, S = 12, 2
+    cfg = _config(T, S)
+    cfg.dyn_pol["q_points"] = [[1.0, 0.0, 0.0], [0.5, 0.5, 0.0]]
+    solver = tb.Solver(s, cfg, directory=str(tmp_path / "sim"), prefix="p", echo_details=False)
+    c = solver.calc_corr_dyn_pol()
+    assert c.shape == (2, T) and c.dtype == np.float64
+    assert (tmp_path / "sim" / "p.corr_dyn_pol.npy").exists()
+    ip, ix, dt = oracle.build_ham_csr(s.orb_eng, s.hop_i, s.hop_j, s.hop_v)
+    tau = 0.5 * 2 * math.pi / s.energy_range * s.rescale
+    bes = oracle.bessel_series(tau, cfg.generic["Bessel_precision"])
+    beta_re = cfg.generic["beta"] * s.rescale
+    cF = oracle.orc_fermi_coef(2 ** 15, beta_re, 0.0, False, 1e-12)
+    cO = oracle.orc_fermi_coef(2 ** 15, beta_re, 0.0, True, 1e-12)
+    psi = states.random_states(s.num_orb, states.sample_seeds(1337, S))
+    r = oracle.orc_tbpm_dyn_pol(ip, ix, dt, bes, cF, cO, psi, T, cfg.dyn_pol["q_points"],
+                                s.orb_pos)
+    ref = np.zeros_like(r[0])
+    for i in range(S):
+        ref = ref + r[i] / S
+    assert np.max(np.abs(c - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))

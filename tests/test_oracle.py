@@ -367,3 +367,61 @@ def test_orc_ldos_psi_t_qe_match_reference_live():
         avg = avg + q[s] / S
     assert r.shape == (len(en), n)
     assert np.max(np.abs(avg - r)) < 1e-13 * max(1.0, np.max(r))
+
+
+def test_orc_dyn_pol_vs_dense_exact():
+    """psi1 = V g(E) V^H (rho* psi0), psi2 = V f(E) V^H psi0 with the series
+    f(E) = sum_k c_k cos(k arccos E) evaluated on the eigenvalues, both
+    propagated by e^{-i E tau}: Im <psi2 | rho psi1> (closed form)."""
+    ip, ix, dt = _small_disordered(seed=19)
+    n = len(ip) - 1
+    orb, hi, hj, hv, dr, pos = _graphene(5)
+    tau = math.pi / 2
+    bes = oracle.bessel_series(tau, 1e-15)
+    beta_re = 1.0 / (KB * 600.0) * 9.0
+    cF = oracle.orc_fermi_coef(2 ** 12, beta_re, 0.02, False, 1e-12)
+    cO = oracle.orc_fermi_coef(2 ** 12, beta_re, 0.02, True, 1e-12)
+    rng = np.random.default_rng(4)
+    psi = np.exp(2j * np.pi * rng.random((2, n)))
+    psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+    qs = [[1.3, 0.4, 0.0], [0.0, 2.1, 0.0]]
+    T = 12
+    c = oracle.orc_tbpm_dyn_pol(ip, ix, dt, bes, cF, cO, psi, T, qs, pos)
+    E, V = _dense_eig(ip, ix, dt)
+    th = np.arccos(np.clip(E, -1, 1))
+    f = np.array([np.sum(cF * np.cos(np.arange(len(cF)) * x)) for x in th])
+    g = np.array([np.sum(cO * np.cos(np.arange(len(cO)) * x)) for x in th])
+    for iq, q in enumerate(qs):
+        rho = np.exp(1j * (pos @ np.asarray(q)))
+        for s in range(2):
+            a1 = g * (V.conj().T @ (np.conj(rho) * psi[s]))
+            a2 = f * (V.conj().T @ psi[s])
+            for t in range(T):
+                ph = np.exp(-1j * E * t * tau)
+                p1, p2 = V @ (ph * a1), V @ (ph * a2)
+                assert abs(c[s, iq, t] - np.vdot(p2, rho * p1).imag) < 1e-12
+
+
+@needs_ref
+def test_orc_dyn_pol_matches_reference_live():
+    orb, hi, hj, hv, dr, pos = _graphene(5)
+    rng = np.random.default_rng(23)
+    orb = rng.uniform(-0.3, 0.3, len(orb)) / 9.0
+    ip, ix, dt, dx, dy = oracle.build_ham_dxy_loop(orb, hi, hj, hv, dr)
+    n = len(orb)
+    tau = math.pi / 2
+    bes = oracle.bessel_series(tau, 1e-14)
+    beta_re, mu_re = 1.0 / (KB * 300.0) * 9.0, 0.01
+    S, T, seed = 2, 10, 3
+    qs = [[0.7, 0.2, 0.0], [1.5, -0.4, 0.0]]
+    ref = oracle.ref_tbpm_dyn_pol(bes, beta_re, mu_re, ip, ix, dt, 9.0, dx, dy, pos, seed, T, S,
+                                  qs)
+    cF = oracle.orc_fermi_coef(2 ** 15, beta_re, mu_re, False, 1e-12)
+    cO = oracle.orc_fermi_coef(2 ** 15, beta_re, mu_re, True, 1e-12)
+    psi = np.array([oracle.ref_random_state(n, seed * (s + 1)) for s in range(S)])
+    c = oracle.orc_tbpm_dyn_pol(ip, ix, dt, bes, cF, cO, psi, T, qs, pos)
+    avg = np.zeros_like(c[0])
+    for s in range(S):
+        avg = avg + c[s] / S
+    assert ref.shape == (2, T)
+    assert np.max(np.abs(avg - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
