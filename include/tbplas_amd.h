@@ -248,6 +248,17 @@ int tbg_corr_dyn_pol(tbg_handle *h, const tbg_ham *H, const double *bes,
                      const double *orb_pos, int chk_step, double chk_thr,
                      double *corr, int *fail_step);
 
+/* Replaces f2py tbpm_kbdc (tbplas/fortran/tbpm.f90:836-960; called from
+ * tbplas/tbpm/solver.py:459-468).  H: complex128 with current operators
+ * (cur_scale = rescale).  psi0: device complex128 (K, n).  mu: device
+ * complex128 (K, n_kernel, n_kernel), state s in column-major order like
+ * the reference's corr_mu (element (i, j) at s*N*N + i + j*N), already
+ * multiplied by the Jackson weights g_i g_j (kpm.f90:8-29); the
+ * reference's corr_mu_avg is the mean over states.  direction 1: xx, 2:
+ * xy. */
+int tbg_hall_mu(tbg_handle *h, const tbg_ham *H, const double *psi0, int K,
+                int n_kernel, int direction, double *mu);
+
 /* Replaces f2py tbpm_eigenstates (tbplas/fortran/tbpm.f90:721-830; called
  * from tbplas/tbpm/solver.py:472-505).  energies: host, n_en energies (eV,
  * with t_step in hbar/eV as for tbg_corr_dc).  qe: device float64

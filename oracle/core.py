@@ -11,6 +11,8 @@ __all__ = [
     "orc_tbpm_dos", "orc_tbpm_accond", "orc_tbpm_dccond", "orc_step",
     "orc_fermi_coef", "orc_csr_mv", "orc_tbpm_ldos", "orc_psi_t", "orc_quasi_eigenstates",
     "orc_cheb_apply", "orc_tbpm_dyn_pol", "ref_tbpm_dyn_pol",
+    "jackson_kernel", "orc_tbpm_kbdc", "ref_tbpm_kbdc", "orc_cond_from_trace",
+    "ref_cond_from_trace",
     "ref_random_state", "ref_tbpm_dos", "ref_tbpm_accond", "ref_tbpm_dccond",
     "ref_fermi_coef", "ref_tbpm_ldos", "ref_tbpm_psi_t", "ref_tbpm_eigenstates",
     "bessel_series", "build_ham_csr", "build_ham_dxy_loop",
@@ -276,6 +278,80 @@ def orc_tbpm_dyn_pol(indptr, indices, hop, bes, coef_F, coef_omF, psi0, n_timest
     return out
 
 
+def jackson_kernel(n_kernel):
+    """jackson_kernel (kpm.f90:8-29)."""
+    g = np.zeros(n_kernel)
+    g[0] = 0.5
+    q = math.pi / n_kernel
+    cotq = math.cos(q) / math.sin(q)
+    for k in range(1, n_kernel):
+        g[k] = ((n_kernel - k) * math.cos(q * k) + cotq * math.sin(q * k)) / n_kernel
+    return g
+
+
+def orc_tbpm_kbdc(indptr, indices, hop, dx, dy, rescale, psi0, n_kernel, direction=1):
+    """tbpm_kbdc (tbpm.f90:836-960) on caller states psi0 (S, n): per state
+    g_i g_j <T_i(H) J_x psi | J_b T_j(H) psi> (S, N, N), [s, i, j]; the
+    currents J_a = i rescale hop d_a on H's structure (funcs.f90:12-33),
+    T_j by the recurrence of tbpm.f90:892-900 with orc_csr_mv."""
+    psi0 = np.atleast_2d(np.asarray(psi0, dtype=np.complex128))
+    hop = np.asarray(hop, dtype=np.complex128)
+    jx = 1j * rescale * hop * np.asarray(dx)
+    jb = jx if direction == 1 else 1j * rescale * hop * np.asarray(dy)
+    g = jackson_kernel(n_kernel)
+    out = np.zeros((psi0.shape[0], n_kernel, n_kernel), dtype=np.complex128)
+
+    def cheb_vectors(v):
+        t = [v]
+        if n_kernel > 1:
+            t.append(orc_csr_mv(indptr, indices, hop, v))
+        for j in range(2, n_kernel):
+            t.append(2.0 * orc_csr_mv(indptr, indices, hop, t[j - 1]) - t[j - 2])
+        return t
+
+    for s, w in enumerate(psi0):
+        W = np.array([orc_csr_mv(indptr, indices, jb, t) for t in cheb_vectors(w)])
+        P = np.array(cheb_vectors(orc_csr_mv(indptr, indices, jx, w)))
+        out[s] = (g[:, None] * g[None, :]) * (P.conj() @ W.T)
+    return out
+
+
+def orc_cond_from_trace(mu_mn, mu, h_rescale, beta, ne_integral):
+    """cond_from_trace (analysis.f90:6-69), loops as written (small sizes):
+    Gamma_mn of get_gamma_mn (kpm.f90:31-56), Fermi_dist (funcs.f90:36-57)."""
+    N = mu_mn.shape[0]
+    dE = math.pi / ne_integral
+    ne = ne_integral - 1
+    sums = np.zeros(ne)
+    for k in range(1, ne + 1):
+        x = k * dE
+        sx, cx = math.sin(x), math.cos(x)
+        acc = 0.0
+        for j in range(N):
+            for i in range(N):
+                gam = (complex(cx, -j * sx) * math.cos(i * x) * complex(math.cos(j * x),
+                                                                         math.sin(j * x))
+                       + complex(cx, i * sx) * math.cos(j * x) * complex(math.cos(i * x),
+                                                                         -math.sin(i * x)))
+                acc += (gam * mu_mn[i, j]).real
+        sums[k - 1] = acc
+    out = np.zeros(len(mu))
+    for i, m in enumerate(mu):
+        dcx = 0.0
+        for k in range(1, ne + 1):
+            x = k * dE
+            en = math.cos(x) * h_rescale
+            if en >= m:
+                e = math.exp(beta * (m - en))
+                fd = e / (1 + e)
+            else:
+                e = math.exp(beta * (en - m))
+                fd = 1 / (1 + e)
+            dcx += sums[k - 1] * fd * dE / math.sin(x) ** 3
+        out[i] = dcx
+    return out
+
+
 def orc_csr_mv(indptr, indices, val, x):
     """csr_mv (csr.F90:71-101)."""
     lib = oracle_lib()
@@ -484,6 +560,38 @@ def ref_tbpm_dyn_pol(bes, beta_re, mu_re, indptr, indices, hop, rescale, dx, dy,
                       r(D(fermi_prec)), _ptr(qf), r(I(len(q))), fn, _ptr(corr), r(I(0)),
                       r(I(chk_step)), r(D(chk_thr)), fl)
     return np.ascontiguousarray(corr.T)
+
+
+def ref_tbpm_kbdc(seed, indptr, indices, hop, rescale, dx, dy, n_samples, n_kernel,
+                  direction=1):
+    """The reference tbpm_kbdc (tbpm.f90:836-960): (N, N) [i, j]."""
+    lib = ref_lib()
+    ip, ix = _i32(indptr), _i32(indices)
+    hop = np.ascontiguousarray(hop, dtype=np.complex128)
+    dx = np.ascontiguousarray(dx, dtype=np.float64)
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    out = np.zeros((n_kernel, n_kernel), dtype=np.complex128)   # Fortran (i, j) -> [j, i]
+    I, D = ctypes.c_int, ctypes.c_double
+    r = ctypes.byref
+    lib.tbpm_kbdc_(r(I(seed)), _ptr(ip, I), r(I(len(ip))), _ptr(ix, I), r(I(len(ix))),
+                   _ptr(hop), r(I(len(hop))), r(D(rescale)), _ptr(dx), r(I(len(dx))),
+                   _ptr(dy), r(I(len(dy))), r(I(n_samples)), r(I(n_kernel)),
+                   r(I(direction)), _ptr(out), r(I(0)))
+    return np.ascontiguousarray(out.T)
+
+
+def ref_cond_from_trace(mu_mn, mu, h_rescale, beta, ne_integral, fermi_prec=1e-12):
+    """The reference cond_from_trace (analysis.f90:6-69)."""
+    lib = ref_lib()
+    m = np.asfortranarray(mu_mn, dtype=np.complex128)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    cond = np.zeros(len(mu))
+    I, D = ctypes.c_int, ctypes.c_double
+    r = ctypes.byref
+    lib.cond_from_trace_(m.ctypes.data_as(ctypes.c_void_p), r(I(m.shape[0])), _ptr(mu),
+                         r(I(len(mu))), r(D(h_rescale)), r(D(beta)), r(I(ne_integral)),
+                         r(D(fermi_prec)), r(I(0)), _ptr(cond))
+    return cond
 
 
 def ref_fermi_coef(nr_fermi, beta_re, mu_re, one_minus, eps):

@@ -26,6 +26,40 @@ def window_exp_ten(i, tnr):
     return 10. ** power
 
 
+def fermi_dist(beta, ef, energy):
+    """Fermi_dist (funcs.f90:36-57), elementwise over energy."""
+    energy = np.asarray(energy, dtype=np.float64)
+    up = energy >= ef
+    x = np.exp(np.where(up, beta * (ef - energy), beta * (energy - ef)))
+    return np.where(up, x / (1 + x), 1 / (1 + x))
+
+
+def cond_from_trace(mu_mn, mu, h_rescale, beta, ne_integral):
+    """cond_from_trace (analysis.f90:6-69) for the (N, N) moments mu_mn and
+    chemical potentials mu: on the grid x_k = k pi / NE (k = 1..NE-1),
+    s_k = sum_ij Re(Gamma_ij(x_k) mu_ij) with Gamma of get_gamma_mn
+    (kpm.f90:31-56), then cond = sum_k s_k f(cos x_k rescale) dx / sin^3 x_k.
+    Gamma_ij = c_i A_j + B_i c_j with c_i = cos(i x), A_j = (cos x - i j
+    sin x) e^{i j x}, B_i = (cos x + i i sin x) e^{-i i x}, so s_k comes
+    from two matrix products with mu for all k at once."""
+    mu_mn = np.asarray(mu_mn, dtype=np.complex128)
+    N = mu_mn.shape[0]
+    dE = math.pi / ne_integral
+    x = np.arange(1, ne_integral) * dE                          # (NE,)
+    idx = np.arange(N)
+    cx, sx = np.cos(x)[:, None], np.sin(x)[:, None]
+    ph = np.outer(x, idx)                                       # (NE, N)
+    c = np.cos(ph)
+    A = (cx - 1j * idx[None, :] * sx) * np.exp(1j * ph)
+    B = (cx + 1j * idx[None, :] * sx) * np.exp(-1j * ph)
+    MA = mu_mn @ A.T                                            # (N, NE)
+    Mc = mu_mn @ c.T
+    s = np.real(np.sum(c.T * MA, axis=0) + np.sum(B.T * Mc, axis=0))
+    en = np.cos(x) * h_rescale
+    w = dE / np.sin(x) ** 3
+    return np.array([np.sum(s * fermi_dist(beta, m, en) * w) for m in np.atleast_1d(mu)])
+
+
 class Analyzer:
     """Post-processing on the host (analysis.py:54-87)."""
 
@@ -103,6 +137,24 @@ class Analyzer:
         if self._config.generic['correct_spin']:
             ac_cond = 2. * ac_cond
         return omegas, ac_cond
+
+    def calc_hall_cond(self, mu_mn, unit="h_bar"):
+        """analysis.py:513-573: (energies, Hall conductivity) at
+        config.dckb['energies'] from the moments of Solver.calc_hall_mu,
+        in e^2/h_bar (unit 'h_bar') or e^2/h (unit 'h')."""
+        if unit not in ("h_bar", "h"):
+            raise ValueError(f"Illegal unit {unit}")
+        s = self._sample
+        energies = np.array(self._config.dckb['energies'])
+        pref = 16 * s.nr_orbitals / (math.pi * s.energy_range ** 2)
+        pref /= s.area_unit_cell if self._dimension == 2 else s.volume_unit_cell
+        if unit == "h":
+            pref *= 2 * math.pi
+        cond = cond_from_trace(mu_mn, energies, s.rescale, self._config.generic['beta'],
+                               self._config.dckb['ne_integral']) * pref
+        if self._config.generic['correct_spin']:
+            cond = cond * 2
+        return energies, cond
 
     def calc_dc_cond(self, corr_dos, corr_dc, window_dos=window_hanning,
                      window_dc=window_exp):

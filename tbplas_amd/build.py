@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libtbplas_amd.so")
 SOURCES = ["tbg_prop.hip", "tbg_block.hip", "tbg_csr.hip", "tbg_cond.hip", "tbg_api.hip",
-           "tbg_mlp.hip"]
+           "tbg_mlp.hip", "tbg_hall.hip"]
 ARCH = os.environ.get("TBG_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -42,7 +42,8 @@ def build(verbose=False):
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs,
+    # rocBLAS: the complex128 Gram matrix of the Hall moments (one GEMM)
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-lrocblas",
            "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -808,6 +808,19 @@ int tbg_corr_dyn_pol(tbg_handle* h, const tbg_ham* H, const double* bes, int n_b
     });
 }
 
+int tbg_hall_mu(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, int n_kernel,
+                int direction, double* mu) {
+    return guard([&] {
+        check_handle(h);
+        check_psi_args(H, psi0, K, 1);
+        TBG_REQUIRE(H->has_current, "Hamiltonian was prepared without current operators");
+        TBG_REQUIRE(H->precision == TBG_FP64, "the Hall moments need a complex128 Hamiltonian");
+        TBG_REQUIRE(n_kernel >= 1 && mu != nullptr, "need n_kernel >= 1 and an output");
+        TBG_REQUIRE(direction == 1 || direction == 2, "direction must be 1 (xx) or 2 (xy)");
+        hall_mu_impl(h, H, psi0, K, n_kernel, direction, mu);
+    });
+}
+
 int tbg_quasi_eigenstates(tbg_handle* h, const tbg_ham* H, const double* bes, int n_bes,
                           const double* psi0, int K, int T, double t_step,
                           const double* energies, int n_en, int chk_step, double chk_thr,

@@ -28,6 +28,10 @@ void corr_ac_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
                   int nF, const double* cO, int nO, const double* psi0, int K, int T,
                   int chk_step, double chk_thr, double* corr, int* fail_step);
 
+// tbg_hall.hip
+void hall_mu_impl(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, int N, int dir,
+                  double* mu);
+
 template <typename R>
 void dyn_pol_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, const double* cF,
                   int nF, const double* cO, int nO, const double* psi0, int K, int T,

@@ -3,9 +3,9 @@
 Mirrors tbplas/tbpm/solver.py: the constructor's output/config side effects
 (solver.py:84-153), ``calc_corr_dos`` (256-284), ``calc_corr_ldos``
 (286-315), ``calc_corr_ac_cond`` (317-352), ``calc_corr_dyn_pol``
-(354-391), ``calc_corr_dc_cond`` (393-441), ``calc_quasi_eigenstates``
-(472-505) and ``calc_psi_t`` (543-600) keep their signatures and return
-shapes.  The f2py Fortran calls are replaced by the HIP library;
+(354-391), ``calc_corr_dc_cond`` (393-441), ``calc_hall_mu`` (443-470),
+``calc_quasi_eigenstates`` (472-505) and ``calc_psi_t`` (543-600) keep
+their signatures and return shapes.  The f2py Fortran calls are replaced by the HIP library;
 initial states come from the seeded host generator (tbplas_amd.states) and
 are averaged over samples exactly as the reference accumulates
 ``corr += corrval / n_ran_samples``.  A failed norm check raises
@@ -22,7 +22,7 @@ from . import _lib, states, tune
 from .config import Config
 from .sample import Sample
 from .tbpm import (Hamiltonian, bessel_series, corr_ac, corr_dc, corr_dos, corr_dyn_pol,
-                   corr_ldos, fermi_cheb_coef, ldos_states, psi_t, quasi_eigenstates)
+                   corr_ldos, fermi_cheb_coef, hall_mu, ldos_states, psi_t, quasi_eigenstates)
 
 __all__ = ["Solver", "H_BAR_EV"]
 
@@ -199,6 +199,29 @@ class Solver:
         corr = self.all_average(corr)
         self._save_data(corr, self._output["corr_LDOS"])
         return corr
+
+    def calc_hall_mu(self):
+        """(n_kernel, n_kernel) complex128 Kubo-Bastin moments mu_mn with
+        config.dckb['n_kernel'] and ['direction'] (solver.py:443-470),
+        averaged over the random states."""
+        gpu = self._gpu()
+        h = self._handle()
+        self._sample.device = gpu["device"]
+        indptr, indices, hop, dx, dy = self._sample.build_ham_dxy_device()
+        num_sample = self._dist_sample()
+        H = Hamiltonian(self._sample.num_orb, indptr, indices, hop, dx, dy,
+                        cur_scale=self._sample.rescale, precision="fp64", handle=h)
+        nk = int(self._config.dckb["n_kernel"])
+        direction = int(self._config.dckb["direction"])
+        psi0 = self._states(num_sample)
+        mu = np.zeros((nk, nk), dtype=np.complex128)
+        for a, b in self._batches(num_sample, 4):
+            m = hall_mu(H, psi0[a:b], nk, direction)
+            for s in range(b - a):
+                mu = mu + m[s] / num_sample
+        mu = self.all_average(mu)
+        self._save_data(mu, self._output["hall_mu"])
+        return mu
 
     def calc_quasi_eigenstates(self, save_data=False):
         """(n_energies, num_orb) float64 quasi-eigenstates at

@@ -13,7 +13,7 @@ from . import _lib
 
 __all__ = ["Hamiltonian", "DosRun", "bessel_series", "fermi_cheb_coef", "corr_dos",
            "corr_dos_batch", "corr_ac", "corr_dc", "corr_ldos", "ldos_states", "psi_t",
-           "quasi_eigenstates", "corr_dyn_pol", "cheb_step", "cheb_poly",
+           "quasi_eigenstates", "corr_dyn_pol", "hall_mu", "cheb_step", "cheb_poly",
            "precision_code"]
 
 
@@ -271,6 +271,22 @@ def corr_dyn_pol(H, bes, coef_F, coef_omF, psi0, n_timestep, q_points, orb_pos,
                                      _lib.ptr(out), ctypes.byref(fs))
     _lib.check(rc, fs)
     return out.cpu().numpy()
+
+
+def hall_mu(H, psi0, n_kernel, direction=1):
+    """Per-state Kubo-Bastin moments g_i g_j <T_i(H) J_x psi | J_b T_j(H) psi>
+    (tbpm_kbdc, tbpm.f90:836-960): (K, n_kernel, n_kernel) complex128,
+    [s, i, j]; direction 1: b = x, 2: b = y.  H: complex128 with currents."""
+    torch = _lib.torch_cuda()
+    dv = H.handle.device
+    p, (K, n) = _states_dev(psi0, dv)
+    if n != H.n:
+        raise ValueError("state length does not match the Hamiltonian")
+    out = torch.empty((K, n_kernel, n_kernel), dtype=torch.complex128, device=f"cuda:{dv}")
+    _lib.check(_lib.lib().tbg_hall_mu(H.handle.ptr, H.ptr, _lib.ptr(p), K, int(n_kernel),
+                                      int(direction), _lib.ptr(out)))
+    # each state is column-major (i + j N): the array read row-major is [s, j, i]
+    return out.cpu().numpy().transpose(0, 2, 1).copy()
 
 
 def quasi_eigenstates(H, bes, psi0, n_timestep, t_step, energies, chk_step=128, chk_thr=1e-9):

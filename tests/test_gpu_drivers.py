@@ -236,3 +236,75 @@ def test_solver_dyn_pol(gpu, tmp_path):
     for i in range(S):
         ref = ref + r[i] / S
     assert np.max(np.abs(c - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+def _dxy_ham(s, prec="fp64"):
+    ip, ix, dt, dx, dy = oracle.build_ham_dxy_loop(s.orb_eng, s.hop_i, s.hop_j, s.hop_v, s.dr)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, dx, dy, cur_scale=s.rescale, precision=prec)
+    return H, (ip, ix, dt, dx, dy)
+
+
+@pytest.mark.parametrize("N,field,direction", [(16, 0.0, 1), (64, 25.0, 2), (300, 10.0, 1)])
+def test_hall_mu_vs_oracle(gpu, N, field, direction):
+    """Moments through one or two GEMM blocks of 256 rows."""
+    s = _sample(L=6, field=field)
+    H, (ip, ix, dt, dx, dy) = _dxy_ham(s)
+    psi = states.random_states(s.num_orb, states.sample_seeds(17, 2))
+    g = tbpm.hall_mu(H, psi, N, direction)
+    r = oracle.orc_tbpm_kbdc(ip, ix, dt, dx, dy, s.rescale, psi, N, direction)
+    assert g.shape == (2, N, N)
+    assert np.max(np.abs(g - r)) < 1e-12 * max(1.0, np.max(np.abs(r)))
+    H.close()
+
+
+@needs_ref
+def test_hall_mu_vs_reference_live(gpu):
+    s = _sample(L=5, field=15.0)
+    H, (ip, ix, dt, dx, dy) = _dxy_ham(s)
+    S, seed, N = 2, 9, 32
+    ref = oracle.ref_tbpm_kbdc(seed, ip, ix, dt, s.rescale, dx, dy, S, N, 1)
+    psi = np.array([oracle.ref_random_state(s.num_orb, seed * (i + 1)) for i in range(S)])
+    g = tbpm.hall_mu(H, psi, N, 1)
+    avg = np.zeros_like(g[0])
+    for i in range(S):
+        avg = avg + g[i] / S
+    assert np.max(np.abs(avg - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+    H.close()
+
+
+def test_hall_mu_needs_fp64_and_currents(gpu):
+    s = _sample(L=4)
+    H32, _ = _dxy_ham(s, "fp32")
+    psi = states.random_states(s.num_orb, states.sample_seeds(1, 1))
+    with pytest.raises(tb._lib.TBGError):
+        tbpm.hall_mu(H32, psi, 8)
+    H32.close()
+    ip, ix, dt = oracle.build_ham_csr(s.orb_eng, s.hop_i, s.hop_j, s.hop_v)
+    Hn = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+    with pytest.raises(tb._lib.TBGError):
+        tbpm.hall_mu(Hn, psi, 8)
+    Hn.close()
+
+
+def test_solver_hall_mu_and_analyzer(gpu, tmp_path):
+    s = _sample(L=6, field=20.0)
+    S = 3
+    cfg = _config(16, S)
+    cfg.dckb["n_kernel"] = 48
+    cfg.dckb["direction"] = 2
+    cfg.dckb["energies"] = [-0.3, 0.0, 0.2]
+    cfg.dckb["ne_integral"] = 256
+    solver = tb.Solver(s, cfg, directory=str(tmp_path / "sim"), prefix="h", echo_details=False)
+    mu = solver.calc_hall_mu()
+    assert mu.shape == (48, 48) and (tmp_path / "sim" / "h.hall_mu.npy").exists()
+    ip, ix, dt, dx, dy = oracle.build_ham_dxy_loop(s.orb_eng, s.hop_i, s.hop_j, s.hop_v, s.dr)
+    psi = states.random_states(s.num_orb, states.sample_seeds(1337, S))
+    r = oracle.orc_tbpm_kbdc(ip, ix, dt, dx, dy, s.rescale, psi, 48, 2)
+    ref = np.zeros_like(r[0])
+    for i in range(S):
+        ref = ref + r[i] / S
+    assert np.max(np.abs(mu - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+    en, cond = tb.Analyzer(s, cfg, dimension=2).calc_hall_cond(mu)
+    pref = 16 * s.nr_orbitals / (math.pi * s.energy_range ** 2) / s.area_unit_cell
+    oc = oracle.orc_cond_from_trace(mu, [-0.3, 0.0, 0.2], s.rescale, cfg.generic["beta"], 256)
+    assert np.max(np.abs(cond - pref * oc)) < 1e-10 * max(1.0, np.max(np.abs(pref * oc)))

@@ -425,3 +425,78 @@ def test_orc_dyn_pol_matches_reference_live():
         avg = avg + c[s] / S
     assert ref.shape == (2, T)
     assert np.max(np.abs(avg - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+# -- Kubo-Bastin Hall moments and conductivity -------------------------------------
+def _dxy_sample(L=4, seed=29, field=0.0):
+    orb, hi, hj, hv, dr, pos = _graphene(L)
+    rng = np.random.default_rng(seed)
+    orb = rng.uniform(-0.3, 0.3, len(orb)) / 9.0
+    if field:
+        hv = hv * np.exp(1j * field * rng.uniform(-1, 1, len(hv)))
+    return oracle.build_ham_dxy_loop(orb, hi, hj, hv, dr)
+
+
+def test_orc_kbdc_vs_dense_exact():
+    """mu_ij = g_i g_j psi^H J_x^H T_i(H) J_b T_j(H) psi with T_k(H) = V
+    cos(k arccos E) V^H from the eigendecomposition (closed form)."""
+    ip, ix, dt, dx, dy = _dxy_sample(field=0.7)
+    n = len(ip) - 1
+    E, V = _dense_eig(ip, ix, dt)
+    dense = lambda vals: _dense_eig_matrix(ip, ix, vals)
+    Jx, Jy = dense(1j * 9.0 * dt * dx), dense(1j * 9.0 * dt * dy)
+    rng = np.random.default_rng(2)
+    psi = np.exp(2j * np.pi * rng.random((2, n)))
+    psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+    N = 12
+    th = np.arccos(np.clip(E, -1, 1))
+    Tk = [V @ np.diag(np.cos(k * th)) @ V.conj().T for k in range(N)]
+    g = oracle.jackson_kernel(N)
+    for direction, Jb in ((1, Jx), (2, Jy)):
+        mu = oracle.orc_tbpm_kbdc(ip, ix, dt, dx, dy, 9.0, psi, N, direction)
+        for s in range(2):
+            P = np.array([T @ (Jx @ psi[s]) for T in Tk])
+            W = np.array([Jb @ (T @ psi[s]) for T in Tk])
+            ex = np.outer(g, g) * (P.conj() @ W.T)
+            assert np.max(np.abs(mu[s] - ex)) < 1e-11 * max(1.0, np.max(np.abs(ex)))
+
+
+def _dense_eig_matrix(ip, ix, vals):
+    n = len(ip) - 1
+    M = np.zeros((n, n), dtype=complex)
+    for i in range(n):
+        for p in range(ip[i], ip[i + 1]):
+            M[i, ix[p]] += vals[p]
+    return M
+
+
+def test_cond_from_trace_vectorised_vs_loops():
+    """The factored Gamma_mn sums (tbplas_amd.analysis.cond_from_trace)
+    against the loop restatement on random moments."""
+    from tbplas_amd.analysis import cond_from_trace
+    rng = np.random.default_rng(8)
+    N = 9
+    m = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    mu = [-0.5, 0.0, 0.3]
+    a = cond_from_trace(m, mu, 9.0, 1.0 / (KB * 300), 40)
+    b = oracle.orc_cond_from_trace(m, mu, 9.0, 1.0 / (KB * 300), 40)
+    assert np.max(np.abs(a - b)) < 1e-10 * max(1.0, np.max(np.abs(b)))
+
+
+@needs_ref
+def test_orc_kbdc_and_cond_match_reference_live():
+    ip, ix, dt, dx, dy = _dxy_sample(L=5, seed=31, field=0.4)
+    n = len(ip) - 1
+    S, seed, N = 2, 9, 16
+    psi = np.array([oracle.ref_random_state(n, seed * (s + 1)) for s in range(S)])
+    for direction in (1, 2):
+        ref = oracle.ref_tbpm_kbdc(seed, ip, ix, dt, 9.0, dx, dy, S, N, direction)
+        mu = oracle.orc_tbpm_kbdc(ip, ix, dt, dx, dy, 9.0, psi, N, direction)
+        avg = np.zeros_like(mu[0])
+        for s in range(S):
+            avg = avg + mu[s] / S
+        assert np.max(np.abs(avg - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+    mus = [-0.4, 0.0, 0.25]
+    rc = oracle.ref_cond_from_trace(ref, mus, 9.0, 1.0 / (KB * 300), 64)
+    oc = oracle.orc_cond_from_trace(ref, mus, 9.0, 1.0 / (KB * 300), 64)
+    assert np.max(np.abs(rc - oc)) < 1e-10 * max(1.0, np.max(np.abs(rc)))
