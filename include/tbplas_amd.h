@@ -248,6 +248,17 @@ int tbg_corr_dyn_pol(tbg_handle *h, const tbg_ham *H, const double *bes,
                      const double *orb_pos, int chk_step, double chk_thr,
                      double *corr, int *fail_step);
 
+/* Haydock recursion coefficients of ldos_haydock (tbplas/fortran/
+ * tbpm.f90:964-1024; Haydock_coef, propagation.f90:85-155; called from
+ * tbplas/tbpm/solver.py:507-541) for K starting states psi0 (device
+ * complex128 (K, n), normalised, non-zero on the selected sites), H ->
+ * rescale * H~ (amv).  coefa: HOST complex128 (K, depth) = a_i, coefb: HOST
+ * float64 (K, depth) = b_i; the Green's function and the LDOS from their
+ * averages stay with the caller (green_function, funcs.f90:151-172). */
+int tbg_haydock_coef(tbg_handle *h, const tbg_ham *H, const double *psi0,
+                     int K, int depth, double rescale, double *coefa,
+                     double *coefb);
+
 /* Replaces f2py tbpm_kbdc (tbplas/fortran/tbpm.f90:836-960; called from
  * tbplas/tbpm/solver.py:459-468).  H: complex128 with current operators
  * (cur_scale = rescale).  psi0: device complex128 (K, n).  mu: device

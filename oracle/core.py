@@ -12,7 +12,7 @@ __all__ = [
     "orc_fermi_coef", "orc_csr_mv", "orc_tbpm_ldos", "orc_psi_t", "orc_quasi_eigenstates",
     "orc_cheb_apply", "orc_tbpm_dyn_pol", "ref_tbpm_dyn_pol",
     "jackson_kernel", "orc_tbpm_kbdc", "ref_tbpm_kbdc", "orc_cond_from_trace",
-    "ref_cond_from_trace",
+    "ref_cond_from_trace", "orc_haydock_coef", "orc_green_function", "ref_ldos_haydock",
     "ref_random_state", "ref_tbpm_dos", "ref_tbpm_accond", "ref_tbpm_dccond",
     "ref_fermi_coef", "ref_tbpm_ldos", "ref_tbpm_psi_t", "ref_tbpm_eigenstates",
     "bessel_series", "build_ham_csr", "build_ham_dxy_loop",
@@ -352,6 +352,33 @@ def orc_cond_from_trace(mu_mn, mu, h_rescale, beta, ne_integral):
     return out
 
 
+def orc_haydock_coef(indptr, indices, hop, rescale, psi, depth):
+    """Haydock_coef (propagation.f90:85-155) with orc_csr_mv: (a, b)."""
+    hop = np.asarray(hop, dtype=np.complex128)
+    cur = np.asarray(psi, dtype=np.complex128).copy()
+    prev = np.zeros_like(cur)
+    a = np.zeros(depth, dtype=np.complex128)
+    b = np.zeros(depth)
+    bprev = 0.0
+    for i in range(depth):
+        buf = rescale * orc_csr_mv(indptr, indices, hop, cur)
+        a[i] = np.vdot(cur, buf)
+        buf = a[i] * -1 * cur + (-bprev) * prev + buf
+        b[i] = math.sqrt(np.vdot(buf, buf).real)
+        buf = buf / b[i]
+        prev, cur, bprev = cur, buf, b[i]
+    return a, b
+
+
+def orc_green_function(energy, delta, coefa, coefb):
+    """green_function (funcs.f90:151-172) at one energy."""
+    z = complex(energy, delta)
+    g = 0j
+    for i in range(len(coefa) - 1, -1, -1):
+        g = 1.0 / (z - coefa[i] - coefb[i] ** 2 * g)
+    return g
+
+
 def orc_csr_mv(indptr, indices, val, x):
     """csr_mv (csr.F90:71-101)."""
     lib = oracle_lib()
@@ -592,6 +619,26 @@ def ref_cond_from_trace(mu_mn, mu, h_rescale, beta, ne_integral, fermi_prec=1e-1
                          r(I(len(mu))), r(D(h_rescale)), r(D(beta)), r(I(ne_integral)),
                          r(D(fermi_prec)), r(I(0)), _ptr(cond))
     return cond
+
+
+def ref_ldos_haydock(site_indices, delta, e_range, indptr, indices, hop, rescale, seed, depth,
+                     n_timestep, n_samples):
+    """The reference ldos_haydock (tbpm.f90:964-1024): (energy, ldos), each
+    (2T+1,)."""
+    lib = ref_lib()
+    si = _i32(site_indices)
+    ip, ix = _i32(indptr), _i32(indices)
+    hop = np.ascontiguousarray(hop, dtype=np.complex128)
+    energy = np.zeros(2 * n_timestep + 1)
+    ldos = np.zeros(2 * n_timestep + 1)
+    fn, fl = _fstr("unused")
+    I, D = ctypes.c_int, ctypes.c_double
+    r = ctypes.byref
+    lib.ldos_haydock_(_ptr(si, I), r(I(len(si))), r(D(delta)), r(D(e_range)), _ptr(ip, I),
+                      r(I(len(ip))), _ptr(ix, I), r(I(len(ix))), _ptr(hop), r(I(len(hop))),
+                      r(D(rescale)), r(I(seed)), r(I(depth)), r(I(n_timestep)),
+                      r(I(n_samples)), fn, _ptr(energy), _ptr(ldos), fl)
+    return energy, ldos
 
 
 def ref_fermi_coef(nr_fermi, beta_re, mu_re, one_minus, eps):

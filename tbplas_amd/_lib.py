@@ -51,6 +51,7 @@ _SIGS = {
     "tbg_quasi_eigenstates": [_P, _P, _P, _I, _P, _I, _I, _D, _P, _I, _I, _D, _P,
                               ctypes.POINTER(_I)],
     "tbg_hall_mu": [_P, _P, _P, _I, _I, _I, _P],
+    "tbg_haydock_coef": [_P, _P, _P, _I, _I, _D, _P, _P],
     "tbg_corr_dyn_pol": [_P, _P, _P, _I, _P, _I, _P, _I, _P, _I, _I, _P, _I, _P, _I, _D,
                          _P, ctypes.POINTER(_I)],
     "tbg_corr_ac": [_P, _P, _P, _I, _P, _I, _P, _I, _P, _I, _I, _I, _D, _P,

@@ -808,6 +808,17 @@ int tbg_corr_dyn_pol(tbg_handle* h, const tbg_ham* H, const double* bes, int n_b
     });
 }
 
+int tbg_haydock_coef(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, int depth,
+                     double rescale, double* coefa, double* coefb) {
+    return guard([&] {
+        check_handle(h);
+        check_psi_args(H, psi0, K, 1);
+        TBG_REQUIRE(H->precision == TBG_FP64, "the Haydock recursion needs a complex128 Hamiltonian");
+        TBG_REQUIRE(depth >= 1 && coefa && coefb, "need depth >= 1 and outputs");
+        haydock_impl(h, H, psi0, K, depth, rescale, coefa, coefb);
+    });
+}
+
 int tbg_hall_mu(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, int n_kernel,
                 int direction, double* mu) {
     return guard([&] {

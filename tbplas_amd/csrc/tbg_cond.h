@@ -28,7 +28,9 @@ void corr_ac_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
                   int nF, const double* cO, int nO, const double* psi0, int K, int T,
                   int chk_step, double chk_thr, double* corr, int* fail_step);
 
-// tbg_hall.hip
+// tbg_hall.hip (coefa, coefb: host outputs)
+void haydock_impl(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, int depth,
+                  double rescale, double* coefa, double* coefb);
 void hall_mu_impl(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, int N, int dir,
                   double* mu);
 

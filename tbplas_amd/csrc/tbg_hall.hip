@@ -1,5 +1,7 @@
 // Kubo-Bastin Hall conductivity moments (KPM): mu_mn of tbpm_kbdc
-// (tbplas/fortran/tbpm.f90:836-960) on the device.
+// (tbplas/fortran/tbpm.f90:836-960), and the Haydock recursion
+// coefficients of ldos_haydock (tbpm.f90:964-1024, Haydock_coef
+// propagation.f90:85-155), on the device.
 //
 // For each state psi: W_j = J_b T_j(H~) psi (j < N, b = x or y), P_i =
 // T_i(H~) J_x psi (i < N) and mu(i, j) = g_i g_j <P_i | W_j> with the Jackson
@@ -47,7 +49,96 @@ struct Blas {
     }
 };
 
+// Haydock step i on one vector (Kp = 1): buf -= a_i cur + b_{i-1} prev with
+// a_i = dots[3i] + i dots[3i+1] and b_{i-1} = sqrt(nrm[3(i-1)+2]) read on
+// the device (axpbypz, propagation.f90:144).
+__global__ void k_lz_update(double2* __restrict__ buf, const double2* __restrict__ cur,
+                            const double2* __restrict__ prev, int64_t np,
+                            const double* __restrict__ dots, const double* __restrict__ nrm,
+                            int i) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= np) return;
+    const double ar = dots[3 * i], ai = dots[3 * i + 1];
+    const double b = i > 0 ? sqrt(nrm[3 * (i - 1) + 2]) : 0.0;
+    const double2 c = cur[r], p = prev[r];
+    double2 v = buf[r];
+    v.x = v.x - (ar * c.x - ai * c.y) - b * p.x;
+    v.y = v.y - (ar * c.y + ai * c.x) - b * p.y;
+    buf[r] = v;
+}
+
+// buf /= b_i = sqrt(nrm[3i+2]) (propagation.f90:146)
+__global__ void k_lz_scale(double2* __restrict__ buf, int64_t np, const double* __restrict__ nrm,
+                           int i) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= np) return;
+    const double b = sqrt(nrm[3 * i + 2]);
+    double2 v = buf[r];
+    v.x = v.x / b;
+    v.y = v.y / b;
+    buf[r] = v;
+}
+
 }  // namespace
+
+void haydock_impl(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, int depth,
+                  double rescale, double* coefa, double* coefb) {
+    const hipStream_t st = h->stream;
+    Engine<double> e;
+    e.init(h, view_of(H), {0, H->nslices}, 1, chunk_target(h, H->nslices), nullptr);
+    TBG_REQUIRE(e.Kp == 1, "internal: one state per Haydock launch");
+    const int64_t n = H->n, np = e.n_pad;
+    const size_t vb = (size_t)np * sizeof(double2);
+    DevBuf v0(vb), v1(vb), v2(vb);
+    DevBuf dots((size_t)depth * 3 * sizeof(double)), nrm((size_t)depth * 3 * sizeof(double));
+    double* part = e.part.template as<double>();
+    const int* range = e.plan.inst_chunk_dev.template as<int>();
+    const unsigned nb = (unsigned)((np + 255) / 256);
+    std::vector<double> hd((size_t)depth * 3), hn((size_t)depth * 3);
+    for (int s = 0; s < K; ++s) {
+        void* cur = v0.p;
+        void* prev = v1.p;
+        void* buf = v2.p;
+        launch_fill_zero(cur, vb, st);
+        launch_fill_zero(prev, vb, st);
+        launch_pack<double>(psi0 + (size_t)s * n * 2, 1, n, cur, 1, 0, H->perm_ptr(), st);
+        for (int i = 0; i < depth; ++i) {
+            TermArgs a = e.base_args(0, 0);   // buf = rescale H~ cur  (amv)
+            a.pin = cur;
+            a.pout = buf;
+            a.ar[0] = a.ar[1] = rescale;
+            a.ai[0] = a.ai[1] = 0.0;
+            launch_term<double>(F_STORE, a, e.plan.nchunks, st);
+            launch_dotnorm<double>(e.plan, cur, buf, 1, 1, part, st);   // a_i = <cur|buf>
+            launch_reduce(part, 3, range, 1, dots.as<double>(), 0, nullptr, i, st);
+            hipLaunchKernelGGL(k_lz_update, dim3(nb), dim3(256), 0, st,
+                               reinterpret_cast<double2*>(buf),
+                               reinterpret_cast<const double2*>(cur),
+                               reinterpret_cast<const double2*>(prev), np, dots.as<double>(),
+                               nrm.as<double>(), i);
+            TBG_HIP(hipGetLastError());
+            launch_dotnorm<double>(e.plan, buf, buf, 1, 1, part, st);   // b_i = |buf|
+            launch_reduce(part, 3, range, 1, nrm.as<double>(), 0, nullptr, i, st);
+            hipLaunchKernelGGL(k_lz_scale, dim3(nb), dim3(256), 0, st,
+                               reinterpret_cast<double2*>(buf), np, nrm.as<double>(), i);
+            TBG_HIP(hipGetLastError());
+            void* t = prev;
+            prev = cur;
+            cur = buf;
+            buf = t;
+        }
+        TBG_HIP(hipMemcpyAsync(hd.data(), dots.p, hd.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        TBG_HIP(hipMemcpyAsync(hn.data(), nrm.p, hn.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        TBG_HIP(hipStreamSynchronize(st));
+        for (int i = 0; i < depth; ++i) {
+            coefa[((size_t)s * depth + i) * 2 + 0] = hd[(size_t)i * 3 + 0];
+            coefa[((size_t)s * depth + i) * 2 + 1] = hd[(size_t)i * 3 + 1];
+            coefb[(size_t)s * depth + i] = std::sqrt(hn[(size_t)i * 3 + 2]);
+        }
+    }
+}
 
 void hall_mu_impl(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, int N, int dir,
                   double* mu) {

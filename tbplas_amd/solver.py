@@ -4,8 +4,8 @@ Mirrors tbplas/tbpm/solver.py: the constructor's output/config side effects
 (solver.py:84-153), ``calc_corr_dos`` (256-284), ``calc_corr_ldos``
 (286-315), ``calc_corr_ac_cond`` (317-352), ``calc_corr_dyn_pol``
 (354-391), ``calc_corr_dc_cond`` (393-441), ``calc_hall_mu`` (443-470),
-``calc_quasi_eigenstates`` (472-505) and ``calc_psi_t`` (543-600) keep
-their signatures and return shapes.  The f2py Fortran calls are replaced by the HIP library;
+``calc_quasi_eigenstates`` (472-505), ``calc_ldos_haydock`` (507-541)
+and ``calc_psi_t`` (543-600) keep their signatures and return shapes.  The f2py Fortran calls are replaced by the HIP library;
 initial states come from the seeded host generator (tbplas_amd.states) and
 are averaged over samples exactly as the reference accumulates
 ``corr += corrval / n_ran_samples``.  A failed norm check raises
@@ -22,7 +22,8 @@ from . import _lib, states, tune
 from .config import Config
 from .sample import Sample
 from .tbpm import (Hamiltonian, bessel_series, corr_ac, corr_dc, corr_dos, corr_dyn_pol,
-                   corr_ldos, fermi_cheb_coef, hall_mu, ldos_states, psi_t, quasi_eigenstates)
+                   corr_ldos, fermi_cheb_coef, hall_mu, haydock_coef, haydock_green,
+                   ldos_states, psi_t, quasi_eigenstates)
 
 __all__ = ["Solver", "H_BAR_EV"]
 
@@ -252,6 +253,41 @@ class Solver:
         if save_data:
             self._save_data(out, self._output["qe"])
         return out
+
+    def calc_ldos_haydock(self):
+        """(energies (2T+1,), ldos (2T+1,)) by the Haydock recursion at
+        config.LDOS['site_indices'] (solver.py:507-541): random states on
+        the sites, coefficients averaged over samples (tbpm.f90:1000-1016),
+        LDOS = -Im G00(E + i delta) / pi on E = 0.5 i E_range / T, i = -T..T."""
+        if self.size != 1:
+            raise RuntimeError("Using more than 1 mpi process is not allowed"
+                               " for ldos_haydock")
+        gpu = self._gpu()
+        h = self._handle()
+        self._sample.device = gpu["device"]
+        indptr, indices, data = self._sample.build_ham_csr_device()
+        H = Hamiltonian(self._sample.num_orb, indptr, indices, data, precision="fp64",
+                        handle=h)
+        ld = self._config.LDOS
+        sites = np.asarray(ld["site_indices"], dtype=np.int64)
+        num_sample = self._config.generic["nr_random_samples"]
+        seeds = states.sample_seeds(self._config.generic["seed"], num_sample, self.rank)
+        local = states.random_states(len(sites), seeds, gpu["state_kind"])
+        psi0 = np.zeros((num_sample, self._sample.num_orb), dtype=np.complex128)
+        psi0[:, sites] = local
+        depth = int(ld["recursion_depth"])
+        a, b = haydock_coef(H, psi0, depth, self._sample.rescale)
+        coefa = np.zeros(depth, dtype=np.complex128)
+        coefb = np.zeros(depth)
+        for s in range(num_sample):
+            coefa = coefa + a[s] / num_sample
+            coefb = coefb + b[s] / num_sample
+        T = self._config.generic["nr_time_steps"]
+        en_range = self._sample.energy_range
+        energies = np.array([0.5 * i * en_range / T for i in range(-T, T + 1)])
+        ldos = -1.0 / math.pi * haydock_green(coefa, coefb, energies, ld["delta"]).imag
+        self._save_data(ldos, self._output["ldos_haydock"])
+        return energies, ldos
 
     def calc_psi_t(self, psi_0, time_log, dt_scale=1.0, save_data=False):
         """(num_time, num_orb) complex128 wave function psi_0 propagated and

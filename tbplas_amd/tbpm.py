@@ -13,7 +13,8 @@ from . import _lib
 
 __all__ = ["Hamiltonian", "DosRun", "bessel_series", "fermi_cheb_coef", "corr_dos",
            "corr_dos_batch", "corr_ac", "corr_dc", "corr_ldos", "ldos_states", "psi_t",
-           "quasi_eigenstates", "corr_dyn_pol", "hall_mu", "cheb_step", "cheb_poly",
+           "quasi_eigenstates", "corr_dyn_pol", "hall_mu", "haydock_coef", "haydock_green",
+           "cheb_step", "cheb_poly",
            "precision_code"]
 
 
@@ -271,6 +272,31 @@ def corr_dyn_pol(H, bes, coef_F, coef_omF, psi0, n_timestep, q_points, orb_pos,
                                      _lib.ptr(out), ctypes.byref(fs))
     _lib.check(rc, fs)
     return out.cpu().numpy()
+
+
+def haydock_coef(H, psi0, depth, rescale):
+    """Haydock recursion coefficients (Haydock_coef, propagation.f90:85-155)
+    of each starting state psi0 (K, n) under rescale * H~: (a (K, depth)
+    complex128, b (K, depth) float64).  H: complex128."""
+    dv = H.handle.device
+    p, (K, n) = _states_dev(psi0, dv)
+    if n != H.n:
+        raise ValueError("state length does not match the Hamiltonian")
+    a = np.zeros((K, int(depth)), dtype=np.complex128)
+    b = np.zeros((K, int(depth)), dtype=np.float64)
+    _lib.check(_lib.lib().tbg_haydock_coef(H.handle.ptr, H.ptr, _lib.ptr(p), K, int(depth),
+                                           float(rescale), a.ctypes.data, b.ctypes.data))
+    return a, b
+
+
+def haydock_green(coefa, coefb, energies, delta):
+    """G00(E + i delta) by the continued fraction of green_function
+    (funcs.f90:151-172), for all energies at once."""
+    z = np.asarray(energies, dtype=np.float64) + 1j * delta
+    g = np.zeros_like(z)
+    for i in range(len(coefa) - 1, -1, -1):
+        g = 1.0 / (z - coefa[i] - coefb[i] ** 2 * g)
+    return g
 
 
 def hall_mu(H, psi0, n_kernel, direction=1):

@@ -308,3 +308,50 @@ def test_solver_hall_mu_and_analyzer(gpu, tmp_path):
     pref = 16 * s.nr_orbitals / (math.pi * s.energy_range ** 2) / s.area_unit_cell
     oc = oracle.orc_cond_from_trace(mu, [-0.3, 0.0, 0.2], s.rescale, cfg.generic["beta"], 256)
     assert np.max(np.abs(cond - pref * oc)) < 1e-10 * max(1.0, np.max(np.abs(pref * oc)))
+
+
+@pytest.mark.parametrize("field,depth", [(0.0, 24), (20.0, 60)])
+def test_haydock_coef_vs_oracle(gpu, field, depth):
+    s = _sample(L=8, field=field)
+    ip, ix, dt = oracle.build_ham_csr(s.orb_eng, s.hop_i, s.hop_j, s.hop_v)
+    H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64")
+    sites = [0, 7, 30, 31]
+    local = states.random_states(len(sites), states.sample_seeds(3, 2))
+    psi = np.zeros((2, s.num_orb), complex)
+    psi[:, sites] = local
+    a, b = tbpm.haydock_coef(H, psi, depth, s.rescale)
+    for k in range(2):
+        ra, rb = oracle.orc_haydock_coef(ip, ix, dt, s.rescale, psi[k], depth)
+        # Lanczos amplifies rounding with depth; the first levels agree to
+        # rounding, the LDOS from all of them to a loose relative bound
+        assert np.max(np.abs(a[k, :12] - ra[:12])) < 1e-11
+        assert np.max(np.abs(b[k, :12] - rb[:12])) < 1e-11
+        en = np.linspace(-8, 8, 33)
+        g = tbpm.haydock_green(a[k], b[k], en, 0.1).imag
+        r = tbpm.haydock_green(ra, rb, en, 0.1).imag
+        assert np.max(np.abs(g - r)) < 1e-6 * np.max(np.abs(r))
+    H.close()
+
+
+def test_solver_ldos_haydock(gpu, tmp_path):
+    s = _sample(L=8, W=0.0)
+    T, S = 32, 2
+    cfg = _config(T, S)
+    cfg.LDOS["site_indices"] = [2, 3, 40]
+    cfg.LDOS["recursion_depth"] = 20
+    cfg.LDOS["delta"] = 0.05
+    solver = tb.Solver(s, cfg, directory=str(tmp_path / "sim"), prefix="y", echo_details=False)
+    en, ld = solver.calc_ldos_haydock()
+    assert en.shape == (2 * T + 1,) and ld.shape == (2 * T + 1,)
+    assert (tmp_path / "sim" / "y.ldos_haydock.npy").exists()
+    ip, ix, dt = oracle.build_ham_csr(s.orb_eng, s.hop_i, s.hop_j, s.hop_v)
+    local = states.random_states(3, states.sample_seeds(1337, S))
+    ca, cb = np.zeros(20, complex), np.zeros(20)
+    for k in range(S):
+        psi = np.zeros(s.num_orb, complex)
+        psi[[2, 3, 40]] = local[k]
+        a, b = oracle.orc_haydock_coef(ip, ix, dt, s.rescale, psi, 20)
+        ca, cb = ca + a / S, cb + b / S
+    ref = np.array([-oracle.orc_green_function(e, 0.05, ca, cb).imag / math.pi for e in en])
+    assert np.max(np.abs(ld - ref)) < 1e-8 * np.max(np.abs(ref))
+    assert np.all(ld >= 0)     # -Im G / pi of a Jacobi matrix with delta > 0

@@ -500,3 +500,48 @@ def test_orc_kbdc_and_cond_match_reference_live():
     rc = oracle.ref_cond_from_trace(ref, mus, 9.0, 1.0 / (KB * 300), 64)
     oc = oracle.orc_cond_from_trace(ref, mus, 9.0, 1.0 / (KB * 300), 64)
     assert np.max(np.abs(rc - oc)) < 1e-10 * max(1.0, np.max(np.abs(rc)))
+
+
+def test_orc_haydock_moments_and_resolvent_exact():
+    """Lanczos: the tridiagonal T_d of (a, b) reproduces <psi|H^k|psi> for
+    k < 2d, and the continued fraction equals ((z - T_d)^{-1})_00."""
+    ip, ix, dt = _small_disordered(L=4, seed=37)
+    n = len(ip) - 1
+    Hd = _dense_eig_matrix(ip, ix, dt) * 9.0
+    rng = np.random.default_rng(12)
+    psi = np.zeros(n, complex)
+    psi[[1, 5, 6]] = np.exp(2j * np.pi * rng.random(3))
+    psi /= np.linalg.norm(psi)
+    d = 8
+    a, b = oracle.orc_haydock_coef(ip, ix, dt, 9.0, psi, d)
+    Td = np.diag(a) + np.diag(b[:-1], 1) + np.diag(b[:-1], -1)
+    Hk, Tk = np.eye(n), np.eye(d)
+    for k in range(2 * d):
+        m_h = np.vdot(psi, Hk @ psi)
+        assert abs(Tk[0, 0] - m_h) < 1e-10 * max(1.0, abs(m_h)), k
+        Hk, Tk = Hk @ Hd, Tk @ Td
+    for e in (-3.0, 0.2, 5.5):
+        g = oracle.orc_green_function(e, 0.05, a, b)
+        ex = np.linalg.inv((e + 0.05j) * np.eye(d) - Td)[0, 0]
+        assert abs(g - ex) < 1e-10 * abs(ex)
+
+
+@needs_ref
+def test_orc_haydock_matches_reference_live():
+    orb, hi, hj, hv, dr, pos = _graphene(6)
+    rng = np.random.default_rng(41)
+    orb = rng.uniform(-0.3, 0.3, len(orb)) / 9.0
+    ip, ix, dt = oracle.build_ham_csr(orb, hi, hj, hv)
+    n = len(orb)
+    sites, seed, S, depth, T, delta = [3, 10, 20], 11, 2, 16, 40, 0.05
+    en, ref = oracle.ref_ldos_haydock(sites, delta, 18.0, ip, ix, dt, 9.0, seed, depth, T, S)
+    ca = np.zeros(depth, complex)
+    cb = np.zeros(depth)
+    for s in range(S):
+        psi = np.zeros(n, complex)
+        psi[sites] = oracle.ref_random_state(len(sites), seed * (s + 1))
+        a, b = oracle.orc_haydock_coef(ip, ix, dt, 9.0, psi, depth)
+        ca, cb = ca + a / S, cb + b / S
+    assert np.allclose(en, [0.5 * i * 18.0 / T for i in range(-T, T + 1)])
+    ld = np.array([-oracle.orc_green_function(e, delta, ca, cb).imag / math.pi for e in en])
+    assert np.max(np.abs(ld - ref)) < 1e-9 * max(1.0, np.max(np.abs(ref)))
