@@ -248,6 +248,28 @@ int tbg_corr_dyn_pol(tbg_handle *h, const tbg_ham *H, const double *bes,
                      const double *orb_pos, int chk_step, double chk_thr,
                      double *corr, int *fail_step);
 
+/* Replaces the Cython builders build_hop_pbc and build_hop_gen
+ * (tbplas/cython/super.pyx:643-783, 860-986) as SuperCell.get_hop combines
+ * them without a hop modifier (tbplas/builder/super.py:703-775): without
+ * vacancies the periodic terms (in (cell, hop) order) then the terms that
+ * cross an open boundary (in (orbital, hop) order); with vacancies every
+ * term in (orbital, hop) order, dropping terms that leave an open boundary
+ * or land on a vacancy.  All arrays are HOST arrays, the work runs on h's
+ * device: pc_hop_ind (num_hop_pc, 5) int32 (ra, rb, rc, orb_i, orb_j),
+ * pc_hop_eng complex128, dim/pbc (3,) int32, orb_id_pc (num_orb_sc, 4)
+ * int32 (ia, ib, ic, orb) of the kept orbitals, vac_id_sc sorted vacancy
+ * ids in the full-supercell numbering, sc_lattice (3, 3) Cartesian
+ * supercell vectors, sc_orb_pos (num_orb_sc, 3).  Outputs of capacity
+ * terms; *num_hop = the number written.  Bit-identical to the Cython. */
+int tbg_supercell_hops(tbg_handle *h, const int32_t *pc_hop_ind,
+                       const double *pc_hop_eng, int num_hop_pc,
+                       const int32_t *dim, const int32_t *pbc, int num_orb_pc,
+                       const int32_t *orb_id_pc, int64_t num_orb_sc,
+                       const int64_t *vac_id_sc, int64_t num_vac,
+                       const double *sc_lattice, const double *sc_orb_pos,
+                       int64_t capacity, int64_t *hop_i, int64_t *hop_j,
+                       double *hop_v, double *dr, int64_t *num_hop);
+
 /* Haydock recursion coefficients of ldos_haydock (tbplas/fortran/
  * tbpm.f90:964-1024; Haydock_coef, propagation.f90:85-155; called from
  * tbplas/tbpm/solver.py:507-541) for K starting states psi0 (device

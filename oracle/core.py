@@ -13,6 +13,7 @@ __all__ = [
     "orc_cheb_apply", "orc_tbpm_dyn_pol", "ref_tbpm_dyn_pol",
     "jackson_kernel", "orc_tbpm_kbdc", "ref_tbpm_kbdc", "orc_cond_from_trace",
     "ref_cond_from_trace", "orc_haydock_coef", "orc_green_function", "ref_ldos_haydock",
+    "orc_supercell_hops",
     "ref_random_state", "ref_tbpm_dos", "ref_tbpm_accond", "ref_tbpm_dccond",
     "ref_fermi_coef", "ref_tbpm_ldos", "ref_tbpm_psi_t", "ref_tbpm_eigenstates",
     "bessel_series", "build_ham_csr", "build_ham_dxy_loop",
@@ -839,6 +840,102 @@ def graphene_hops_loop(L1, L2, t=-2.7, pbc=(True, True), a=0.246):
     return (np.array(hi, dtype=np.int64), np.array(hj, dtype=np.int64),
             np.full(m, t, dtype=np.complex128), np.array(dr).reshape(m, 3),
             orb_pos)
+
+
+def orc_supercell_hops(pc_lattice, pc_orb_frac, pc_hop_ind, pc_hop_eng, dim, pbc,
+                       vacancies=()):
+    """SuperCell.get_hop without a hop modifier (builder/super.py:703-775),
+    loop for loop: orbitals (ia, ib, ic, orb) minus vacancies
+    (super.pyx:383-443), positions build_orb_pos (super.pyx:569-612);
+    without vacancies split_pc_hop (786-857) then build_hop_pbc (860-986)
+    and build_hop_gen (643-783), with vacancies build_hop_gen on all terms;
+    _wrap_pbc, _fast_div and _id_pc2sc(_vac2) (19-150, 446-539).  Returns
+    orb_pos, hop_i, hop_j, hop_v, dr."""
+    lat = np.asarray(pc_lattice, dtype=np.float64)
+    frac = np.asarray(pc_orb_frac, dtype=np.float64)
+    d0, d1, d2 = (int(x) for x in dim)
+    npc = len(frac)
+    per = [bool(x) for x in pbc]
+    vac = sorted({((a * d1 + b) * d2 + c) * npc + o for a, b, c, o in vacancies})
+    vset = set(vac)
+    orb_id, pos = [], []
+    for ia in range(d0):
+        for ib in range(d1):
+            for ic in range(d2):
+                for o in range(npc):
+                    if (((ia * d1 + ib) * d2 + ic) * npc + o) in vset:
+                        continue
+                    orb_id.append((ia, ib, ic, o))
+                    ra, rb, rc = frac[o, 0] + ia, frac[o, 1] + ib, frac[o, 2] + ic
+                    pos.append([ra * lat[0, x] + rb * lat[1, x] + rc * lat[2, x]
+                                for x in range(3)])
+    pos = np.array(pos).reshape(-1, 3)
+    sc = lat * np.array([d0, d1, d2], dtype=np.float64)[:, None]
+    dims = (d0, d1, d2)
+
+    def wrap(j, n, p):
+        if not p:
+            return j if 0 <= j < n else -1
+        return j % n
+
+    def pc2sc(a, b, c, o):
+        return ((a * d1 + b) * d2 + c) * npc + o
+
+    def sc_vac(idx):
+        if idx in vset:
+            return -1
+        return idx - sum(1 for v in vac if v < idx)
+
+    def dr_of(i, j, n):
+        return [pos[j, x] - pos[i, x] + n[0] * sc[0, x] + n[1] * sc[1, x] + n[2] * sc[2, x]
+                for x in range(3)]
+
+    hops = [tuple(int(v) for v in h) for h in np.asarray(pc_hop_ind).reshape(-1, 5)]
+    eng = list(np.asarray(pc_hop_eng, dtype=np.complex128).reshape(-1))
+    hi, hj, hv, dr = [], [], [], []
+    if not vac:
+        hp = [k for k, h in enumerate(hops) if all(per[x] or h[x] == 0 for x in range(3))]
+    else:
+        hp = []
+    hf = [k for k in range(len(hops)) if k not in hp]
+    for ia in range(d0):                                  # build_hop_pbc
+        for ib in range(d1):
+            for ic in range(d2):
+                oi = pc2sc(ia, ib, ic, 0)
+                for k in hp:
+                    h = hops[k]
+                    if h[0] == 0 and h[1] == 0 and h[2] == 0:
+                        oj, n = oi, (0, 0, 0)
+                    else:
+                        j0 = (ia + h[0], ib + h[1], ic + h[2])
+                        oj = pc2sc(*(wrap(j0[x], dims[x], per[x]) for x in range(3)), 0)
+                        n = tuple(j0[x] // dims[x] for x in range(3))
+                    i, j = oi + h[3], oj + h[4]
+                    hi.append(i)
+                    hj.append(j)
+                    hv.append(eng[k])
+                    dr.append(dr_of(i, j, n))
+    for io, (ia, ib, ic, o) in enumerate(orb_id):         # build_hop_gen
+        for k in hf:
+            h = hops[k]
+            if h[3] != o:
+                continue
+            j0 = (ia + h[0], ib + h[1], ic + h[2])
+            jw = [wrap(j0[x], dims[x], per[x]) for x in range(3)]
+            if -1 in jw:
+                continue
+            j = pc2sc(jw[0], jw[1], jw[2], h[4])
+            if vac:
+                j = sc_vac(j)
+                if j == -1:
+                    continue
+            hi.append(io)
+            hj.append(j)
+            hv.append(eng[k])
+            dr.append(dr_of(io, j, tuple(j0[x] // dims[x] for x in range(3))))
+    m = len(hi)
+    return (pos, np.array(hi, dtype=np.int64), np.array(hj, dtype=np.int64),
+            np.array(hv, dtype=np.complex128), np.array(dr).reshape(m, 3))
 
 
 def graphene_hops_vec(L1, L2, t=-2.7, pbc=(True, True), a=0.246):

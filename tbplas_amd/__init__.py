@@ -12,7 +12,7 @@ The compute runs in hand-written HIP kernels behind the C ABI of
 used only to hold device buffers.
 """
 from .config import Config, read_config
-from .sample import (Sample, make_graphene_sample, graphene_hops,
+from .sample import (Sample, make_graphene_sample, graphene_hops, supercell_hops,
                      add_anderson_disorder, apply_uniaxial_strain,
                      random_vacancies)
 from .solver import Solver

@@ -52,6 +52,8 @@ _SIGS = {
                               ctypes.POINTER(_I)],
     "tbg_hall_mu": [_P, _P, _P, _I, _I, _I, _P],
     "tbg_haydock_coef": [_P, _P, _P, _I, _I, _D, _P, _P],
+    "tbg_supercell_hops": [_P, _P, _P, _I, _P, _P, _I, _P, _I64, _P, _I64, _P, _P, _I64, _P, _P,
+                           _P, _P, ctypes.POINTER(_I64)],
     "tbg_corr_dyn_pol": [_P, _P, _P, _I, _P, _I, _P, _I, _P, _I, _I, _P, _I, _P, _I, _D,
                          _P, ctypes.POINTER(_I)],
     "tbg_corr_ac": [_P, _P, _P, _I, _P, _I, _P, _I, _P, _I, _I, _I, _D, _P,

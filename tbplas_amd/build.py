@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libtbplas_amd.so")
 SOURCES = ["tbg_prop.hip", "tbg_block.hip", "tbg_csr.hip", "tbg_cond.hip", "tbg_api.hip",
-           "tbg_mlp.hip", "tbg_hall.hip"]
+           "tbg_mlp.hip", "tbg_hall.hip", "tbg_super.hip"]
 ARCH = os.environ.get("TBG_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -808,6 +808,41 @@ int tbg_corr_dyn_pol(tbg_handle* h, const tbg_ham* H, const double* bes, int n_b
     });
 }
 
+int tbg_supercell_hops(tbg_handle* h, const int32_t* pc_hop_ind, const double* pc_hop_eng,
+                       int num_hop_pc, const int32_t* dim, const int32_t* pbc, int num_orb_pc,
+                       const int32_t* orb_id_pc, int64_t num_orb_sc, const int64_t* vac_id_sc,
+                       int64_t num_vac, const double* sc_lattice, const double* sc_orb_pos,
+                       int64_t capacity, int64_t* hop_i, int64_t* hop_j, double* hop_v,
+                       double* dr, int64_t* num_hop) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(num_hop_pc >= 0 && num_orb_pc >= 1 && num_orb_sc >= 0 && num_vac >= 0,
+                    "bad sizes");
+        TBG_REQUIRE(dim && pbc && sc_lattice && num_hop && (num_orb_sc == 0 || (orb_id_pc &&
+                    sc_orb_pos)) && (num_hop_pc == 0 || (pc_hop_ind && pc_hop_eng)),
+                    "null argument");
+        TBG_REQUIRE(dim[0] >= 1 && dim[1] >= 1 && dim[2] >= 1, "supercell dimensions must be >= 1");
+        TBG_REQUIRE(num_vac == 0 || vac_id_sc != nullptr, "null vacancy list");
+        for (int64_t k = 1; k < num_vac; ++k)
+            TBG_REQUIRE(vac_id_sc[k] > vac_id_sc[k - 1], "vacancies must be sorted and unique");
+        for (int k = 0; k < num_hop_pc; ++k)
+            TBG_REQUIRE(pc_hop_ind[k * 5 + 3] >= 0 && pc_hop_ind[k * 5 + 3] < num_orb_pc &&
+                            pc_hop_ind[k * 5 + 4] >= 0 && pc_hop_ind[k * 5 + 4] < num_orb_pc,
+                        "hopping orbital index out of range");
+        for (int64_t k = 0; k < num_orb_sc; ++k)
+            for (int x = 0; x < 3; ++x)
+                TBG_REQUIRE(orb_id_pc[k * 4 + x] >= 0 && orb_id_pc[k * 4 + x] < dim[x],
+                            "orbital cell index out of range");
+        TBG_REQUIRE(num_orb_sc + num_vac == (int64_t)dim[0] * dim[1] * dim[2] * num_orb_pc,
+                    "orbital count does not match the supercell and its vacancies");
+        TBG_REQUIRE(capacity >= 0 && (capacity == 0 || (hop_i && hop_j && hop_v && dr)),
+                    "null output");
+        *num_hop = supercell_hops(h, pc_hop_ind, pc_hop_eng, num_hop_pc, dim, pbc, num_orb_pc,
+                                  orb_id_pc, num_orb_sc, vac_id_sc, num_vac, sc_lattice,
+                                  sc_orb_pos, capacity, hop_i, hop_j, hop_v, dr);
+    });
+}
+
 int tbg_haydock_coef(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, int depth,
                      double rescale, double* coefa, double* coefb) {
     return guard([&] {

@@ -32,4 +32,12 @@ void build_sell(tbg_handle* h, tbg_ham* H, const int64_t* indptr, const int64_t*
                 const double* hop, const double* dx, const double* dy, double cur_scale,
                 const int64_t* perm);
 
+// tbg_super.hip: SuperCell.get_hop without a hop modifier (host arrays in
+// and out, work on h's device); returns the number of terms.
+int64_t supercell_hops(tbg_handle* h, const int* pc_hop_ind, const double* pc_hop_eng, int nh,
+                       const int* dim, const int* pbc, int norb_pc, const int* orb_id_pc,
+                       int64_t norb_sc, const int64_t* vac, int64_t nvac, const double* sc_lat,
+                       const double* sc_pos, int64_t capacity, int64_t* hop_i, int64_t* hop_j,
+                       double* hop_v, double* dr);
+
 }  // namespace tbg

@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["Sample", "make_graphene_sample", "graphene_hops",
+__all__ = ["Sample", "make_graphene_sample", "graphene_hops", "supercell_hops",
            "add_anderson_disorder", "apply_uniaxial_strain", "random_vacancies"]
 
 GRAPHENE_A = 0.246      # nm, make_graphene_diamond (materials/graphene.py:16-31)
@@ -325,6 +325,60 @@ class _DevHops:
         self.hop_j = _lib.dev(s.hop_j, np.int64, device)
         self.hop_v = _lib.dev(s.hop_v, np.complex128, device)
         self.dr = _lib.dev(s.dr, np.float64, device) if with_dr else None
+
+
+def supercell_hops(pc_lattice, pc_orb_frac, pc_hop_ind, pc_hop_eng, dim,
+                   pbc=(True, True, True), vacancies=None, device=0):
+    """Supercell arrays of a general primitive cell in the reference's
+    order, generated on the GPU (``tbg_supercell_hops``).
+
+    pc_lattice (3, 3) Cartesian vectors in nm; pc_orb_frac (n_pc, 3)
+    fractional orbital positions; pc_hop_ind (m_pc, 5) reduced hopping terms
+    (ra, rb, rc, orb_i, orb_j) as the primitive cell stores them after the
+    conjugate reduction, pc_hop_eng (m_pc,) complex; dim (3,) repetitions;
+    vacancies: (ia, ib, ic, orb) tuples.  Orbital order (ia, ib, ic, orb)
+    with vacancies skipped (super.pyx:383-443), positions as build_orb_pos
+    (super.pyx:569-612), hop order of SuperCell.get_hop
+    (builder/super.py:703-775).  Returns orb_pos, hop_i, hop_j, hop_v, dr."""
+    import ctypes
+    lat = np.ascontiguousarray(pc_lattice, dtype=np.float64).reshape(3, 3)
+    frac = np.ascontiguousarray(pc_orb_frac, dtype=np.float64).reshape(-1, 3)
+    ind = np.ascontiguousarray(pc_hop_ind, dtype=np.int32).reshape(-1, 5)
+    eng = np.ascontiguousarray(pc_hop_eng, dtype=np.complex128).reshape(-1)
+    dim = np.ascontiguousarray(dim, dtype=np.int32).reshape(3)
+    pbc = np.ascontiguousarray([int(bool(x)) for x in pbc], dtype=np.int32)
+    npc = len(frac)
+    cells = np.indices(tuple(int(d) for d in dim)).reshape(3, -1).T      # ia, ib, ic
+    full = np.concatenate([np.repeat(cells, npc, axis=0),
+                           np.tile(np.arange(npc), len(cells))[:, None]], axis=1)
+    if vacancies is not None and len(vacancies):
+        v = np.asarray(vacancies, dtype=np.int64).reshape(-1, 4)
+        vac = np.unique(((v[:, 0] * dim[1] + v[:, 1]) * dim[2] + v[:, 2]) * npc + v[:, 3])
+    else:
+        vac = np.zeros(0, dtype=np.int64)
+    keep = np.ones(len(full), dtype=bool)
+    keep[vac] = False
+    orb_id = np.ascontiguousarray(full[keep], dtype=np.int32)
+    r = frac[orb_id[:, 3]] + orb_id[:, :3]
+    pos = np.empty((len(orb_id), 3))
+    for x in range(3):
+        pos[:, x] = r[:, 0] * lat[0, x] + r[:, 1] * lat[1, x] + r[:, 2] * lat[2, x]
+    sc_lat = np.ascontiguousarray(lat * dim[:, None].astype(np.float64))
+    cap = max(1, len(ind) * int(np.prod(dim)))
+    hi = np.empty(cap, dtype=np.int64)
+    hj = np.empty(cap, dtype=np.int64)
+    hv = np.empty(cap, dtype=np.complex128)
+    dr = np.empty((cap, 3))
+    m = ctypes.c_int64()
+    h = _lib.default_handle(device)
+    _lib.check(_lib.lib().tbg_supercell_hops(
+        h.ptr, ind.ctypes.data if len(ind) else None, eng.ctypes.data if len(eng) else None,
+        len(ind), dim.ctypes.data, pbc.ctypes.data, npc, orb_id.ctypes.data, len(orb_id),
+        vac.ctypes.data if len(vac) else None, len(vac), sc_lat.ctypes.data,
+        np.ascontiguousarray(pos).ctypes.data, cap, hi.ctypes.data, hj.ctypes.data,
+        hv.ctypes.data, dr.ctypes.data, ctypes.byref(m)))
+    k = m.value
+    return pos, hi[:k].copy(), hj[:k].copy(), hv[:k].copy(), dr[:k].copy()
 
 
 def make_graphene_sample(L1, L2=None, t=GRAPHENE_T, pbc=(True, True),

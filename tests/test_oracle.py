@@ -545,3 +545,67 @@ def test_orc_haydock_matches_reference_live():
     assert np.allclose(en, [0.5 * i * 18.0 / T for i in range(-T, T + 1)])
     ld = np.array([-oracle.orc_green_function(e, delta, ca, cb).imag / math.pi for e in en])
     assert np.max(np.abs(ld - ref)) < 1e-9 * max(1.0, np.max(np.abs(ref)))
+
+
+# -- general supercell hop lists --------------------------------------------------
+GRAPHENE_PC = dict(
+    lat=np.array([[0.246, 0, 0], [0.246 * math.cos(math.pi / 3), 0.246 * math.sin(math.pi / 3), 0],
+                  [0, 0, 1.0]]),
+    frac=np.array([[0.0, 0.0, 0.0], [1 / 3., 1 / 3., 0.0]]),
+    ind=np.array([(0, 0, 0, 0, 1), (1, 0, 0, 1, 0), (0, 1, 0, 1, 0)]),
+    eng=np.full(3, -2.7 + 0j))
+
+
+def _odd_cell():
+    """Three orbitals, hops up to two cells away, a 3-D lattice."""
+    lat = np.array([[0.31, 0.02, 0.0], [-0.05, 0.27, 0.01], [0.0, 0.03, 0.42]])
+    frac = np.array([[0.0, 0.0, 0.0], [0.5, 0.1, 0.2], [0.2, 0.6, 0.7]])
+    ind = np.array([(0, 0, 0, 0, 1), (0, 0, 0, 1, 2), (1, 0, 0, 0, 0), (2, -1, 0, 2, 1),
+                    (0, 1, 1, 1, 0), (-1, 2, 0, 0, 2), (0, 0, 1, 2, 2), (1, 1, -1, 1, 2)])
+    eng = np.array([-1.0, 0.5 + 0.2j, -0.3, 0.1j, 0.7, -0.05 + 0.01j, 0.2, -0.4])
+    return lat, frac, ind, eng
+
+
+def test_orc_supercell_matches_graphene_loop():
+    """The general restatement on graphene's primitive cell reproduces the
+    graphene-specific loop (pinned above), periodic and open."""
+    g = GRAPHENE_PC
+    for L1, L2, pbc in ((4, 3, (True, True, False)), (5, 4, (True, False, False)),
+                        (3, 5, (False, False, False))):
+        pos, hi, hj, hv, dr = oracle.orc_supercell_hops(g["lat"], g["frac"], g["ind"],
+                                                        g["eng"], (L1, L2, 1), pbc)
+        ri, rj, rv, rd, rp = oracle.graphene_hops_loop(L1, L2, pbc=pbc[:2])
+        np.testing.assert_array_equal(pos, rp)
+        np.testing.assert_array_equal(hi, ri)
+        np.testing.assert_array_equal(hj, rj)
+        np.testing.assert_array_equal(dr, rd)
+
+
+def test_orc_supercell_geometry_invariants():
+    """Every term's dr is the primitive hop vector (R + f_j - f_i) A, its
+    ket is the hop's orbital in the wrapped cell, vacancies and open
+    boundaries remove exactly the terms that touch them."""
+    lat, frac, ind, eng = _odd_cell()
+    dim, pbc = (3, 4, 2), (True, False, True)
+    vac = [(0, 1, 1, 2), (2, 3, 0, 0), (1, 0, 1, 1)]
+    pos, hi, hj, hv, dr = oracle.orc_supercell_hops(lat, frac, ind, eng, dim, pbc, vac)
+    full = [(a, b, c, o) for a in range(3) for b in range(4) for c in range(2) for o in range(3)]
+    kept = [x for x in full if x not in vac]
+    assert len(pos) == len(kept)
+    count = 0
+    for io, (a, b, c, o) in enumerate(kept):
+        for k, h in enumerate(ind):
+            if h[3] != o:
+                continue
+            tgt = [a + h[0], b + h[1], c + h[2]]
+            if any(not pbc[x] and not 0 <= tgt[x] < dim[x] for x in range(3)):
+                continue
+            w = tuple(tgt[x] % dim[x] for x in range(3)) + (int(h[4]),)
+            if w in vac:
+                continue
+            count += 1
+            sel = np.where((hi == io) & (hj == kept.index(w)) & (hv == eng[k]))[0]
+            assert len(sel) >= 1
+            exp = (np.asarray(h[:3], float) + frac[h[4]] - frac[o]) @ lat
+            assert np.min(np.abs(dr[sel] - exp).max(axis=1)) < 1e-12
+    assert count == len(hi)
