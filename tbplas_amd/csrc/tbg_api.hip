@@ -1,5 +1,6 @@
 // C ABI of the library (declared in include/tbplas_amd.h) and the DOS,
 // step, polynomial and batched drivers.
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -62,6 +63,47 @@ struct tbg_run {
 
 namespace tbg {
 
+// Three-term tables over a batch's block-diagonal concatenation when every
+// instance has spatial tiles: instance i's tiles shifted to its first row,
+// blocks never spanning two instances (so the per-tile correlation sums
+// stay per instance).  Kept on the handle for the last two batches, since
+// the autotuner starts several runs of the same batch.
+template <typename R>
+static std::shared_ptr<Blocking> batch_blocking(tbg_handle* h, int ni, const tbg_ham* const* hams,
+                                                const Concat& cat,
+                                                const std::vector<int64_t>& inst_slices, int K) {
+    if (!h->blocking || ni < 2) return nullptr;
+    for (int i = 0; i < ni; ++i)
+        if (hams[i]->tiles.size() < 2) return nullptr;
+    const int row_bytes = slots_for(K) * (int)sizeof(typename CplxT<R>::T);
+    std::string key = std::to_string(row_bytes) + "|" + std::to_string(h->block_rows) + "|" +
+                      std::to_string(h->max_lds);
+    for (int i = 0; i < ni; ++i) key += "|" + std::to_string(hams[i]->uid);
+    for (const auto& kv : h->batch_blk)
+        if (kv.first == key) return kv.second;
+    std::vector<std::vector<int64_t>> segs((size_t)ni);
+    int64_t padded = 0;
+    for (int i = 0; i < ni; ++i) {
+        const int64_t off = inst_slices[i] * 64;
+        for (int64_t t : hams[i]->tiles) segs[i].push_back(off + t);
+        padded += hams[i]->padded;
+    }
+    BlockSrc src;
+    const MatView mv = cat.view();
+    src.cols = mv.cols;
+    src.vals = mv.vals;
+    src.sw = mv.sw;
+    src.soff = mv.soff;
+    src.nslices = cat.nslices;
+    src.padded = padded;
+    src.nrows = cat.nslices * 64;
+    src.vsz = cat.real_vals ? sizeof(R) : sizeof(typename CplxT<R>::T);
+    auto B = build_blocking(h, src, segs, row_bytes);
+    h->batch_blk.emplace_back(key, B);
+    if (h->batch_blk.size() > 2) h->batch_blk.erase(h->batch_blk.begin());
+    return B;
+}
+
 template <typename R>
 struct DosRun : tbg_run {
     int n_inst = 0, K = 0, nv = 0;
@@ -87,7 +129,10 @@ struct DosRun : tbg_run {
             concat_hams(h, ni, hams, cat);  // block-diagonal batch
             mv = cat.view();
         }
-        e.init(h, mv, inst_slices, K, chunk_target(h, mv.nslices), ni == 1 ? hams[0] : nullptr);
+        std::shared_ptr<Blocking> pre;
+        if (ni > 1) pre = batch_blocking<R>(h, ni, hams, cat, inst_slices, K);
+        e.init(h, mv, inst_slices, K, chunk_target(h, mv.nslices), ni == 1 ? hams[0] : nullptr,
+               pre);
         const int Kp = e.Kp;
         nv = 3 * Kp;
         psi.alloc(e.vbytes);
@@ -530,6 +575,8 @@ int tbg_ham_create_ordered(tbg_handle* h, int64_t n, const int64_t* indptr,
         TBG_REQUIRE(precision == TBG_FP64 || precision == TBG_FP32, "unknown precision");
         TBG_REQUIRE((dx == nullptr) == (dy == nullptr), "dx and dy must be given together");
         std::unique_ptr<tbg_ham> H(new tbg_ham());
+        static std::atomic<uint64_t> next_uid{1};
+        H->uid = next_uid++;
         H->h = h;
         H->n = n;
         H->precision = precision;

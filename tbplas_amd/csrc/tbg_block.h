@@ -3,6 +3,7 @@
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "tbg_prop.h"
 
@@ -25,6 +26,7 @@ struct BlockMeta {
 struct Blocking {
     int row_bytes = 0;       // bytes of one row of states it was sized for
     int nblocks = 0;
+    std::vector<int> seg_block;   // blocks of segment i: [seg_block[i], seg_block[i+1])
     int nA_max = 0, nB_max = 0;
     double halo = 0.0;       // (sum nA) / n: rows computed per row owned, term m
     DevBuf meta;             // BlockMeta [nblocks]
@@ -38,6 +40,24 @@ struct Blocking {
 // kept on H), or null when H has no tiles, blocking is off on the handle, or
 // a block-tile would not fit the LDS of one workgroup.
 std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_bytes);
+
+// A sliced-ELL matrix on the device (one Hamiltonian, or the
+// block-diagonal concatenation of a batch): nrows rows in the index space.
+struct BlockSrc {
+    const int* cols = nullptr;
+    const void* vals = nullptr;
+    const int* sw = nullptr;
+    const int64_t* soff = nullptr;
+    int64_t nslices = 0, padded = 0, nrows = 0;
+    size_t vsz = 0;            // bytes per stored value
+};
+
+// Tables for tiles given per segment (absolute row offsets, first = the
+// segment's start, last = its end); blocks never cross a segment end; null
+// when a block-tile would not fit the LDS of one workgroup.
+std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
+                                         const std::vector<std::vector<int64_t>>& segs,
+                                         int row_bytes);
 
 // Kinds of pass over the block-tiles (k_term3):
 //   P3_NORMAL  terms m, m+1, m+2: acc += c_m v_m + c_{m+1} v_{m+1} + c_{m+2} v_{m+2}

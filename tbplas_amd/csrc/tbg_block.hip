@@ -39,28 +39,25 @@ namespace tbg {
 // LDS per workgroup the blocking aims at (two workgroups per CU).
 constexpr int kBlockLdsTarget = 72 * 1024;
 
-std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_bytes) {
-    if (!h->blocking || H->tiles.size() < 2) return nullptr;
-    const bool same = H->blk_row_bytes == row_bytes && H->blk_rows_cap == h->block_rows;
-    if (H->blk && same) return H->blk;
-    if (H->blk_failed && same) return nullptr;
-    H->blk.reset();
-    H->blk_failed = false;
-    H->blk_row_bytes = row_bytes;
-    H->blk_rows_cap = h->block_rows;
-
-    const int64_t n = H->n, ns = H->nslices;
-    const size_t vsz = H->vals.bytes / (size_t)std::max<int64_t>(H->padded, 1);
-    std::vector<int> cols((size_t)H->padded), sw((size_t)ns);
+// Tables of one sliced-ELL matrix (device arrays in src) for spatial tiles
+// given per segment (absolute row offsets, first = the segment's start,
+// last = its end): whole tiles are grouped up to the row target, never
+// across a segment end; rows outside every segment are not computed.
+std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
+                                         const std::vector<std::vector<int64_t>>& segs,
+                                         int row_bytes) {
+    const int64_t n = src.nrows, ns = src.nslices;
+    const size_t vsz = src.vsz;
+    std::vector<int> cols((size_t)src.padded), sw((size_t)ns);
     std::vector<int64_t> soff((size_t)ns + 1);
-    std::vector<unsigned char> vals((size_t)H->padded * vsz);
-    TBG_HIP(hipMemcpyAsync(cols.data(), H->cols.p, cols.size() * sizeof(int),
+    std::vector<unsigned char> vals((size_t)src.padded * vsz);
+    TBG_HIP(hipMemcpyAsync(cols.data(), src.cols, cols.size() * sizeof(int),
                            hipMemcpyDeviceToHost, h->stream));
-    TBG_HIP(hipMemcpyAsync(vals.data(), H->vals.p, vals.size(), hipMemcpyDeviceToHost,
+    TBG_HIP(hipMemcpyAsync(vals.data(), src.vals, vals.size(), hipMemcpyDeviceToHost,
                            h->stream));
-    TBG_HIP(hipMemcpyAsync(sw.data(), H->slice_w.p, sw.size() * sizeof(int),
+    TBG_HIP(hipMemcpyAsync(sw.data(), src.sw, sw.size() * sizeof(int),
                            hipMemcpyDeviceToHost, h->stream));
-    TBG_HIP(hipMemcpyAsync(soff.data(), H->slice_off.p, soff.size() * sizeof(int64_t),
+    TBG_HIP(hipMemcpyAsync(soff.data(), src.soff, soff.size() * sizeof(int64_t),
                            hipMemcpyDeviceToHost, h->stream));
     TBG_HIP(hipStreamSynchronize(h->stream));
     auto width = [&](int r) { return sw[r >> 6]; };
@@ -71,21 +68,24 @@ std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_
     const int lds = std::min(h->max_lds, kBlockLdsTarget);
     int64_t target = std::max<int64_t>(64, (int64_t)(lds / (2.6 * row_bytes)));
     // small samples: at least two workgroups per CU
-    target = std::min<int64_t>(target, std::max<int64_t>(64, n / (2 * (int64_t)h->num_cus)));
+    int64_t covered = 0;
+    for (const auto& t : segs) covered += t.back() - t.front();
+    target = std::min<int64_t>(target, std::max<int64_t>(64, covered / (2 * (int64_t)h->num_cus)));
     if (h->block_rows > 0) target = std::min<int64_t>(target, h->block_rows);
-    std::vector<int64_t> starts;
-    {
-        const std::vector<int64_t>& t = H->tiles;
+    std::vector<std::pair<int64_t, int64_t>> spans;   // [r0, r1) per block
+    std::vector<int> seg_block;
+    for (const auto& t : segs) {
+        seg_block.push_back((int)spans.size());
         size_t i = 0;
         while (i + 1 < t.size()) {
-            starts.push_back(t[i]);
             size_t j = i + 1;
             while (j + 1 < t.size() && t[j + 1] - t[i] <= target) ++j;
+            spans.emplace_back(t[i], t[j]);
             i = j;
         }
-        starts.push_back(n);
     }
-    const int nb = (int)starts.size() - 1;
+    seg_block.push_back((int)spans.size());
+    const int nb = (int)spans.size();
     std::vector<BlockMeta> meta((size_t)nb);
     std::vector<int> rows, colL, colG;
     std::vector<unsigned char> valG;
@@ -96,7 +96,7 @@ std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_
     std::vector<int> mark((size_t)n, -1), loc((size_t)n, 0);
     int nA_max = 0, nB_max = 0;
     for (int b = 0; b < nb; ++b) {
-        const int r0 = (int)starts[b], r1 = (int)starts[b + 1];
+        const int r0 = (int)spans[b].first, r1 = (int)spans[b].second;
         const size_t base = rows.size();
         for (int r = r0; r < r1; ++r) {
             mark[r] = b;
@@ -171,16 +171,14 @@ std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_
             TBG_REQUIRE(c >= 0 && c < n, "internal: blocking table out of range");
         }
     }
-    if ((int64_t)(nA_max + nB_max) * row_bytes > h->max_lds) {
-        H->blk_failed = true;
-        return nullptr;
-    }
+    if ((int64_t)(nA_max + nB_max) * row_bytes > h->max_lds) return nullptr;
     auto B = std::make_shared<Blocking>();
     B->row_bytes = row_bytes;
     B->nblocks = nb;
+    B->seg_block = seg_block;
     B->nA_max = nA_max;
     B->nB_max = nB_max;
-    B->halo = (double)rows.size() / (double)n;
+    B->halo = (double)rows.size() / (double)std::max<int64_t>(covered, 1);
     B->meta.alloc(meta.size() * sizeof(BlockMeta));
     B->rows.alloc(rows.size() * sizeof(int));
     B->colL.alloc(std::max<size_t>(colL.size(), 1) * sizeof(int));
@@ -200,6 +198,29 @@ std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_
                                h->stream));
     }
     TBG_HIP(hipStreamSynchronize(h->stream));
+    return B;
+}
+
+std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_bytes) {
+    if (!h->blocking || H->tiles.size() < 2) return nullptr;
+    const bool same = H->blk_row_bytes == row_bytes && H->blk_rows_cap == h->block_rows;
+    if (H->blk && same) return H->blk;
+    if (H->blk_failed && same) return nullptr;
+    H->blk.reset();
+    H->blk_failed = false;
+    H->blk_row_bytes = row_bytes;
+    H->blk_rows_cap = h->block_rows;
+    BlockSrc src;
+    src.cols = H->cols.as<int>();
+    src.vals = H->vals.p;
+    src.sw = H->slice_w.as<int>();
+    src.soff = H->slice_off.as<int64_t>();
+    src.nslices = H->nslices;
+    src.padded = H->padded;
+    src.nrows = H->n;
+    src.vsz = H->vals.bytes / (size_t)std::max<int64_t>(H->padded, 1);
+    auto B = build_blocking(h, src, {H->tiles}, row_bytes);
+    if (!B) H->blk_failed = true;
     H->blk = B;
     return B;
 }

@@ -97,8 +97,11 @@ struct Engine {
     DevBuf Q, S, tile_range;
     bool epi_tiles = false;
 
+    // H: the run's single Hamiltonian (its spatial tiles enable three terms
+    // per pass); pre: tables built elsewhere (a batch's concatenation).
     void init(tbg_handle* hh, const MatView& m, const std::vector<int64_t>& inst_slices,
-              int k_needed, int chunk_target, const tbg_ham* H = nullptr) {
+              int k_needed, int chunk_target, const tbg_ham* H = nullptr,
+              std::shared_ptr<Blocking> pre = nullptr) {
         h = hh;
         M = m;
         Kp = slots_for(k_needed);
@@ -118,17 +121,22 @@ struct Engine {
         launch_fill_zero(P.p, vbytes, h->stream);
         launch_fill_zero(Y.p, vbytes, h->stream);
         blk.reset();
-        if (H != nullptr) blk = get_blocking(h, H, Kp * (int)sizeof(typename CplxT<R>::T));
+        if (pre) blk = pre;
+        else if (H != nullptr) blk = get_blocking(h, H, Kp * (int)sizeof(typename CplxT<R>::T));
         int nparts = plan.nchunks;
         if (blk) {
+            TBG_REQUIRE(blk->row_bytes == Kp * (int)sizeof(typename CplxT<R>::T) &&
+                            (int)blk->seg_block.size() == (int)inst_slices.size(),
+                        "internal: blocking tables of another shape");
             Q.alloc(vbytes);
             S.alloc(vbytes);
             launch_fill_zero(Q.p, vbytes, h->stream);
             launch_fill_zero(S.p, vbytes, h->stream);
-            // partial sums of a pair pass: one row per tile, tiles [0, nblocks)
-            const int range[2] = {0, blk->nblocks};
-            tile_range.alloc(sizeof(range));
-            TBG_HIP(hipMemcpyAsync(tile_range.p, range, sizeof(range), hipMemcpyHostToDevice,
+            // partial sums of a pair pass: one row per tile; instance i owns
+            // tiles [seg_block[i], seg_block[i+1])
+            tile_range.alloc(blk->seg_block.size() * sizeof(int));
+            TBG_HIP(hipMemcpyAsync(tile_range.p, blk->seg_block.data(),
+                                   blk->seg_block.size() * sizeof(int), hipMemcpyHostToDevice,
                                    h->stream));
             TBG_HIP(hipStreamSynchronize(h->stream));
             nparts = std::max(nparts, blk->nblocks);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <memory>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,9 @@ struct tbg_handle {
     int max_lds = 65536;      // LDS bytes one workgroup may use
     tbg::DevBuf scratch;      // reduction partials and small tables
     tbg::DevBuf counter;      // step counter for graph-replayed reductions
+    // three-term tables of recent batches (key: state row bytes, limits and
+    // the instances' ids); a null entry records tiles that do not fit
+    std::vector<std::pair<std::string, std::shared_ptr<tbg::Blocking>>> batch_blk;
 };
 
 // Hamiltonian prepared for propagation: sliced-ELL (64-row slices, column
@@ -131,4 +135,5 @@ struct tbg_ham {
     mutable bool blk_failed = false;
     mutable int blk_row_bytes = 0;
     mutable int blk_rows_cap = 0;
+    uint64_t uid = 0;         // unique per Hamiltonian created in the process
 };

@@ -108,7 +108,7 @@ def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
         for cand in candidates or CANDIDATES:
             timings[cand + (False,)] = timed(cand + (False,))
         best = min(timings, key=timings.get)
-        if len(hams) == 1 and getattr(H, "tiled", False):
+        if all(getattr(Hi, "tiled", False) for Hi in hams):
             blocked = best[:2] + (True,)
             timings[blocked] = timed(blocked)
             best = min(timings, key=timings.get)

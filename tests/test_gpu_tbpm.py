@@ -539,6 +539,52 @@ def test_batch_equals_individual(gpu):
         assert np.max(np.abs(o - r)) < 1e-12
 
 
+@pytest.mark.parametrize("prec,K", [("fp64", 2), ("fp32", 3), ("fp64", 1)])
+def test_batch_three_term_pass(gpu, prec, K):
+    """A batch whose instances all have spatial tiles runs three terms per
+    pass over the concatenation (workgroups of at most 48 rows, never across
+    two instances); every instance matches the oracle and the one-term-per-
+    pass batch, and a second run reuses the cached tables."""
+    hams = {True: [], False: []}
+    psis, refs = [], []
+    bes = None
+    handles = {}
+    for blocked in (True, False):
+        hd = tb._lib.Handle(0)
+        hd.set_blocking(blocked, 48)
+        handles[blocked] = hd
+    for i, L in enumerate((9, 14, 5, 11)):
+        n0 = 2 * L * L
+        s = tb.make_graphene_sample(L, vacancies=tb.random_vacancies(n0, 0.02 * (i % 2), 40 + i))
+        tb.add_anderson_disorder(s, 0.3 * i, 50 + i)
+        if i == 3:
+            s.set_magnetic_field(25.0, 0)
+        s.rescale_ham(9.0)
+        ip, ix, dt = _csr(s)
+        bes = tbpm.bessel_series(tau_of(s), 1e-15)
+        order, tiles = s.tile_order(0.6, return_tiles=True)
+        for blocked in (True, False):
+            hams[blocked].append(tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec,
+                                                  handle=handles[blocked], order=order,
+                                                  tiles=tiles))
+        psi = _states(s.num_orb, K, seed=400 + i)
+        psis.append(psi)
+        refs.append(oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 20))
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    out_b = tbpm.corr_dos_batch(hams[True], bes, psis, 20)
+    out_u = tbpm.corr_dos_batch(hams[False], bes, psis, 20)
+    again = tbpm.corr_dos_batch(hams[True], bes, psis, 20)
+    for ob, ou, oa, r in zip(out_b, out_u, again, refs):
+        assert ob.shape == r.shape
+        assert np.max(np.abs(ob - r)) < tol
+        assert np.max(np.abs(ob - ou)) < (1e-13 if prec == "fp64" else 2e-6)
+        assert np.array_equal(ob, oa)
+    for blocked in (True, False):
+        for H in hams[blocked]:
+            H.close()
+        handles[blocked].close()
+
+
 def _random_order(n, seed):
     return np.random.default_rng(seed).permutation(n).astype(np.int64)
 
