@@ -17,10 +17,13 @@ namespace tbg {
 // their own column), stored column-major so a workgroup reads them
 // coalesced: colG[offG + j*nA + l] (layout column), valG[offG + j*nA + l]
 // (value, the layout's element type) and, for the rows of B,
-// colL[offL + j*nB + l] (position of that column in A).
+// colL[offL + j*nB + l] (position of that column in A, 16 bits: A fits the
+// LDS, so it has fewer than 65536 rows).  rows[offA + l] = r0 + l for the
+// owned rows l < nR.
 struct BlockMeta {
     int64_t offA, offL, offG;
     int nR, nB, nA, w;
+    int r0;
 };
 
 struct Blocking {
@@ -31,7 +34,7 @@ struct Blocking {
     double halo = 0.0;       // (sum nA) / n: rows computed per row owned, term m
     DevBuf meta;             // BlockMeta [nblocks]
     DevBuf rows;             // int32 [sum nA]
-    DevBuf colL;             // int32 [sum w * nB]
+    DevBuf colL;             // uint16 [sum w * nB]
     DevBuf colG;             // int32 [sum w * nA]
     DevBuf valG;             // value [sum w * nA]
 };
@@ -79,7 +82,7 @@ enum PassMode : int { P3_NORMAL = 0, P3_FIRST, P3_M1, P2_LAST, P2_NORM, P2_DOTNO
 struct Term3Args {
     const BlockMeta* meta;
     const int* rows;
-    const int* colL;
+    const uint16_t* colL;
     const int* colG;
     const void* valG;
     const void* p0;

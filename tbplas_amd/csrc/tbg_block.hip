@@ -87,7 +87,8 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
     seg_block.push_back((int)spans.size());
     const int nb = (int)spans.size();
     std::vector<BlockMeta> meta((size_t)nb);
-    std::vector<int> rows, colL, colG;
+    std::vector<int> rows, colG;
+    std::vector<uint16_t> colL;
     std::vector<unsigned char> valG;
     rows.reserve((size_t)(n * 1.6));
     colL.reserve((size_t)(n * 4));
@@ -121,6 +122,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
         const size_t endB = rows.size();
         ring(endR, endB);                    // E2
         const int nR = r1 - r0, nB = (int)(endB - base), nA = (int)(rows.size() - base);
+        if (nA > 65535) return nullptr;     // LDS positions are 16 bits (never fits anyway)
         int wA = 0;
         for (size_t i = base; i < rows.size(); ++i) wA = std::max(wA, width(rows[i]));
         BlockMeta m;
@@ -131,11 +133,12 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
         m.nB = nB;
         m.nA = nA;
         m.w = wA;
+        m.r0 = r0;
         meta[b] = m;
         colL.resize(colL.size() + (size_t)wA * nB);
         colG.resize(colG.size() + (size_t)wA * nA);
         valG.resize(valG.size() + (size_t)wA * nA * vsz, 0);
-        int* cl = colL.data() + m.offL;
+        uint16_t* cl = colL.data() + m.offL;
         int* cg = colG.data() + m.offG;
         unsigned char* vg = valG.data() + (size_t)m.offG * vsz;
         for (int l = 0; l < nA; ++l) {
@@ -149,7 +152,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
                 } else {
                     cg[q] = r;       // zero value on the row's own column
                 }
-                if (l < nB) cl[(size_t)j * nB + l] = j < w ? loc[col(r, j)] : l;
+                if (l < nB) cl[(size_t)j * nB + l] = (uint16_t)(j < w ? loc[col(r, j)] : l);
             }
         }
         nA_max = std::max(nA_max, nA);
@@ -166,6 +169,8 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
                 TBG_REQUIRE(v >= 0 && v < lim, "internal: blocking table out of range");
             }
         }
+        for (int l = 0; l < m.nR; ++l)
+            TBG_REQUIRE(rows[m.offA + l] == m.r0 + l, "internal: blocking table out of order");
         for (int64_t q = 0; q < (int64_t)m.w * m.nA; ++q) {
             const int c = colG[m.offG + q];
             TBG_REQUIRE(c >= 0 && c < n, "internal: blocking table out of range");
@@ -181,7 +186,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
     B->halo = (double)rows.size() / (double)std::max<int64_t>(covered, 1);
     B->meta.alloc(meta.size() * sizeof(BlockMeta));
     B->rows.alloc(rows.size() * sizeof(int));
-    B->colL.alloc(std::max<size_t>(colL.size(), 1) * sizeof(int));
+    B->colL.alloc(std::max<size_t>(colL.size(), 1) * sizeof(uint16_t));
     B->colG.alloc(std::max<size_t>(colG.size(), 1) * sizeof(int));
     B->valG.alloc(std::max<size_t>(valG.size(), 16));
     TBG_HIP(hipMemcpyAsync(B->meta.p, meta.data(), meta.size() * sizeof(BlockMeta),
@@ -189,7 +194,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
     TBG_HIP(hipMemcpyAsync(B->rows.p, rows.data(), rows.size() * sizeof(int),
                            hipMemcpyHostToDevice, h->stream));
     if (!colL.empty())
-        TBG_HIP(hipMemcpyAsync(B->colL.p, colL.data(), colL.size() * sizeof(int),
+        TBG_HIP(hipMemcpyAsync(B->colL.p, colL.data(), colL.size() * sizeof(uint16_t),
                                hipMemcpyHostToDevice, h->stream));
     if (!colG.empty()) {
         TBG_HIP(hipMemcpyAsync(B->colG.p, colG.data(), colG.size() * sizeof(int),
@@ -233,7 +238,8 @@ std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_
 // value at vals[j * nA + l]) of the next row are loaded before the current
 // row's work, so their latency hides behind it.  body(l, t) does the row's
 // work from its prefetched entries t (sums via tab_sum, in j order as
-// row_sum).
+// row_sum).  With rows == nullptr the rows are the owned run r0 + l (the
+// phases after the first only write owned rows), read from no table.
 template <typename R, bool RV, int NJ>
 struct RowTab {
     int gr;
@@ -243,11 +249,11 @@ struct RowTab {
 
 template <typename R, bool RV, int NJ, typename IDX, typename BODY>
 __device__ __forceinline__ void rows_prefetched(int l0, int n, int step, const int* rows,
-                                                const unsigned char* vals, int nA, IDX idx,
-                                                BODY body) {
+                                                int r0, const unsigned char* vals, int nA,
+                                                IDX idx, BODY body) {
     auto load = [&](int l) {
         RowTab<R, RV, NJ> t;
-        t.gr = rows[l];
+        t.gr = rows != nullptr ? rows[l] : r0 + l;
 #pragma unroll
         for (int u = 0; u < NJ; ++u) {
             t.col[u] = idx(u, l);
@@ -310,7 +316,7 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
     }
     const BlockMeta m = a.meta[b];
     const int* rows = a.rows + m.offA;
-    const int* colL = a.colL + m.offL;
+    const uint16_t* colL = a.colL + m.offL;
     const int* colG = a.colG + m.offG;
     const int K = a.K;
     unsigned char* ldsA = tbg_smem3;
@@ -340,7 +346,7 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
     const R beta = (R)a.beta;
     constexpr size_t vsz = RV ? sizeof(R) : sizeof(C);
     const unsigned char* vals = reinterpret_cast<const unsigned char*>(a.valG) + m.offG * vsz;
-    const int w = m.w, nA = m.nA, nB = m.nB, nR = m.nR;
+    const int w = m.w, nA = m.nA, nB = m.nB, nR = m.nR, r0 = m.r0;
     // rows phase A computes, and rows whose v_{m-1} it parks for the next
     // phase: the pair needs one ring less
     const int nAx = PAIR ? nB : nA;
@@ -447,25 +453,25 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
     if constexpr (V >= 2) {
         auto pipelined = [&](auto nj) {
             constexpr int NJ = decltype(nj)::value;
-            rows_prefetched<R, RV, NJ>(rsub, nAx, rpp, rows, vals, nA, idxG,
+            rows_prefetched<R, RV, NJ>(rsub, nAx, rpp, rows, 0, vals, nA, idxG,
                                        [&](int l, const RowTab<R, RV, NJ>& t) {
                 parkA(l, t.gr);
                 rowA(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherA));
             });
             __syncthreads();
             if constexpr (PAIR) {
-                rows_prefetched<R, RV, NJ>(rsub, nR, rpp, rows, vals, nA, idxL,
+                rows_prefetched<R, RV, NJ>(rsub, nR, rpp, nullptr, r0, vals, nA, idxL,
                                            [&](int l, const RowTab<R, RV, NJ>& t) {
                     const Vec<R, V> o = acc0(t.gr);
                     rowP(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherB), o);
                 });
             } else {
-                rows_prefetched<R, RV, NJ>(rsub, nB, rpp, rows, vals, nA, idxL,
+                rows_prefetched<R, RV, NJ>(rsub, nB, rpp, nullptr, r0, vals, nA, idxL,
                                            [&](int l, const RowTab<R, RV, NJ>& t) {
                     rowB(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherB));
                 });
                 __syncthreads();
-                rows_prefetched<R, RV, NJ>(rsub, nR, rpp, rows, vals, nA, idxL,
+                rows_prefetched<R, RV, NJ>(rsub, nR, rpp, nullptr, r0, vals, nA, idxL,
                                            [&](int l, const RowTab<R, RV, NJ>& t) {
                     const Vec<R, V> o = acc0(t.gr);
                     rowC(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherC), o);
@@ -493,7 +499,7 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
         __syncthreads();
         if constexpr (PAIR) {
             for (int l = rsub; l < nR; l += rpp) {
-                const int gr = rows[l];
+                const int gr = r0 + l;
                 const Vec<R, V> o = acc0(gr);
                 rowP(l, gr, row_sum<R, V, RV>(
                                 w, vals, [&](int j) { return colL[j * nB + l]; },
@@ -501,14 +507,14 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
             }
         } else {
             for (int l = rsub; l < nB; l += rpp) {
-                const int gr = rows[l];
+                const int gr = r0 + l;      // used for owned rows only
                 rowB(l, gr, row_sum<R, V, RV>(
                                 w, vals, [&](int j) { return colL[j * nB + l]; },
                                 [&](int j) { return j * nA + l; }, gatherB));
             }
             __syncthreads();
             for (int l = rsub; l < nR; l += rpp) {
-                const int gr = rows[l];
+                const int gr = r0 + l;
                 const Vec<R, V> o = acc0(gr);
                 rowC(l, gr, row_sum<R, V, RV>(
                                 w, vals, [&](int j) { return colL[j * nB + l]; },

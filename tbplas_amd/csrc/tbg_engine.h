@@ -184,7 +184,7 @@ struct Engine {
         Term3Args base{};
         base.meta = blk->meta.as<BlockMeta>();
         base.rows = blk->rows.as<int>();
-        base.colL = blk->colL.as<int>();
+        base.colL = blk->colL.as<uint16_t>();
         base.colG = blk->colG.as<int>();
         base.valG = blk->valG.p;
         base.acc = Y.p;
