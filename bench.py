@@ -176,13 +176,18 @@ def cpu_baseline(args):
 def layout_name(args, shape):
     """Row order and pass mode of a run (matches the traffic profile)."""
     base = "caller order" if args.order == "caller" else f"tiles of {args.tile} nm"
-    return base + (", three terms per pass" if len(shape) > 2 and shape[2] else
-                   ", one term per pass")
+    return base + passes_name(shape)
+
+
+def passes_name(shape):
+    t = int(shape[2]) if len(shape) > 2 else 1
+    return {1: ", one term per pass", 3: ", three terms per pass",
+            4: ", four terms per pass"}[t]
 
 
 def shape_json(shape):
     return {"blocks_per_cu": shape[0], "lds_stage": bool(shape[1]),
-            "three_terms_per_pass": bool(shape[2]) if len(shape) > 2 else False}
+            "terms_per_pass": int(shape[2]) if len(shape) > 2 else 1}
 
 
 def term_bytes(n, nnz, K, s, real=False):
@@ -254,7 +259,7 @@ def dc_workload(args):
                       "n_orbitals": n, "nnz": int(ix.numel()), "energies": en[inds].tolist(),
                       "layout": ("caller order" if order is None
                                  else f"tiles of {args.cond_tile:g} nm")
-                      + (", three terms per pass" if shape[2] else ", one term per pass"),
+                      + passes_name(shape),
                       "launch_shape": shape_json(shape)},
            "dc_corr_t1_mean": [float(np.mean(c[:, a, :, 0]).real) for a in range(2)]}
     print(json.dumps(out), flush=True)
@@ -311,7 +316,7 @@ def ac_workload(args):
                       "n_orbitals": n, "nnz": int(ix.numel()),
                       "layout": ("caller order" if order is None
                                  else f"tiles of {args.cond_tile:g} nm")
-                      + (", three terms per pass" if shape[2] else ", one term per pass"),
+                      + passes_name(shape),
                       "launch_shape": shape_json(shape)},
            "corr_xx_t2_mean": complex(np.mean(c[:, 0, 1])).real}
     H.close()
@@ -347,7 +352,7 @@ def latency_workload(args):
         res[prec] = {"us_per_step": round(ms / T * 1e3, 2),
                      "us_per_term": round(ms / T / (len(bes) - 1) * 1e3, 3),
                      "layout": ("caller order" if order is None else "tiles of 2.0 nm")
-                     + (", three terms per pass" if shape[2] else ", one term per pass"),
+                     + passes_name(shape),
                      "launch_shape": shape_json(shape)}
     out = {"metric": "TBPM microseconds per step, 131,072-orbital graphene, 1 state",
            "value": res["fp64"]["us_per_step"], "unit": "us/step", "n_gpus": 1,

@@ -56,11 +56,12 @@ int tbg_set_tuning(tbg_handle *h, int blocks_per_cu, int stage);
  * counterpart (the reference always stores complex128,
  * tbplas/fortran/csr.f90:9-14). */
 int tbg_set_real_layout(tbg_handle *h, int enable);
-/* Three terms per pass for Hamiltonians created with spatial tiles
- * (tbg_ham_create_ordered); enable = 0 runs them one term per pass (same
- * states, for comparison).  max_rows > 0 caps the rows one workgroup owns
- * (tiles are merged up to it; 0 = sized from the LDS).  Default (1, 0).
- * No reference counterpart. */
+/* Several Chebyshev terms per pass for Hamiltonians created with spatial
+ * tiles (tbg_ham_create_ordered): enable = 1 or 3 three terms, 4 four terms
+ * (tiles with one more halo ring, so smaller tiles fit the LDS), 0 one term
+ * per pass (same states, for comparison).  max_rows > 0 caps the rows one
+ * workgroup owns (tiles are merged up to it; 0 = sized from the LDS).
+ * Default (3, 0).  No reference counterpart. */
 int tbg_set_blocking(tbg_handle *h, int enable, int max_rows);
 
 /* ---- CSR assembly --------------------------------------------------------
@@ -163,13 +164,14 @@ int tbg_ham_info(const tbg_ham *H, int64_t *n, int64_t *nnz,
                  int64_t *padded, int *max_width);
 /* *real = 1 when H stores real values (see tbg_set_real_layout). */
 int tbg_ham_is_real(const tbg_ham *H, int *real);
-/* Temporal blocking of H for `states` vectors of the given precision
- * (building its tables if needed): *nblocks workgroup tiles (0 when H runs
- * one term per pass) and *halo = rows computed per row owned in the first
- * of the three terms.  No reference counterpart. */
+/* Temporal blocking of H for `states` vectors of the given precision and
+ * the handle's terms per pass (building its tables if needed): *nblocks
+ * workgroup tiles (0 when H runs one term per pass) and *halo = rows
+ * computed per row owned in the first term of a pass.  No reference
+ * counterpart. */
 int tbg_ham_blocking_info(tbg_handle *h, const tbg_ham *H, int states,
                           int precision, int *nblocks, double *halo);
-/* Launch shape of the three-term kernel for the same arguments: *max_rows =
+/* Launch shape of the tile-pass kernel for the same arguments: *max_rows =
  * rows of the widest workgroup tile (halo included), *threads = threads
  * per workgroup (both 0 when H runs one term per pass).  No reference
  * counterpart. */

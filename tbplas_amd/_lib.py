@@ -171,11 +171,14 @@ class Handle:
         check(lib().tbg_set_tuning(self._h, int(blocks_per_cu), 1 if stage else 0))
         self.tuning = (int(blocks_per_cu), bool(stage))
 
-    def set_blocking(self, enable=True, max_rows=0):
-        """Three Chebyshev terms per pass for Hamiltonians with spatial
-        tiles (same states; off runs one term per pass); max_rows caps the
-        rows one workgroup owns (0: sized from the LDS)."""
-        check(lib().tbg_set_blocking(self._h, 1 if enable else 0, int(max_rows)))
+    def set_blocking(self, enable=True, max_rows=0, terms=3):
+        """Several Chebyshev terms per pass (terms = 3 or 4) for
+        Hamiltonians with spatial tiles (same states; off runs one term per
+        pass); max_rows caps the rows one workgroup owns (0: sized from the
+        LDS)."""
+        if terms not in (3, 4):
+            raise ValueError("terms per pass must be 3 or 4")
+        check(lib().tbg_set_blocking(self._h, int(terms) if enable else 0, int(max_rows)))
 
     def set_real_layout(self, enable=True):
         """Store real values for Hamiltonians without imaginary parts

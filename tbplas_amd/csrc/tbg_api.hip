@@ -77,7 +77,7 @@ static std::shared_ptr<Blocking> batch_blocking(tbg_handle* h, int ni, const tbg
         if (hams[i]->tiles.size() < 2) return nullptr;
     const int row_bytes = slots_for(K) * (int)sizeof(typename CplxT<R>::T);
     std::string key = std::to_string(row_bytes) + "|" + std::to_string(h->block_rows) + "|" +
-                      std::to_string(h->max_lds);
+                      std::to_string(h->max_lds) + "|" + std::to_string(h->blocking);
     for (int i = 0; i < ni; ++i) key += "|" + std::to_string(hams[i]->uid);
     for (const auto& kv : h->batch_blk)
         if (kv.first == key) return kv.second;
@@ -98,7 +98,7 @@ static std::shared_ptr<Blocking> batch_blocking(tbg_handle* h, int ni, const tbg
     src.padded = padded;
     src.nrows = cat.nslices * 64;
     src.vsz = cat.real_vals ? sizeof(R) : sizeof(typename CplxT<R>::T);
-    auto B = build_blocking(h, src, segs, row_bytes);
+    auto B = build_blocking(h, src, segs, row_bytes, h->blocking - 1);
     h->batch_blk.emplace_back(key, B);
     if (h->batch_blk.size() > 2) h->batch_blk.erase(h->batch_blk.begin());
     return B;
@@ -367,7 +367,9 @@ int tbg_set_blocking(tbg_handle* h, int enable, int max_rows) {
     return guard([&] {
         check_handle(h);
         TBG_REQUIRE(max_rows >= 0, "max_rows must be >= 0");
-        h->blocking = enable ? 1 : 0;
+        TBG_REQUIRE(enable == 0 || enable == 1 || enable == 3 || enable == 4,
+                    "enable must be 0 (off), 1 or 3 (three terms per pass) or 4 (four)");
+        h->blocking = enable == 0 ? 0 : enable == 4 ? 4 : 3;
         h->block_rows = max_rows;
     });
 }

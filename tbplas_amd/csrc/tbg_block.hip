@@ -45,7 +45,8 @@ constexpr int kBlockLdsTarget = 72 * 1024;
 // across a segment end; rows outside every segment are not computed.
 std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
                                          const std::vector<std::vector<int64_t>>& segs,
-                                         int row_bytes) {
+                                         int row_bytes, int depth) {
+    TBG_REQUIRE(depth == 2 || depth == 3, "internal: halo depth must be 2 or 3");
     const int64_t n = src.nrows, ns = src.nslices;
     const size_t vsz = src.vsz;
     std::vector<int> cols((size_t)src.padded), sw((size_t)ns);
@@ -66,7 +67,8 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
 
     // owned rows per block-tile: whole tiles, up to the row target
     const int lds = std::min(h->max_lds, kBlockLdsTarget);
-    int64_t target = std::max<int64_t>(64, (int64_t)(lds / (2.6 * row_bytes)));
+    // LDS rows (A + B) per owned row: about 2.6 with two rings, 3 with three
+    int64_t target = std::max<int64_t>(64, (int64_t)(lds / ((depth == 2 ? 2.6 : 3.0) * row_bytes)));
     // small samples: at least two workgroups per CU
     int64_t covered = 0;
     for (const auto& t : segs) covered += t.back() - t.front();
@@ -117,11 +119,13 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
                 }
             }
         };
-        const size_t endR = rows.size();
-        ring(base, endR);                    // E1
-        const size_t endB = rows.size();
-        ring(endR, endB);                    // E2
-        const int nR = r1 - r0, nB = (int)(endB - base), nA = (int)(rows.size() - base);
+        size_t ends[4];
+        ends[0] = rows.size();
+        for (int k = 1; k <= depth; ++k) {   // E1 .. E_depth
+            ring(k == 1 ? base : ends[k - 2], ends[k - 1]);
+            ends[k] = rows.size();
+        }
+        const int nA = (int)(ends[depth] - base), nB = (int)(ends[depth - 1] - base);
         if (nA > 65535) return nullptr;     // LDS positions are 16 bits (never fits anyway)
         int wA = 0;
         for (size_t i = base; i < rows.size(); ++i) wA = std::max(wA, width(rows[i]));
@@ -129,7 +133,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
         m.offA = (int64_t)base;
         m.offL = (int64_t)colL.size();
         m.offG = (int64_t)colG.size();
-        m.nR = nR;
+        for (int k = 0; k < 4; ++k) m.n[k] = (int)(ends[std::min(k, depth)] - base);
         m.nB = nB;
         m.nA = nA;
         m.w = wA;
@@ -158,18 +162,20 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
         nA_max = std::max(nA_max, nA);
         nB_max = std::max(nB_max, nB);
     }
-    // every LDS index in range: phase B reads A positions (< nA), phase C
-    // reads B positions (< nB) for the owned rows
+    // every LDS index in range: a row of R + E1 + .. + Ek reads positions
+    // in R + E1 + .. + E(k+1), which a pass has computed one phase earlier
     for (int b = 0; b < nb; ++b) {
         const BlockMeta& m = meta[b];
         for (int l = 0; l < m.nB; ++l) {
-            const int lim = l < m.nR ? m.nB : m.nA;
+            int k = 0;
+            while (l >= m.n[k]) ++k;
+            const int lim = m.n[k + 1];
             for (int j = 0; j < m.w; ++j) {
                 const int v = colL[m.offL + (int64_t)j * m.nB + l];
                 TBG_REQUIRE(v >= 0 && v < lim, "internal: blocking table out of range");
             }
         }
-        for (int l = 0; l < m.nR; ++l)
+        for (int l = 0; l < m.n[0]; ++l)
             TBG_REQUIRE(rows[m.offA + l] == m.r0 + l, "internal: blocking table out of order");
         for (int64_t q = 0; q < (int64_t)m.w * m.nA; ++q) {
             const int c = colG[m.offG + q];
@@ -179,6 +185,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
     if ((int64_t)(nA_max + nB_max) * row_bytes > h->max_lds) return nullptr;
     auto B = std::make_shared<Blocking>();
     B->row_bytes = row_bytes;
+    B->depth = depth;
     B->nblocks = nb;
     B->seg_block = seg_block;
     B->nA_max = nA_max;
@@ -208,13 +215,16 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
 
 std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_bytes) {
     if (!h->blocking || H->tiles.size() < 2) return nullptr;
-    const bool same = H->blk_row_bytes == row_bytes && H->blk_rows_cap == h->block_rows;
+    const int depth = h->blocking - 1;
+    const bool same = H->blk_row_bytes == row_bytes && H->blk_rows_cap == h->block_rows &&
+                      H->blk_depth == depth;
     if (H->blk && same) return H->blk;
     if (H->blk_failed && same) return nullptr;
     H->blk.reset();
     H->blk_failed = false;
     H->blk_row_bytes = row_bytes;
     H->blk_rows_cap = h->block_rows;
+    H->blk_depth = depth;
     BlockSrc src;
     src.cols = H->cols.as<int>();
     src.vals = H->vals.p;
@@ -224,7 +234,7 @@ std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_
     src.padded = H->padded;
     src.nrows = H->n;
     src.vsz = H->vals.bytes / (size_t)std::max<int64_t>(H->padded, 1);
-    auto B = build_blocking(h, src, {H->tiles}, row_bytes);
+    auto B = build_blocking(h, src, {H->tiles}, row_bytes, depth);
     if (!B) H->blk_failed = true;
     H->blk = B;
     return B;
@@ -238,8 +248,8 @@ std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_
 // value at vals[j * nA + l]) of the next row are loaded before the current
 // row's work, so their latency hides behind it.  body(l, t) does the row's
 // work from its prefetched entries t (sums via tab_sum, in j order as
-// row_sum).  With rows == nullptr the rows are the owned run r0 + l (the
-// phases after the first only write owned rows), read from no table.
+// row_sum).  With GR the row numbers come from rows[l], else they are the
+// owned run r0 + l (the phases after the first only write owned rows).
 template <typename R, bool RV, int NJ>
 struct RowTab {
     int gr;
@@ -247,13 +257,14 @@ struct RowTab {
     HVal<R, RV> h[NJ];
 };
 
-template <typename R, bool RV, int NJ, typename IDX, typename BODY>
+template <typename R, bool RV, int NJ, bool GR, typename IDX, typename BODY>
 __device__ __forceinline__ void rows_prefetched(int l0, int n, int step, const int* rows,
                                                 int r0, const unsigned char* vals, int nA,
                                                 IDX idx, BODY body) {
     auto load = [&](int l) {
         RowTab<R, RV, NJ> t;
-        t.gr = rows != nullptr ? rows[l] : r0 + l;
+        if constexpr (GR) t.gr = rows[l];
+        else t.gr = r0 + l;
 #pragma unroll
         for (int u = 0; u < NJ; ++u) {
             t.col[u] = idx(u, l);
@@ -294,23 +305,34 @@ __device__ __forceinline__ Vec<R, V> tab_sum(const RowTab<R, RV, NJ>& t, GATHER 
 // says so) to keep enough gathers in flight.  G = K/V lanes share a row,
 // NT/G rows per pass of a phase.
 //
-// MODE (PassMode): three terms m, m+1, m+2 (phase A: v_m on R+E1+E2 from
-// v_{m-1} gathers in HBM/L2 and v_{m-2}; phase B: v_{m+1} on R+E1 from
-// LDS; phase C: v_{m+2} on R, the step output) with the output read
-// (P3_NORMAL), formed from v_{m-2} and v_{m-1} (P3_FIRST), or, for
-// m = 1, formed from v_0 with the first term's coefficient and no v_{-1}
-// (P3_M1); or the step's last two terms (P2_*: v_m on R+E1, v_{m+1} on R,
-// only the step output written) with no epilogue, the FP64 norm, or the
-// FP64 dot with psi0 and the norm, summed per tile into part.
-template <typename R, int V, bool RV, int MODE, int NT>
-__global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
+// T terms m .. m+T-1 per pass (2 <= T <= the tables' depth + 1), phase by
+// phase, n[k] = rows of R and the first k halo rings:
+//   1: v_m on n[T-1] rows from the gathered v_{m-1} (HBM/L2) and v_{m-2},
+//      into LDS area A; the n[T-2] rows of phase 2 also park their v_{m-1}
+//      in (the still unused) area B: read next to this tile's gathers of the
+//      same rows, it mostly hits L2;
+//   2: v_{m+1} on n[T-2] rows from A and the parked v_{m-1}, into B;
+//   3: v_{m+2} on n[T-3] rows from B and v_m.  For T = 4 it goes over v_m in
+//      A (each row reads its own v_m first, and the phase gathers only from
+//      B) and the owned rows' output so far is stored in acc;
+//   4: v_{m+3} on the owned rows from A (v_{m+2}) and B (v_{m+1}).
+// The last phase works on the owned rows only: acc gets the pass's terms
+// (read first, formed from v_0 for PM_M1, or the partial of phase 3), the
+// last two terms go to q1, q2 unless the pass ends the step, and the
+// ending modes sum the FP64 norm / dot per tile into part.
+template <typename R, int V, bool RV, int MODE, int NT, int T>
+__global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
+    static_assert(T >= 2 && T <= 4, "two to four terms per pass");
+    static_assert(T > 2 || MODE >= PM_LAST, "a pair only ends a step");
     using C = typename CplxT<R>::T;
-    constexpr bool PAIR = MODE >= P2_LAST;
-    constexpr bool NORM = MODE == P2_NORM || MODE == P2_DOTNORM;
-    constexpr bool DOT = MODE == P2_DOTNORM;
+    using VT = Vec<R, V>;
+    constexpr bool END = MODE >= PM_LAST;
+    constexpr bool NORM = MODE == PM_NORM || MODE == PM_DOTNORM;
+    constexpr bool DOT = MODE == PM_DOTNORM;
+    constexpr bool M1 = MODE == PM_M1;
     extern __shared__ __align__(16) unsigned char tbg_smem3[];
     const int b = remap_chunk(blockIdx.x, gridDim.x);
-    if constexpr (MODE == P3_M1) {
+    if constexpr (M1) {
         // the step counter the last term's epilogue and the reductions read
         if (a.counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.counter += 1;
     }
@@ -327,8 +349,8 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
     const int rpp = NT >> lgG;
     const int k0 = g * V;
 
-    R ar[V], ai[V], cf[5][V];
-    R ar1[MODE == P3_M1 ? V : 1], ai1[MODE == P3_M1 ? V : 1];
+    R ar[V], ai[V], cf[6][V];
+    R ar1[M1 ? V : 1], ai1[M1 ? V : 1];
 #pragma unroll
     for (int v = 0; v < V; ++v) {
         const int k = k0 + v;
@@ -336,70 +358,27 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
         const bool bwd = ((a.bwd_mask >> k) & 1ull) != 0;
         ar[v] = (R)a.ar[cl];
         ai[v] = bwd ? (R)(-a.ai[cl]) : (R)a.ai[cl];
-        if constexpr (MODE == P3_M1) {
+        if constexpr (M1) {
             ar1[v] = (R)a.ar1[cl];
             ai1[v] = bwd ? (R)(-a.ai1[cl]) : (R)a.ai1[cl];
         }
 #pragma unroll
-        for (int q = 0; q < 5; ++q) cf[q][v] = (R)a.c[q][cl];
+        for (int q = 0; q < 6; ++q) cf[q][v] = (R)a.c[q][cl];
     }
     const R beta = (R)a.beta;
     constexpr size_t vsz = RV ? sizeof(R) : sizeof(C);
     const unsigned char* vals = reinterpret_cast<const unsigned char*>(a.valG) + m.offG * vsz;
-    const int w = m.w, nA = m.nA, nB = m.nB, nR = m.nR, r0 = m.r0;
-    // rows phase A computes, and rows whose v_{m-1} it parks for the next
-    // phase: the pair needs one ring less
-    const int nAx = PAIR ? nB : nA;
-    const int nPark = PAIR ? nR : nB;
-
-    // phase A: v_m from the gathered v_{m-1}.  The rows of the next phase
-    // also park their v_{m-1} in the (still unused) B area of the LDS: read
-    // now, next to this block's gathers of the same rows, it mostly hits L2.
-    auto rowA = [&](int l, int gr, const Vec<R, V>& s) {
-        Vec<R, V> v;
-        if constexpr (MODE == P3_M1) {
-            v = recur<R, V, false>(ar1, ai1, beta, s, s);
-        } else {
-            const Vec<R, V> prev = vld<R, V>(a.p0, (int64_t)gr * K + k0);
-            v = recur<R, V, true>(ar, ai, beta, s, prev);
-        }
-        vst<R, V>(ldsA, (int64_t)l * K + k0, v);
+    const int w = m.w, nA = m.nA, nB = m.nB, r0 = m.r0;
+    const int nR = m.n[0];
+    auto lds = [&](unsigned char* area, int l) { return vld<R, V>(area, (int64_t)l * K + k0); };
+    auto lds_st = [&](unsigned char* area, int l, const VT& x) {
+        vst<R, V>(area, (int64_t)l * K + k0, x);
     };
-    auto parkA = [&](int l, int gr) {
-        if (l < nPark)
-            vst<R, V>(ldsB, (int64_t)l * K + k0, vld<R, V>(a.p1, (int64_t)gr * K + k0));
-    };
-    auto gatherA = [&](int c) { return vld<R, V>(a.p1, (int64_t)c * K + k0); };
-    // phase B: v_{m+1} on R + E1 (each lane reads its own row's parked
-    // v_{m-1} before overwriting it with v_{m+1})
-    auto rowB = [&](int l, int gr, const Vec<R, V>& s) {
-        const Vec<R, V> prev = vld<R, V>(ldsB, (int64_t)l * K + k0);
-        const Vec<R, V> v1 = recur<R, V, true>(ar, ai, beta, s, prev);
-        vst<R, V>(ldsB, (int64_t)l * K + k0, v1);
-        if (l < nR) vst<R, V>(a.q1, (int64_t)gr * K + k0, v1);
-    };
-    auto gatherB = [&](int c) { return vld<R, V>(ldsA, (int64_t)c * K + k0); };
-    // phase C: v_{m+2} on R, and the step output
-    auto rowC = [&](int l, int gr, const Vec<R, V>& s, const Vec<R, V>& acc0) {
-        const int64_t vidx = (int64_t)gr * K + k0;
-        const Vec<R, V> vm = vld<R, V>(ldsA, (int64_t)l * K + k0);
-        const Vec<R, V> v2 = recur<R, V, true>(ar, ai, beta, s, vm);
-        vst<R, V>(a.q2, vidx, v2);
-        const Vec<R, V> v1 = vld<R, V>(ldsB, (int64_t)l * K + k0);
-        Vec<R, V> o = acc0;
-        axpy<R, V>(o, cf[2], vm);
-        axpy<R, V>(o, cf[3], v1);
-        axpy<R, V>(o, cf[4], v2);
-        vst<R, V>(a.acc, vidx, o);
-    };
+    // the output before this pass's terms
     auto acc0 = [&](int gr) {
         const int64_t vidx = (int64_t)gr * K + k0;
-        Vec<R, V> o;
-        if constexpr (MODE == P3_FIRST) {
-            o = vzero<R, V>();
-            axpy<R, V>(o, cf[0], vld<R, V>(a.p0, vidx));
-            axpy<R, V>(o, cf[1], vld<R, V>(a.p1, vidx));
-        } else if constexpr (MODE == P3_M1) {
+        VT o;
+        if constexpr (M1) {
             o = vzero<R, V>();
             axpy<R, V>(o, cf[1], vld<R, V>(a.p1, vidx));
         } else {
@@ -407,23 +386,62 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
         }
         return o;
     };
-    auto gatherC = [&](int c) { return vld<R, V>(ldsB, (int64_t)c * K + k0); };
-    // last two terms: v_{m+1} on R from v_m (LDS A) and the parked v_{m-1};
-    // the step output and its FP64 sums
+    auto none = [&](int, int) { return vzero<R, V>(); };
+
+    // One phase over rows l < n (row numbers from the table with GR, else
+    // owned), sums of the row's entries over gather(position); pre(l, gr)
+    // runs (and its loads issue) before the gathers, body(l, gr, s, pre).
+    // With two or more states per lane, rows of up to four entries (lattice
+    // models) prefetch the next row's table entries before the current
+    // row's work, so each row waits one memory round trip (+4.5% on
+    // 8.4M-orbital graphene, K=8); with one state per lane the workgroup has
+    // 16 waves to hide latency and the prefetch's registers cost more than
+    // they gain (-2%).
+    auto phase = [&](auto grc, int n, auto idx, auto gather, auto pre, auto body) {
+        constexpr bool GR = decltype(grc)::value;
+        if constexpr (V >= 2) {
+            auto piped = [&](auto nj) {
+                constexpr int NJ = decltype(nj)::value;
+                rows_prefetched<R, RV, NJ, GR>(rsub, n, rpp, rows, r0, vals, nA, idx,
+                                               [&](int l, const RowTab<R, RV, NJ>& t) {
+                    const VT p = pre(l, t.gr);
+                    body(l, t.gr, tab_sum<R, V, RV, NJ>(t, gather), p);
+                });
+            };
+            switch (w) {
+                case 1: piped(std::integral_constant<int, 1>{}); return;
+                case 2: piped(std::integral_constant<int, 2>{}); return;
+                case 3: piped(std::integral_constant<int, 3>{}); return;
+                case 4: piped(std::integral_constant<int, 4>{}); return;
+                default: break;
+            }
+        }
+        for (int l = rsub; l < n; l += rpp) {
+            int gr;
+            if constexpr (GR) gr = rows[l];
+            else gr = r0 + l;
+            const VT p = pre(l, gr);
+            body(l, gr, row_sum<R, V, RV>(
+                            w, vals, [&](int j) { return idx(j, l); },
+                            [&](int j) { return j * nA + l; }, gather), p);
+        }
+    };
+    auto idxG = [&](int j, int l) { return colG[j * nA + l]; };
+    auto idxL = [&](int j, int l) { return (int)colL[j * nB + l]; };
+    auto gatherG = [&](int c) { return vld<R, V>(a.p1, (int64_t)c * K + k0); };
+    auto gatherA = [&](int c) { return lds(ldsA, c); };
+    auto gatherB = [&](int c) { return lds(ldsB, c); };
+    const std::true_type table_rows{};
+    const std::false_type owned_rows{};
+
     double dre[V], dim_[V], nrm[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) { dre[v] = 0.0; dim_[v] = 0.0; nrm[v] = 0.0; }
-    auto rowP = [&](int l, int gr, const Vec<R, V>& s, const Vec<R, V>& acc0) {
-        const int64_t vidx = (int64_t)gr * K + k0;
-        const Vec<R, V> prev = vld<R, V>(ldsB, (int64_t)l * K + k0);
-        const Vec<R, V> v1 = recur<R, V, true>(ar, ai, beta, s, prev);
-        const Vec<R, V> vm = vld<R, V>(ldsA, (int64_t)l * K + k0);
-        Vec<R, V> o = acc0;
-        axpy<R, V>(o, cf[2], vm);
-        axpy<R, V>(o, cf[3], v1);
+    // the owned row's new output: stored, and summed by the ending modes
+    auto finish = [&](int64_t vidx, const VT& o) {
         vst<R, V>(a.acc, vidx, o);
         if constexpr (DOT) {
-            const Vec<R, V> p0 = stream_ld<R, V>(a.psi0, vidx);
+            const VT p0 = stream_ld<R, V>(a.psi0, vidx);
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 const double pr = p0.re[v], pi = p0.im[v];
@@ -440,86 +458,91 @@ __global__ void __launch_bounds__(NT) k_term3(Term3Args a) {
             }
         }
     };
-    auto idxG = [&](int j, int l) { return colG[j * nA + l]; };
-    auto idxL = [&](int j, int l) { return colL[j * nB + l]; };
 
-    // With two or more states per lane, rows of up to four entries (lattice
-    // models) prefetch the next row's table entries before the current
-    // row's work, so each row waits one memory round trip (+4.5% on
-    // 8.4M-orbital graphene, K=8); with one state per lane the workgroup has
-    // 16 waves to hide latency and the prefetch's registers cost more than
-    // they gain (-2%).
-    bool done = false;
-    if constexpr (V >= 2) {
-        auto pipelined = [&](auto nj) {
-            constexpr int NJ = decltype(nj)::value;
-            rows_prefetched<R, RV, NJ>(rsub, nAx, rpp, rows, 0, vals, nA, idxG,
-                                       [&](int l, const RowTab<R, RV, NJ>& t) {
-                parkA(l, t.gr);
-                rowA(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherA));
-            });
-            __syncthreads();
-            if constexpr (PAIR) {
-                rows_prefetched<R, RV, NJ>(rsub, nR, rpp, nullptr, r0, vals, nA, idxL,
-                                           [&](int l, const RowTab<R, RV, NJ>& t) {
-                    const Vec<R, V> o = acc0(t.gr);
-                    rowP(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherB), o);
-                });
-            } else {
-                rows_prefetched<R, RV, NJ>(rsub, nB, rpp, nullptr, r0, vals, nA, idxL,
-                                           [&](int l, const RowTab<R, RV, NJ>& t) {
-                    rowB(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherB));
-                });
-                __syncthreads();
-                rows_prefetched<R, RV, NJ>(rsub, nR, rpp, nullptr, r0, vals, nA, idxL,
-                                           [&](int l, const RowTab<R, RV, NJ>& t) {
-                    const Vec<R, V> o = acc0(t.gr);
-                    rowC(l, t.gr, tab_sum<R, V, RV, NJ>(t, gatherC), o);
-                });
-            }
-            done = true;
-        };
-        switch (w) {
-            case 1: pipelined(std::integral_constant<int, 1>{}); break;
-            case 2: pipelined(std::integral_constant<int, 2>{}); break;
-            case 3: pipelined(std::integral_constant<int, 3>{}); break;
-            case 4: pipelined(std::integral_constant<int, 4>{}); break;
-            default: break;
-        }
-    }
-    if (!done) {
-        // one row at a time
-        for (int l = rsub; l < nAx; l += rpp) {
-            const int gr = rows[l];
-            parkA(l, gr);
-            rowA(l, gr, row_sum<R, V, RV>(
-                            w, vals, [&](int j) { return colG[j * nA + l]; },
-                            [&](int j) { return j * nA + l; }, gatherA));
-        }
+    // phase 1: v_m
+    phase(table_rows, m.n[T - 1], idxG, gatherG,
+          [&](int l, int gr) {
+              if (l < m.n[T - 2]) lds_st(ldsB, l, vld<R, V>(a.p1, (int64_t)gr * K + k0));
+              return vzero<R, V>();
+          },
+          [&](int l, int gr, const VT& s, const VT&) {
+              VT v;
+              if constexpr (M1) {
+                  v = recur<R, V, false>(ar1, ai1, beta, s, s);
+              } else {
+                  const VT prev = vld<R, V>(a.p0, (int64_t)gr * K + k0);
+                  v = recur<R, V, true>(ar, ai, beta, s, prev);
+              }
+              lds_st(ldsA, l, v);
+          });
+    __syncthreads();
+    if constexpr (T == 2) {
+        // phase 2 (last): v_{m+1} on the owned rows
+        phase(owned_rows, nR, idxL, gatherA, [&](int, int gr) { return acc0(gr); },
+              [&](int l, int gr, const VT& s, const VT& o0) {
+                  const VT v1 = recur<R, V, true>(ar, ai, beta, s, lds(ldsB, l));
+                  VT o = o0;
+                  axpy<R, V>(o, cf[2], lds(ldsA, l));
+                  axpy<R, V>(o, cf[3], v1);
+                  finish((int64_t)gr * K + k0, o);
+              });
+    } else {
+        // phase 2: v_{m+1} on n[T-2] rows (each lane reads its own row's
+        // parked v_{m-1} before overwriting it)
+        phase(owned_rows, m.n[T - 2], idxL, gatherA, none,
+              [&](int l, int gr, const VT& s, const VT&) {
+                  const VT v1 = recur<R, V, true>(ar, ai, beta, s, lds(ldsB, l));
+                  lds_st(ldsB, l, v1);
+                  if constexpr (T == 3 && !END) {
+                      if (l < nR) vst<R, V>(a.q1, (int64_t)gr * K + k0, v1);
+                  }
+              });
         __syncthreads();
-        if constexpr (PAIR) {
-            for (int l = rsub; l < nR; l += rpp) {
-                const int gr = r0 + l;
-                const Vec<R, V> o = acc0(gr);
-                rowP(l, gr, row_sum<R, V, RV>(
-                                w, vals, [&](int j) { return colL[j * nB + l]; },
-                                [&](int j) { return j * nA + l; }, gatherB), o);
-            }
+        if constexpr (T == 3) {
+            // phase 3 (last): v_{m+2} on the owned rows
+            phase(owned_rows, nR, idxL, gatherB, [&](int, int gr) { return acc0(gr); },
+                  [&](int l, int gr, const VT& s, const VT& o0) {
+                      const int64_t vidx = (int64_t)gr * K + k0;
+                      const VT vm = lds(ldsA, l);
+                      const VT v2 = recur<R, V, true>(ar, ai, beta, s, vm);
+                      if constexpr (!END) vst<R, V>(a.q2, vidx, v2);
+                      VT o = o0;
+                      axpy<R, V>(o, cf[2], vm);
+                      axpy<R, V>(o, cf[3], lds(ldsB, l));
+                      axpy<R, V>(o, cf[4], v2);
+                      finish(vidx, o);
+                  });
         } else {
-            for (int l = rsub; l < nB; l += rpp) {
-                const int gr = r0 + l;      // used for owned rows only
-                rowB(l, gr, row_sum<R, V, RV>(
-                                w, vals, [&](int j) { return colL[j * nB + l]; },
-                                [&](int j) { return j * nA + l; }, gatherB));
-            }
+            // phase 3: v_{m+2} on R + E1, over v_m in A
+            phase(owned_rows, m.n[1], idxL, gatherB,
+                  [&](int l, int gr) { return l < nR ? acc0(gr) : vzero<R, V>(); },
+                  [&](int l, int gr, const VT& s, const VT& o0) {
+                      const VT vm = lds(ldsA, l);
+                      const VT v2 = recur<R, V, true>(ar, ai, beta, s, vm);
+                      lds_st(ldsA, l, v2);
+                      if (l < nR) {
+                          const int64_t vidx = (int64_t)gr * K + k0;
+                          if constexpr (!END) vst<R, V>(a.q1, vidx, v2);
+                          VT o = o0;
+                          axpy<R, V>(o, cf[2], vm);
+                          axpy<R, V>(o, cf[3], lds(ldsB, l));
+                          axpy<R, V>(o, cf[4], v2);
+                          vst<R, V>(a.acc, vidx, o);
+                      }
+                  });
             __syncthreads();
-            for (int l = rsub; l < nR; l += rpp) {
-                const int gr = r0 + l;
-                const Vec<R, V> o = acc0(gr);
-                rowC(l, gr, row_sum<R, V, RV>(
-                                w, vals, [&](int j) { return colL[j * nB + l]; },
-                                [&](int j) { return j * nA + l; }, gatherC), o);
-            }
+            // phase 4 (last): v_{m+3} on the owned rows; the lane that
+            // stored the row's partial output reads it back
+            phase(owned_rows, nR, idxL, gatherA,
+                  [&](int, int gr) { return vld<R, V>(a.acc, (int64_t)gr * K + k0); },
+                  [&](int l, int gr, const VT& s, const VT& o0) {
+                      const int64_t vidx = (int64_t)gr * K + k0;
+                      const VT v3 = recur<R, V, true>(ar, ai, beta, s, lds(ldsB, l));
+                      if constexpr (!END) vst<R, V>(a.q2, vidx, v3);
+                      VT o = o0;
+                      axpy<R, V>(o, cf[5], v3);
+                      finish(vidx, o);
+                  });
         }
     }
 
@@ -576,34 +599,38 @@ int term3_launch_threads(int elem_bytes, bool real_vals, int K, int nA) {
     return (int64_t)nA * (K / V) >= 2048 ? 1024 : 512;
 }
 
-template <typename R, int V, bool RV, int MODE>
-static void launch_f(const Term3Args& a, const Blocking& b, hipStream_t s) {
-    const int nt = term3_launch_threads((int)sizeof(R), RV, a.K, b.nA_max);
+template <typename R, int V, bool RV, int MODE, int T>
+static void launch_f(const PassArgs& a, const Blocking& b, hipStream_t s) {
+    // the instances term3_launch_threads can ask for; complex128 states with
+    // the dot epilogue after three or four terms need more than the 128
+    // registers a lane has at 16 waves, so they take 8
+    constexpr bool wide =
+        V == 1 || (V == 2 && (sizeof(R) == 4 || (RV && !(MODE == PM_DOTNORM && T >= 3))));
+    int nt = term3_launch_threads((int)sizeof(R), RV, a.K, b.nA_max);
+    if (!wide && V >= 2) nt = 512;
     size_t lds = (size_t)(b.nA_max + b.nB_max) * a.K * sizeof(typename CplxT<R>::T);
-    if constexpr (MODE == P2_NORM || MODE == P2_DOTNORM)   // the waves' partial sums
+    if constexpr (MODE == PM_NORM || MODE == PM_DOTNORM)   // the waves' partial sums
         lds = std::max(lds, (size_t)(nt / 64) * 3 * a.K * sizeof(double));
-    // the instances term3_launch_threads can ask for
-    constexpr bool wide = V == 1 || (V == 2 && (sizeof(R) == 4 || RV));
     if constexpr (wide) {
         if (nt == 1024) {
-            hipLaunchKernelGGL((k_term3<R, V, RV, MODE, 1024>), dim3(b.nblocks), dim3(1024),
-                               lds, s, a);
+            hipLaunchKernelGGL((k_block_pass<R, V, RV, MODE, 1024, T>), dim3(b.nblocks),
+                               dim3(1024), lds, s, a);
             TBG_HIP(hipGetLastError());
             return;
         }
     }
     if constexpr (V >= 2) {
         if (nt == 512) {
-            hipLaunchKernelGGL((k_term3<R, V, RV, MODE, 512>), dim3(b.nblocks), dim3(512), lds,
-                               s, a);
+            hipLaunchKernelGGL((k_block_pass<R, V, RV, MODE, 512, T>), dim3(b.nblocks),
+                               dim3(512), lds, s, a);
             TBG_HIP(hipGetLastError());
             return;
         }
     }
-    throw Error{TBG_EINVAL, "internal: no three-term kernel for this workgroup size"};
+    throw Error{TBG_EINVAL, "internal: no tile-pass kernel for this workgroup size"};
 }
 
-// Allow every instance of the three-term kernel the LDS of one workgroup
+// Allow every instance of the tile-pass kernel the LDS of one workgroup
 // (tables are sized up to it).  Called once per handle, outside any stream
 // capture.
 static void allow_lds_fn(const void* fn, int bytes) {
@@ -613,14 +640,25 @@ static void allow_lds_fn(const void* fn, int bytes) {
         (void)hipGetLastError();
 }
 
+template <typename R, int V, bool RV, int NT, int MODE>
+static void allow_lds_mode(int bytes) {
+    if constexpr (MODE >= PM_LAST)
+        allow_lds_fn(reinterpret_cast<const void*>(&k_block_pass<R, V, RV, MODE, NT, 2>), bytes);
+    // not instantiated (launch_f takes 512 threads there)
+    constexpr bool narrow = NT == 1024 && V == 2 && sizeof(R) == 8 && MODE == PM_DOTNORM;
+    if constexpr (!narrow) {
+        allow_lds_fn(reinterpret_cast<const void*>(&k_block_pass<R, V, RV, MODE, NT, 3>), bytes);
+        allow_lds_fn(reinterpret_cast<const void*>(&k_block_pass<R, V, RV, MODE, NT, 4>), bytes);
+    }
+}
+
 template <typename R, int V, bool RV, int NT>
 static void allow_lds_modes(int bytes) {
-    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P3_NORMAL, NT>), bytes);
-    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P3_FIRST, NT>), bytes);
-    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P3_M1, NT>), bytes);
-    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P2_LAST, NT>), bytes);
-    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P2_NORM, NT>), bytes);
-    allow_lds_fn(reinterpret_cast<const void*>(&k_term3<R, V, RV, P2_DOTNORM, NT>), bytes);
+    allow_lds_mode<R, V, RV, NT, PM_NORMAL>(bytes);
+    allow_lds_mode<R, V, RV, NT, PM_M1>(bytes);
+    allow_lds_mode<R, V, RV, NT, PM_LAST>(bytes);
+    allow_lds_mode<R, V, RV, NT, PM_NORM>(bytes);
+    allow_lds_mode<R, V, RV, NT, PM_DOTNORM>(bytes);
 }
 
 template <typename R, int V, int NT>
@@ -641,35 +679,54 @@ void prepare_term3(int max_lds) {
     allow_lds<double, 3, 512>(max_lds);
 }
 
-template <typename R, int V, bool RV>
-static void launch_v(int mode, const Term3Args& a, const Blocking& b, hipStream_t s) {
+template <typename R, int V, bool RV, int T>
+static void launch_t(int mode, const PassArgs& a, const Blocking& b, hipStream_t s) {
     switch (mode) {
-        case P3_NORMAL: launch_f<R, V, RV, P3_NORMAL>(a, b, s); break;
-        case P3_FIRST: launch_f<R, V, RV, P3_FIRST>(a, b, s); break;
-        case P3_M1: launch_f<R, V, RV, P3_M1>(a, b, s); break;
-        case P2_LAST: launch_f<R, V, RV, P2_LAST>(a, b, s); break;
-        case P2_NORM: launch_f<R, V, RV, P2_NORM>(a, b, s); break;
-        case P2_DOTNORM: launch_f<R, V, RV, P2_DOTNORM>(a, b, s); break;
-        default: throw Error{TBG_EINVAL, "internal: unknown pass mode"};
+        case PM_LAST: launch_f<R, V, RV, PM_LAST, T>(a, b, s); return;
+        case PM_NORM: launch_f<R, V, RV, PM_NORM, T>(a, b, s); return;
+        case PM_DOTNORM: launch_f<R, V, RV, PM_DOTNORM, T>(a, b, s); return;
+        default: break;
+    }
+    if constexpr (T >= 3) {
+        switch (mode) {
+            case PM_NORMAL: launch_f<R, V, RV, PM_NORMAL, T>(a, b, s); return;
+            case PM_M1: launch_f<R, V, RV, PM_M1, T>(a, b, s); return;
+            default: break;
+        }
+    }
+    throw Error{TBG_EINVAL, "internal: unknown pass mode for this pass length"};
+}
+
+template <typename R, int V, bool RV>
+static void launch_v(int mode, int T, const PassArgs& a, const Blocking& b, hipStream_t s) {
+    switch (T) {
+        case 2: launch_t<R, V, RV, 2>(mode, a, b, s); break;
+        case 3: launch_t<R, V, RV, 3>(mode, a, b, s); break;
+        case 4: launch_t<R, V, RV, 4>(mode, a, b, s); break;
+        default: throw Error{TBG_EINVAL, "internal: two to four terms per pass"};
     }
 }
 
 template <typename R>
-void launch_term3(int mode, bool real_vals, const Term3Args& a, const Blocking& b,
-                  hipStream_t s) {
+void launch_pass(int mode, int T, bool real_vals, const PassArgs& a, const Blocking& b,
+                 hipStream_t s) {
     const int V = term3_vectors_per_lane(a.K);
     if (a.K > 64 || a.lgG < 0 || a.lgG > 6 || (1 << a.lgG) * V != a.K)
         throw Error{TBG_EINVAL, "internal: bad vector slot count"};
     if (b.row_bytes != a.K * (int)sizeof(typename CplxT<R>::T))
         throw Error{TBG_EINVAL, "internal: blocking built for another row size"};
-    if ((mode == P2_NORM || mode == P2_DOTNORM) && (a.part == nullptr || a.nv < 3 * a.K))
+    if (T < 2 || T > b.depth + 1)
+        throw Error{TBG_EINVAL, "internal: more terms per pass than the tables' halo rings allow"};
+    if ((mode == PM_NORM || mode == PM_DOTNORM) && (a.part == nullptr || a.nv < 3 * a.K))
         throw Error{TBG_EINVAL, "internal: epilogue without a partial-sum buffer"};
-    if (mode == P2_DOTNORM && a.psi0 == nullptr)
+    if (mode == PM_DOTNORM && a.psi0 == nullptr)
         throw Error{TBG_EINVAL, "internal: dot epilogue without psi0"};
+    if (mode < PM_LAST && (a.q1 == nullptr || a.q2 == nullptr))
+        throw Error{TBG_EINVAL, "internal: a pass that continues the step needs q1 and q2"};
 #define TBG_T3(VV)                                                             \
     do {                                                                       \
-        if (real_vals) launch_v<R, VV, true>(mode, a, b, s);                   \
-        else launch_v<R, VV, false>(mode, a, b, s);                            \
+        if (real_vals) launch_v<R, VV, true>(mode, T, a, b, s);                \
+        else launch_v<R, VV, false>(mode, T, a, b, s);                         \
     } while (0)
     switch (V) {
         case 1: TBG_T3(1); break;
@@ -679,7 +736,7 @@ void launch_term3(int mode, bool real_vals, const Term3Args& a, const Blocking& 
     }
 #undef TBG_T3
 }
-template void launch_term3<float>(int, bool, const Term3Args&, const Blocking&, hipStream_t);
-template void launch_term3<double>(int, bool, const Term3Args&, const Blocking&, hipStream_t);
+template void launch_pass<float>(int, int, bool, const PassArgs&, const Blocking&, hipStream_t);
+template void launch_pass<double>(int, int, bool, const PassArgs&, const Blocking&, hipStream_t);
 
 }  // namespace tbg

@@ -170,18 +170,19 @@ struct Engine {
         return a;
     }
 
-    // Three terms per pass: terms 1, 2, 3 (P3_M1), then triples; the last
-    // one or two terms end the step: with two left, one pair pass that also
-    // forms the epilogue's sums (per tile); with one left, the one-term
-    // kernel with the epilogue; with three left, one one-term launch, then
-    // the pair.  The step output receives the terms in the order m = 0, 1,
-    // 2, ... as in the one-term schedule, so the states are bit-identical to
-    // it; the epilogue's sums of a pair pass are grouped by tile instead of
-    // by chunk.  21 Bessel terms: 7 launches (m = 1-3, five triples, 19-20).
+    // T = depth + 1 (3 or 4) terms per pass: terms 1 .. T (PM_M1), then
+    // passes of T; the rest r of the N - 1 - T terms after them end the
+    // step with the epilogue's sums (per tile): r = 0 as the last full pass,
+    // r = 2 or 3 as one pass of r terms, r = 1 with the one-term kernel.
+    // The step output receives the terms in the order m = 0, 1, 2, ... as in
+    // the one-term schedule, so the states are bit-identical to it; the
+    // epilogue's sums of a pass are grouped by tile instead of by chunk.
+    // 21 Bessel terms: 5 launches with T = 4, 7 with T = 3 (the last a pair).
     void step_blocked(const StepPoly& sp, uint64_t bwd, uint64_t cls, int epi,
                       const void* psi0, int* counter) {
         const int N = sp.N;   // terms m = 1 .. N-1
-        Term3Args base{};
+        const int T = blk->depth + 1;
+        PassArgs base{};
         base.meta = blk->meta.as<BlockMeta>();
         base.rows = blk->rows.as<int>();
         base.colL = blk->colL.as<uint16_t>();
@@ -199,23 +200,23 @@ struct Engine {
         base.lgG = lgG3;
         base.nA_max = blk->nA_max;
         base.nv = 3 * Kp;
-        auto pass = [&](int mode, int m, void* p0, void* p1, void* q1, void* q2) {
-            Term3Args t = base;
+        auto pass = [&](int mode, int terms, int m, void* p0, void* p1, void* q1, void* q2) {
+            PassArgs t = base;
             t.p0 = p0;
             t.p1 = p1;
             t.q1 = q1;
             t.q2 = q2;
-            for (int q = 0; q < 5; ++q) {
-                const int j = m - 2 + q;   // c_{m-2} .. c_{m+2}, those in the series
+            for (int q = 0; q < 6; ++q) {
+                const int j = m - 2 + q;   // c_{m-2} .. c_{m+3}, those in the series
                 t.c[q][0] = (j >= 0 && j < N) ? sp.c[0][j] : 0.0;
                 t.c[q][1] = (j >= 0 && j < N) ? sp.c[1][j] : 0.0;
             }
-            if (mode == P3_M1) t.counter = counter;
-            if (mode == P2_NORM || mode == P2_DOTNORM) {
+            if (mode == PM_M1) t.counter = counter;
+            if (mode == PM_NORM || mode == PM_DOTNORM) {
                 t.psi0 = psi0;
                 t.part = part.as<double>();
             }
-            launch_term3<R>(mode, M.real_vals != 0, t, *blk, h->stream);
+            launch_pass<R>(mode, terms, M.real_vals != 0, t, *blk, h->stream);
         };
         // v_{m-2}, v_{m-1} and two free buffers; X (v_0) is free after the
         // first pass and Y receives the output (X and Y swap at the end)
@@ -223,29 +224,14 @@ struct Engine {
         void* b1 = Q.p;
         void* b2 = S.p;
         void* b3 = X.p;
-        pass(P3_M1, 1, nullptr, X.p, b0, b1);
-        int m = 4;
-        const int rem = N - 4;   // terms m = 4 .. N-1
-        const int tail = rem % 3 == 0 ? 3 : rem % 3;
-        for (int i = 0; i < (rem - tail) / 3; ++i, m += 3) {
-            pass(P3_NORMAL, m, b0, b1, b2, b3);
+        pass(PM_M1, T, 1, nullptr, X.p, b0, b1);
+        int m = T + 1;
+        const int rem = N - m;   // terms m .. N-1, at least one (N >= T + 2)
+        const int tail = rem % T == 0 ? T : rem % T;
+        for (int i = 0; i < (rem - tail) / T; ++i, m += T) {
+            pass(PM_NORMAL, T, m, b0, b1, b2, b3);
             std::swap(b0, b2);
             std::swap(b1, b3);
-        }
-        if (tail == 3) {   // one term alone, leaving the pair
-            TermArgs a = base_args(bwd, cls);
-            a.acc = Y.p;
-            a.pin = b1;
-            a.pprev = b0;
-            a.pout = b0;
-            a.ar[0] = a.ar[1] = sp.ar;
-            a.ai[0] = a.ai[1] = sp.ai;
-            a.beta = sp.beta;
-            a.cc[0] = sp.c[0][m];
-            a.cc[1] = sp.c[1][m];
-            launch_term<R>(F_PREV | F_ACC | F_ACCRMW | F_STORE, a, plan.nchunks, h->stream);
-            std::swap(b0, b1);
-            ++m;
         }
         if (tail == 1) {
             TermArgs a = base_args(bwd, cls);
@@ -263,9 +249,9 @@ struct Engine {
             launch_term<R>(F_PREV | F_ACC | F_ACCRMW | epi, a, plan.nchunks, h->stream);
             epi_tiles = false;
         } else {
-            const int mode = (epi & F_DOT) ? P2_DOTNORM : (epi & F_NORM) ? P2_NORM : P2_LAST;
-            pass(mode, m, b0, b1, nullptr, nullptr);
-            epi_tiles = mode != P2_LAST;
+            const int mode = (epi & F_DOT) ? PM_DOTNORM : (epi & F_NORM) ? PM_NORM : PM_LAST;
+            pass(mode, tail, m, b0, b1, nullptr, nullptr);
+            epi_tiles = mode != PM_LAST;
         }
         std::swap(X, Y);
         parity ^= 1;
@@ -276,7 +262,7 @@ struct Engine {
               int* counter) {
         const int N = sp.N;
         TBG_REQUIRE(N >= 3, "at least 3 expansion terms are required");
-        if (blk && N >= 6) {
+        if (blk && N >= blk->depth + 3) {
             step_blocked(sp, bwd, cls, epi, psi0, counter);
             return;
         }

@@ -4,7 +4,8 @@ For a prepared Hamiltonian, a state count and a precision it times a few
 Chebyshev steps (HIP events on the library stream, after warm-up steps) for
 every candidate (persistent blocks per CU, LDS staging of the slice rows)
 and keeps the fastest; for a Hamiltonian with spatial tiles it then times
-the best shape with three terms per pass and keeps that if it is faster.
+the best shape with three and with four terms per pass and keeps the
+fastest.  A shape is (blocks per CU, staging, terms per pass: 1, 3 or 4).
 Results are cached per (device name, library build, instances, rows, stored
 elements, widest slice, value layout, row order and tile count, states,
 precision) in memory and in a JSON file
@@ -21,7 +22,7 @@ from . import _lib
 __all__ = ["autotune", "apply", "CANDIDATES"]
 
 CANDIDATES = [(bpc, stage) for bpc in (4, 8, 12, 16) for stage in (True, False)]
-DEFAULT = (16, True, True)
+DEFAULT = (16, True, 3)
 
 _lock = threading.Lock()
 _mem = {}
@@ -87,7 +88,7 @@ def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
                 return best, None
             disk = _load_disk()
             if key in disk and len(disk[key]) == 3:
-                best = (int(disk[key][0]), bool(disk[key][1]), bool(disk[key][2]))
+                best = (int(disk[key][0]), bool(disk[key][1]), int(disk[key][2]))
                 _mem[key] = best
                 _set(H.handle, best)
                 return best, None
@@ -106,11 +107,12 @@ def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
 
         timings = {}
         for cand in candidates or CANDIDATES:
-            timings[cand + (False,)] = timed(cand + (False,))
+            timings[cand + (1,)] = timed(cand + (1,))
         best = min(timings, key=timings.get)
         if all(getattr(Hi, "tiled", False) for Hi in hams):
-            blocked = best[:2] + (True,)
-            timings[blocked] = timed(blocked)
+            for terms in (3, 4):
+                blocked = best[:2] + (terms,)
+                timings[blocked] = timed(blocked)
             best = min(timings, key=timings.get)
         _set(H.handle, best)
         _mem[key] = best
@@ -123,7 +125,7 @@ def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
 
 def _set(handle, shape):
     handle.set_tuning(shape[0], shape[1])
-    handle.set_blocking(shape[2])
+    handle.set_blocking(shape[2] > 1, terms=max(int(shape[2]), 3))
 
 
 def apply(H, K, bes, enabled=True):

@@ -179,8 +179,9 @@ def test_tile_order_equals_caller_order(gpu, prec):
 
 def _same_correlations(a, b, exact):
     """Blocked and one-term correlations: identical when the step ends with
-    a single term (same partial sums); when it ends with a pair pass the
-    FP64 sums are grouped by tile instead of by chunk, a few ulps."""
+    a single term (same partial sums); when it ends with a pass of two or
+    more terms the FP64 sums are grouped by tile instead of by chunk, a few
+    ulps."""
     if exact:
         np.testing.assert_array_equal(a, b)
     else:
@@ -197,11 +198,13 @@ def _blocked_sample(L=48):
 @pytest.mark.parametrize("prec,K", [("fp32", 8), ("fp32", 3), ("fp32", 1), ("fp64", 1),
                                     ("fp64", 4)])
 @pytest.mark.parametrize("nb_prec", [1e-15, 1e-14, 1e-6])
-def test_three_term_pass_bitwise(gpu, prec, K, nb_prec):
-    """Three terms per pass over spatial tiles gives the same states, bit
-    for bit, as one term per pass, and the same correlations up to the
+@pytest.mark.parametrize("terms", [3, 4])
+def test_three_term_pass_bitwise(gpu, prec, K, nb_prec, terms):
+    """Three or four terms per pass over spatial tiles give the same states,
+    bit for bit, as one term per pass, and the same correlations up to the
     grouping of their final FP64 sums; series lengths 21, 20 and 12 end the
-    step with a pair pass, one single term, and a pair pass."""
+    step with a pair pass, one single term and a pair pass (three terms per
+    pass), or a full pass, a triple and a triple (four)."""
     s = _blocked_sample()
     ip, ix, dt = _csr(s)
     order, tiles = s.tile_order(0.8, return_tiles=True)
@@ -209,17 +212,18 @@ def test_three_term_pass_bitwise(gpu, prec, K, nb_prec):
     psi = _states(s.num_orb, K, seed=21)
     h = tb._lib.Handle(0)
     try:
+        h.set_blocking(True, terms=terms)
         H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec, handle=h, order=order,
                              tiles=tiles)
         nblk, halo = H.blocking_info(K)
-        assert nblk > 1 and 1.0 < halo < 4.0
+        assert nblk > 1 and 1.0 < halo < 5.0
         a = tbpm.corr_dos(H, bes, psi, 30, chk_thr=1e-4)
         sa = tbpm.cheb_step(H, bes, psi, True)
         h.set_blocking(False)
         assert H.blocking_info(K) == (0, 0.0)
         b = tbpm.corr_dos(H, bes, psi, 30, chk_thr=1e-4)
         np.testing.assert_array_equal(sa, tbpm.cheb_step(H, bes, psi, True))
-        _same_correlations(a, b, exact=(len(bes) - 4) % 3 == 1)
+        _same_correlations(a, b, exact=(len(bes) - terms - 1) % terms == 1)
         ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 30)
         assert np.max(np.abs(a - ref)) < (1e-12 if prec == "fp64" else 2e-5)
         H.close()
@@ -228,10 +232,12 @@ def test_three_term_pass_bitwise(gpu, prec, K, nb_prec):
 
 
 @pytest.mark.parametrize("prec,K", [("fp64", 4), ("fp32", 6)])
-def test_three_term_pass_scattered_tiles(gpu, prec, K):
+@pytest.mark.parametrize("terms", [3, 4])
+def test_three_term_pass_scattered_tiles(gpu, prec, K, terms):
     """Tiles that are not spatial (a random row order cut every 16 rows):
     the halo rings then reach far across the sample, which the tables must
-    still get right; the states stay bit-identical to one term per pass."""
+    still get right; the states stay bit-identical to one term per pass
+    (the correlations too where the step ends with a single term)."""
     s = tb.make_graphene_sample(12, vacancies=tb.random_vacancies(2 * 12 * 12, 0.05, 6))
     tb.add_anderson_disorder(s, 0.9, 7)
     s.set_magnetic_field(25.0, 0)
@@ -242,15 +248,17 @@ def test_three_term_pass_scattered_tiles(gpu, prec, K):
     psi = _states(n, K, seed=23)
     h = tb._lib.Handle(0)
     try:
-        h.set_blocking(True, 16)
+        h.set_blocking(True, 16, terms=terms)
         H = tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h,
                              order=_random_order(n, 12), tiles=_row_tiles(n, 16))
         nblk, halo = H.blocking_info(K)
         assert nblk == (n + 15) // 16 and halo > 2.0
         a = tbpm.corr_dos(H, bes, psi, 25, chk_thr=1e-4)
+        sa = tbpm.cheb_step(H, bes, psi, True)
         h.set_blocking(False)
         b = tbpm.corr_dos(H, bes, psi, 25, chk_thr=1e-4)
-        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(sa, tbpm.cheb_step(H, bes, psi, True))
+        _same_correlations(a, b, exact=(len(bes) - terms - 1) % terms == 1)
         ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 25)
         assert np.max(np.abs(a - ref)) < (1e-12 if prec == "fp64" else 2e-5)
         H.close()
@@ -258,7 +266,8 @@ def test_three_term_pass_scattered_tiles(gpu, prec, K):
         h.close()
 
 
-def test_three_term_pass_wide_rows(gpu):
+@pytest.mark.parametrize("terms", [3, 4])
+def test_three_term_pass_wide_rows(gpu, terms):
     """Rows of ~9 entries (a random sparse Hermitian matrix, wider than the
     lattice rows the pipelined phase is specialised for)."""
     rng = np.random.default_rng(31)
@@ -276,14 +285,16 @@ def test_three_term_pass_wide_rows(gpu):
     psi = _states(n, 2, seed=24)
     h = tb._lib.Handle(0)
     try:
-        h.set_blocking(True, 64)
+        h.set_blocking(True, 64, terms=terms)
         H = tbpm.Hamiltonian(n, ip, ix, dt, precision="fp64", handle=h,
                              tiles=_row_tiles(n, 32))
         assert H.blocking_info(2)[0] > 1
         a = tbpm.corr_dos(H, bes, psi, 20)
+        sa = tbpm.cheb_step(H, bes, psi, True)
         h.set_blocking(False)
         b = tbpm.corr_dos(H, bes, psi, 20)
-        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(sa, tbpm.cheb_step(H, bes, psi, True))
+        _same_correlations(a, b, exact=(len(bes) - terms - 1) % terms == 1)
         ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 20)
         assert np.max(np.abs(a - ref)) < 1e-12
         H.close()
@@ -293,13 +304,16 @@ def test_three_term_pass_wide_rows(gpu):
 
 @pytest.mark.parametrize("prec,K,field", [("fp32", 8, 0.0), ("fp32", 16, 0.0), ("fp64", 8, 0.0),
                                           ("fp64", 2, 0.0), ("fp64", 8, 30.0)])
-def test_three_term_pass_workgroup_sizes(gpu, prec, K, field):
-    """Tiles of the benchmark's size (3.5 nm on 131,072-orbital graphene):
-    the three-term kernel runs with 16 waves per workgroup where the widest
+@pytest.mark.parametrize("terms", [3, 4])
+def test_three_term_pass_workgroup_sizes(gpu, prec, K, field, terms):
+    """Tiles of the benchmark's size (3.5 nm on 131,072-orbital graphene;
+    3.0 nm with four terms per pass, whose three halo rings would not fit
+    the LDS at 128 bytes per row):
+    the tile-pass kernel runs with 16 waves per workgroup where the widest
     tile gives every lane (K/2 per row) two rows, else with 8, and always
     with 8 for complex values in complex128 (registers); the states stay
     bit-identical to one term per pass either way (the step of 21 terms
-    ends with a pair pass)."""
+    ends with a pair pass, or a full pass of four terms)."""
     s = tb.make_graphene_sample(256)
     tb.add_anderson_disorder(s, 0.4, 15)
     if field:
@@ -307,11 +321,12 @@ def test_three_term_pass_workgroup_sizes(gpu, prec, K, field):
     s.rescale_ham(9.0)
     ip, ix, dt = _csr(s)
     n = s.num_orb
-    order, tiles = s.tile_order(3.5, return_tiles=True)
+    order, tiles = s.tile_order(3.5 if terms == 3 else 3.0, return_tiles=True)
     bes = tbpm.bessel_series(tau_of(s), 1e-15)
     psi = _states(n, K, seed=27)
     h = tb._lib.Handle(0)
     try:
+        h.set_blocking(True, terms=terms)
         H = tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h, order=order,
                              tiles=tiles)
         assert H.info()["real_values"] == (field == 0.0)
@@ -334,7 +349,8 @@ def test_three_term_pass_workgroup_sizes(gpu, prec, K, field):
         h.close()
 
 
-def test_three_term_pass_graph_and_step(gpu):
+@pytest.mark.parametrize("terms", [3, 4])
+def test_three_term_pass_graph_and_step(gpu, terms):
     """hipGraph replay of blocked steps equals launched steps; one blocked
     step (cheb_step) equals the oracle's step."""
     s = _blocked_sample(32)
@@ -344,6 +360,7 @@ def test_three_term_pass_graph_and_step(gpu):
     psi = _states(s.num_orb, 4, seed=22)
     h = tb._lib.Handle(0)
     try:
+        h.set_blocking(True, terms=terms)
         H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp32", handle=h, order=order,
                              tiles=tiles)
         assert H.blocking_info(4)[0] > 1
@@ -539,8 +556,9 @@ def test_batch_equals_individual(gpu):
         assert np.max(np.abs(o - r)) < 1e-12
 
 
-@pytest.mark.parametrize("prec,K", [("fp64", 2), ("fp32", 3), ("fp64", 1)])
-def test_batch_three_term_pass(gpu, prec, K):
+@pytest.mark.parametrize("prec,K,terms", [("fp64", 2, 3), ("fp32", 3, 3), ("fp64", 1, 3),
+                                          ("fp64", 2, 4), ("fp32", 8, 4)])
+def test_batch_three_term_pass(gpu, prec, K, terms):
     """A batch whose instances all have spatial tiles runs three terms per
     pass over the concatenation (workgroups of at most 48 rows, never across
     two instances); every instance matches the oracle and the one-term-per-
@@ -551,7 +569,7 @@ def test_batch_three_term_pass(gpu, prec, K):
     handles = {}
     for blocked in (True, False):
         hd = tb._lib.Handle(0)
-        hd.set_blocking(blocked, 48)
+        hd.set_blocking(blocked, 48, terms=terms)
         handles[blocked] = hd
     for i, L in enumerate((9, 14, 5, 11)):
         n0 = 2 * L * L
@@ -593,21 +611,22 @@ def _row_tiles(n, rows):
     return np.r_[np.arange(0, n, rows), n].astype(np.int64)
 
 
-@pytest.mark.parametrize("layout", ["plain", "ordered", "tiles"])
+@pytest.mark.parametrize("layout", ["plain", "ordered", "tiles", "tiles4"])
 def test_ac_reference_fixture(gpu, layout):
     """ordered: the device layout stores the rows in a random order (every
-    gather leaves its slice); tiles: three terms per pass, workgroups of 32
-    rows; results stay in the caller's order."""
+    gather leaves its slice); tiles: three terms per pass (tiles4: four),
+    workgroups of 32 rows; results stay in the caller's order."""
     g = golden("ref_ac.npz")
     n = len(g["indptr"]) - 1
-    hd = tb._lib.Handle(0) if layout == "tiles" else None
+    tiled = layout.startswith("tiles")
+    hd = tb._lib.Handle(0) if tiled else None
     if hd is not None:
-        hd.set_blocking(True, 32)
+        hd.set_blocking(True, 32, terms=4 if layout == "tiles4" else 3)
     H = tbpm.Hamiltonian(n, g["indptr"], g["indices"], g["hop"], g["dx"], g["dy"],
                          cur_scale=float(g["rescale"]), precision="fp64", handle=hd,
                          order=_random_order(n, 7) if layout == "ordered" else None,
-                         tiles=_row_tiles(n, 32) if layout == "tiles" else None)
-    if layout == "tiles":
+                         tiles=_row_tiles(n, 32) if tiled else None)
+    if tiled:
         assert H.blocking_info(3 * g["psi0"].shape[0])[0] == 4
     cF = tbpm.fermi_cheb_coef(2 ** 15, float(g["beta_re"]), float(g["mu_re"]), False, 1e-12)
     cO = tbpm.fermi_cheb_coef(2 ** 15, float(g["beta_re"]), float(g["mu_re"]), True, 1e-12)
@@ -656,18 +675,19 @@ def test_fermi_coef_product_vs_reference_fixture(gpu):
     assert np.max(np.abs(c - g["coef_F_cold"])) < 1e-14
 
 
-@pytest.mark.parametrize("layout", ["plain", "ordered", "tiles"])
+@pytest.mark.parametrize("layout", ["plain", "ordered", "tiles", "tiles4"])
 def test_dc_reference_fixture(gpu, layout):
     g = golden("ref_dc.npz")
     n = len(g["indptr"]) - 1
-    hd = tb._lib.Handle(0) if layout == "tiles" else None
+    tiled = layout.startswith("tiles")
+    hd = tb._lib.Handle(0) if tiled else None
     if hd is not None:
-        hd.set_blocking(True, 32)
+        hd.set_blocking(True, 32, terms=4 if layout == "tiles4" else 3)
     H = tbpm.Hamiltonian(n, g["indptr"], g["indices"], g["hop"], g["dx"], g["dy"],
                          cur_scale=float(g["rescale"]), precision="fp64", handle=hd,
                          order=_random_order(n, 8) if layout == "ordered" else None,
-                         tiles=_row_tiles(n, 32) if layout == "tiles" else None)
-    if layout == "tiles":
+                         tiles=_row_tiles(n, 32) if tiled else None)
+    if tiled:
         assert H.blocking_info(2 * g["psi0"].shape[0])[0] == 4
     dos, dc = tbpm.corr_dc(H, g["bes"], g["psi0"], int(g["T"]), float(g["t_step"]),
                            g["energies"], g["en_inds"])
