@@ -67,8 +67,11 @@ def parse():
     p.add_argument("--order", choices=("caller", "tiles"), default="tiles",
                    help="row order of the device layout: the sample's own, or square "
                         "tiles of --tile nm (Sample.tile_order), which also lets the "
-                        "tuner choose three Chebyshev terms per pass")
-    p.add_argument("--tile", type=float, default=3.5)
+                        "tuner choose three or four Chebyshev terms per pass")
+    p.add_argument("--tile", type=float, default=5.0,
+                   help="dos and batch workloads: tile edge in nm of the spatial row order "
+                        "(5.0 nm fits 8 complex64 states a row with three halo rings in "
+                        "the LDS of one workgroup)")
     p.add_argument("--cond-tile", type=float, default=1.5,
                    help="dc and ac workloads: tile edge in nm of a spatial row order "
                         "(0: the sample's own order)")
@@ -504,10 +507,10 @@ def main():
                      "term_avg_us": round(t_step / nterm * 1e6, 2)},
     }
     layout = layout_name(args, shape32)
-    if shape32[2]:
+    if shape32[2] > 1:
         out["roofline"]["note"] = (
             "achieved = the one-term-per-pass algorithmic bytes per term over the "
-            "measured time per term; three terms per pass read the state vectors "
+            "measured time per term; several terms per pass read the state vectors "
             "fewer times, so it can exceed what one term per pass could reach "
             "(traffic: bytes per term actually moved, from the counters)")
     traffic_file = os.path.join(HERE, "profiles", "traffic_term_fp32_k8.json")

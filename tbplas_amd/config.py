@@ -16,10 +16,12 @@ look at:
                        (the reference's Exp(1) amplitudes)    default 'phase'
     gpu['fp32_norm_thr'] norm-check threshold used on the fp32 path, which
                        drifts ~1e-6 per hundred steps         default 1e-4
-    gpu['tile_nm']     DOS runs: store the rows in square tiles of this size
-                       (nm, from the orbital positions) so the tuner may run
-                       three Chebyshev terms per pass; same results, 0 keeps
-                       the sample's order                     default 3.5
+    gpu['tile_nm']     DOS, LDOS and quasi-eigenstate runs: store the rows
+                       in square tiles of this size (nm, from the orbital
+                       positions) so the tuner may run three or four
+                       Chebyshev terms per pass; same results, 0 keeps the
+                       sample's order, 'auto' sizes the tiles for the
+                       states per row (sample.auto_tile_nm)   default 'auto'
     gpu['cond_tile_nm'] the same for AC and DC runs (more vectors per row,
                        so smaller tiles fit the LDS)          default 1.5
 """
@@ -66,7 +68,7 @@ class Config:
                     'max_vectors': 16,
                     'state_kind': 'phase',
                     'fp32_norm_thr': 1.0e-4,
-                    'tile_nm': 3.5,
+                    'tile_nm': 'auto',
                     'cond_tile_nm': 1.5}
         self._legal_params = dict()
         self.set_legal_params()

@@ -2,30 +2,33 @@
 //
 // The one-term kernel reads v_{m-1} (gathered) and v_{m-2} and writes v_m
 // for every term, ~3.6 vector streams per term.  Here one workgroup owns a
-// block-tile R of rows (a run of the caller's spatial tiles) and forms three
-// terms in one pass:
-//   phase A  v_m     on A = R + E1 + E2   from v_{m-1} (HBM/L2) and v_{m-2}
-//   phase B  v_{m+1} on B = R + E1        from v_m (LDS) and v_{m-1}
-//   phase C  v_{m+2} on R                 from v_{m+1} (LDS) and v_m (LDS)
-// E1, E2 are the two halo rings of R.  Only v_{m+1} and v_{m+2} of the owned
-// rows are written (the next pass needs exactly those two), and the step
-// output is updated once per three terms.  Per three terms and owned row:
-// reads of v_{m-1} and v_{m-2} on the rings (~1.3 each for compact tiles),
-// two writes, one output read-modify-write: ~2.2 streams per term.  The
-// halo rows are computed redundantly by neighbouring blocks.
+// block-tile R of rows (a run of the caller's spatial tiles) and forms T = 3
+// or 4 terms in one pass (T - 1 halo rings E1, E2, E3 of R):
+//   phase 1  v_m     on R + E1 .. E_{T-1}  from v_{m-1} (HBM/L2) and v_{m-2}
+//   phase 2  v_{m+1} on R + E1 .. E_{T-2}  from v_m (LDS) and v_{m-1}
+//   phase 3  v_{m+2} on R (+ E1 if T = 4)  from v_{m+1} (LDS) and v_m (LDS)
+//   phase 4  v_{m+3} on R (T = 4)          from v_{m+2} (LDS) and v_{m+1}
+// Only the last two terms of the owned rows are written (the next pass
+// needs exactly those two), and the step output is updated once per pass.
+// Per pass and owned row: reads of v_{m-1} and v_{m-2} on the rings (~1.3
+// to 1.5 each for compact tiles), two writes, one output read-modify-write:
+// about 2.2 streams per term at T = 3 and 1.7 at T = 4.  The halo rows are
+// computed redundantly by neighbouring blocks.
 //
 // The entries of a block's rows are copied into per-block tables (columns,
-// values and, for the rows of B, positions in A), column-major, so each
-// phase reads its entries with one coalesced load before its gathers
-// instead of following row -> slice -> entry.  Workgroups are large (8 or 16
-// waves) because the tile's LDS leaves room for about two per CU.
+// values and, for the rows that gather from LDS, positions in the tile),
+// column-major, so each phase reads its entries with one coalesced load
+// before its gathers instead of following row -> slice -> entry.
+// Workgroups are large (8 or 16 waves) because a tile's LDS leaves room for
+// one or two per CU.
 //
 // Every row sum, recurrence step and output update uses the helpers of
 // tbg_vec.h in the same order as the one-term kernel, so the states and the
-// step output are bit-identical to it (the correlations too: they are formed
-// by the one-term kernel of the step's last term).
+// step output are bit-identical to it; the FP64 correlation sums of a pass
+// that ends the step are grouped by tile instead of by chunk.
 #include <algorithm>
 #include <cstring>
+#include <deque>
 #include <type_traits>
 
 #include "tbg_block.h"
@@ -74,94 +77,137 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
     for (const auto& t : segs) covered += t.back() - t.front();
     target = std::min<int64_t>(target, std::max<int64_t>(64, covered / (2 * (int64_t)h->num_cus)));
     if (h->block_rows > 0) target = std::min<int64_t>(target, h->block_rows);
-    std::vector<std::pair<int64_t, int64_t>> spans;   // [r0, r1) per block
+    // A block-tile whose A and B areas would not fit the LDS budget is split
+    // in two (between its tiles when it has several, else in the middle of
+    // its rows) and each half tried again; the budget shrinks and the whole
+    // build is redone when the widest A and widest B of different blocks
+    // together would not fit.
+    struct Span {
+        size_t seg, i, j;    // tiles [i, j) of segment seg (j = i + 1: part of one)
+        int64_t r0, r1;      // rows [r0, r1)
+    };
+    std::vector<BlockMeta> meta;
     std::vector<int> seg_block;
-    for (const auto& t : segs) {
-        seg_block.push_back((int)spans.size());
-        size_t i = 0;
-        while (i + 1 < t.size()) {
-            size_t j = i + 1;
-            while (j + 1 < t.size() && t[j + 1] - t[i] <= target) ++j;
-            spans.emplace_back(t[i], t[j]);
-            i = j;
-        }
-    }
-    seg_block.push_back((int)spans.size());
-    const int nb = (int)spans.size();
-    std::vector<BlockMeta> meta((size_t)nb);
     std::vector<int> rows, colG;
     std::vector<uint16_t> colL;
     std::vector<unsigned char> valG;
-    rows.reserve((size_t)(n * 1.6));
-    colL.reserve((size_t)(n * 4));
-    colG.reserve((size_t)(n * 5));
-    valG.reserve((size_t)(n * 5) * vsz);
     std::vector<int> mark((size_t)n, -1), loc((size_t)n, 0);
     int nA_max = 0, nB_max = 0;
-    for (int b = 0; b < nb; ++b) {
-        const int r0 = (int)spans[b].first, r1 = (int)spans[b].second;
-        const size_t base = rows.size();
-        for (int r = r0; r < r1; ++r) {
-            mark[r] = b;
-            loc[r] = (int)(rows.size() - base);
-            rows.push_back(r);
-        }
-        auto ring = [&](size_t from, size_t to) {
-            for (size_t i = from; i < to; ++i) {
-                const int r = rows[i];
-                for (int j = 0, w = width(r); j < w; ++j) {
-                    const int c = col(r, j);
-                    if (mark[c] != b) {
-                        mark[c] = b;
-                        loc[c] = (int)(rows.size() - base);
-                        rows.push_back(c);
+    int64_t budget = h->max_lds;
+    for (int attempt = 0;; ++attempt) {
+        meta.clear();
+        seg_block.clear();
+        rows.clear();
+        colG.clear();
+        colL.clear();
+        valG.clear();
+        std::fill(mark.begin(), mark.end(), -1);
+        nA_max = nB_max = 0;
+        for (size_t sg = 0; sg < segs.size(); ++sg) {
+            const std::vector<int64_t>& t = segs[sg];
+            seg_block.push_back((int)meta.size());
+            std::deque<Span> todo;
+            size_t i = 0;
+            while (i + 1 < t.size()) {
+                size_t j = i + 1;
+                while (j + 1 < t.size() && t[j + 1] - t[i] <= target) ++j;
+                todo.push_back(Span{sg, i, j, t[i], t[j]});
+                i = j;
+            }
+            while (!todo.empty()) {
+                const Span sp = todo.front();
+                todo.pop_front();
+                const int b = (int)meta.size();
+                const int r0 = (int)sp.r0, r1 = (int)sp.r1;
+                const size_t base = rows.size();
+                for (int r = r0; r < r1; ++r) {
+                    mark[r] = b;
+                    loc[r] = (int)(rows.size() - base);
+                    rows.push_back(r);
+                }
+                auto ring = [&](size_t from, size_t to) {
+                    for (size_t q = from; q < to; ++q) {
+                        const int r = rows[q];
+                        for (int jj = 0, w = width(r); jj < w; ++jj) {
+                            const int c = col(r, jj);
+                            if (mark[c] != b) {
+                                mark[c] = b;
+                                loc[c] = (int)(rows.size() - base);
+                                rows.push_back(c);
+                            }
+                        }
+                    }
+                };
+                size_t ends[4];
+                ends[0] = rows.size();
+                for (int k = 1; k <= depth; ++k) {   // E1 .. E_depth
+                    ring(k == 1 ? base : ends[k - 2], ends[k - 1]);
+                    ends[k] = rows.size();
+                }
+                const int nA = (int)(ends[depth] - base), nB = (int)(ends[depth - 1] - base);
+                // (LDS positions are 16 bits: nA < 65536 whenever A fits the LDS)
+                if ((int64_t)(nA + nB) * row_bytes > budget || nA > 65535) {
+                    for (size_t q = base; q < rows.size(); ++q) mark[rows[q]] = -1;
+                    rows.resize(base);
+                    if (r1 - r0 < 32) return nullptr;   // a few rows with too wide a halo
+                    Span lo = sp, hi = sp;
+                    if (sp.j - sp.i >= 2) {
+                        const size_t mid = (sp.i + sp.j) / 2;
+                        lo.j = mid;
+                        lo.r1 = t[mid];
+                        hi.i = mid;
+                        hi.r0 = t[mid];
+                    } else {
+                        lo.r1 = hi.r0 = sp.r0 + (sp.r1 - sp.r0) / 2;
+                    }
+                    todo.push_front(hi);
+                    todo.push_front(lo);
+                    continue;
+                }
+                int wA = 0;
+                for (size_t q = base; q < rows.size(); ++q) wA = std::max(wA, width(rows[q]));
+                BlockMeta m;
+                m.offA = (int64_t)base;
+                m.offL = (int64_t)colL.size();
+                m.offG = (int64_t)colG.size();
+                for (int k = 0; k < 4; ++k) m.n[k] = (int)(ends[std::min(k, depth)] - base);
+                m.nB = nB;
+                m.nA = nA;
+                m.w = wA;
+                m.r0 = r0;
+                meta.push_back(m);
+                colL.resize(colL.size() + (size_t)wA * nB);
+                colG.resize(colG.size() + (size_t)wA * nA);
+                valG.resize(valG.size() + (size_t)wA * nA * vsz, 0);
+                uint16_t* cl = colL.data() + m.offL;
+                int* cg = colG.data() + m.offG;
+                unsigned char* vg = valG.data() + (size_t)m.offG * vsz;
+                for (int l = 0; l < nA; ++l) {
+                    const int r = rows[base + l];
+                    const int w = width(r);
+                    for (int jj = 0; jj < wA; ++jj) {
+                        const size_t q = (size_t)jj * nA + l;
+                        if (jj < w) {
+                            cg[q] = col(r, jj);
+                            std::memcpy(vg + q * vsz, vals.data() + (size_t)elem(r, jj) * vsz,
+                                        vsz);
+                        } else {
+                            cg[q] = r;       // zero value on the row's own column
+                        }
+                        if (l < nB)
+                            cl[(size_t)jj * nB + l] = (uint16_t)(jj < w ? loc[col(r, jj)] : l);
                     }
                 }
-            }
-        };
-        size_t ends[4];
-        ends[0] = rows.size();
-        for (int k = 1; k <= depth; ++k) {   // E1 .. E_depth
-            ring(k == 1 ? base : ends[k - 2], ends[k - 1]);
-            ends[k] = rows.size();
-        }
-        const int nA = (int)(ends[depth] - base), nB = (int)(ends[depth - 1] - base);
-        if (nA > 65535) return nullptr;     // LDS positions are 16 bits (never fits anyway)
-        int wA = 0;
-        for (size_t i = base; i < rows.size(); ++i) wA = std::max(wA, width(rows[i]));
-        BlockMeta m;
-        m.offA = (int64_t)base;
-        m.offL = (int64_t)colL.size();
-        m.offG = (int64_t)colG.size();
-        for (int k = 0; k < 4; ++k) m.n[k] = (int)(ends[std::min(k, depth)] - base);
-        m.nB = nB;
-        m.nA = nA;
-        m.w = wA;
-        m.r0 = r0;
-        meta[b] = m;
-        colL.resize(colL.size() + (size_t)wA * nB);
-        colG.resize(colG.size() + (size_t)wA * nA);
-        valG.resize(valG.size() + (size_t)wA * nA * vsz, 0);
-        uint16_t* cl = colL.data() + m.offL;
-        int* cg = colG.data() + m.offG;
-        unsigned char* vg = valG.data() + (size_t)m.offG * vsz;
-        for (int l = 0; l < nA; ++l) {
-            const int r = rows[base + l];
-            const int w = width(r);
-            for (int j = 0; j < wA; ++j) {
-                const size_t q = (size_t)j * nA + l;
-                if (j < w) {
-                    cg[q] = col(r, j);
-                    std::memcpy(vg + q * vsz, vals.data() + (size_t)elem(r, j) * vsz, vsz);
-                } else {
-                    cg[q] = r;       // zero value on the row's own column
-                }
-                if (l < nB) cl[(size_t)j * nB + l] = (uint16_t)(j < w ? loc[col(r, j)] : l);
+                nA_max = std::max(nA_max, nA);
+                nB_max = std::max(nB_max, nB);
             }
         }
-        nA_max = std::max(nA_max, nA);
-        nB_max = std::max(nB_max, nB);
+        seg_block.push_back((int)meta.size());
+        if ((int64_t)(nA_max + nB_max) * row_bytes <= h->max_lds) break;
+        if (attempt == 3) return nullptr;
+        budget = budget * 7 / 8;
     }
+    const int nb = (int)meta.size();
     // every LDS index in range: a row of R + E1 + .. + Ek reads positions
     // in R + E1 + .. + E(k+1), which a pass has computed one phase earlier
     for (int b = 0; b < nb; ++b) {
@@ -182,7 +228,6 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
             TBG_REQUIRE(c >= 0 && c < n, "internal: blocking table out of range");
         }
     }
-    if ((int64_t)(nA_max + nB_max) * row_bytes > h->max_lds) return nullptr;
     auto B = std::make_shared<Blocking>();
     B->row_bytes = row_bytes;
     B->depth = depth;

