@@ -558,6 +558,22 @@ __global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
                       finish(vidx, o);
                   });
         } else {
+            // The owned rows' output after three terms waits for phase 4 in
+            // registers: each lane meets its owned rows in the same order in
+            // both phases, so its first HELD partials go through a queue
+            // with fixed register indices (part[HELD - 1] leaves first),
+            // any further ones through acc.  One state per lane: 1024 lanes
+            // per tile, about one owned row each, and a longer queue would
+            // cost the second workgroup per CU (more than 64 registers).
+            constexpr int HELD = V == 1 ? 1 : (V * sizeof(R) <= 8 ? 4 : 2);
+            VT part[HELD];
+#pragma unroll
+            for (int q = 0; q < HELD; ++q) part[q] = vzero<R, V>();
+            auto shift = [&]() {
+#pragma unroll
+                for (int q = HELD - 1; q > 0; --q) part[q] = part[q - 1];
+            };
+            int held = 0;
             // phase 3: v_{m+2} on R + E1, over v_m in A
             phase(owned_rows, m.n[1], idxL, gatherB,
                   [&](int l, int gr) { return l < nR ? acc0(gr) : vzero<R, V>(); },
@@ -572,14 +588,30 @@ __global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
                           axpy<R, V>(o, cf[2], vm);
                           axpy<R, V>(o, cf[3], lds(ldsB, l));
                           axpy<R, V>(o, cf[4], v2);
-                          vst<R, V>(a.acc, vidx, o);
+                          if (held < HELD) {
+                              shift();
+                              part[0] = o;
+                              ++held;
+                          } else {
+                              vst<R, V>(a.acc, vidx, o);
+                          }
                       }
                   });
+            for (int q = held; q < HELD; ++q) shift();   // first partial to part[HELD - 1]
             __syncthreads();
-            // phase 4 (last): v_{m+3} on the owned rows; the lane that
-            // stored the row's partial output reads it back
+            // phase 4 (last): v_{m+3} on the owned rows, the output from
+            // the queue (or read back from acc, stored by this lane)
+            int used = 0;
             phase(owned_rows, nR, idxL, gatherA,
-                  [&](int, int gr) { return vld<R, V>(a.acc, (int64_t)gr * K + k0); },
+                  [&](int, int gr) {
+                      if (used < held) {
+                          const VT o = part[HELD - 1];
+                          shift();
+                          ++used;
+                          return o;
+                      }
+                      return vld<R, V>(a.acc, (int64_t)gr * K + k0);
+                  },
                   [&](int l, int gr, const VT& s, const VT& o0) {
                       const int64_t vidx = (int64_t)gr * K + k0;
                       const VT v3 = recur<R, V, true>(ar, ai, beta, s, lds(ldsB, l));

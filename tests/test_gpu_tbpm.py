@@ -414,6 +414,38 @@ def test_block_tiles_split_to_fit(gpu, terms):
         h.close()
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_four_term_pass_long_tiles(gpu, prec):
+    """12 nm tiles with one state (8 or 16 bytes a row): a lane owns more
+    rows than the four output partials it holds in registers between the
+    third and fourth phase of a pass (complex64: ~5,500 owned rows for
+    1024 lanes), so the rest go through memory; the states stay
+    bit-identical to one term per pass."""
+    s = _blocked_sample(128)
+    ip, ix, dt = _csr(s)
+    order, tiles = s.tile_order(12.0, return_tiles=True)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, 1, seed=43)
+    h = tb._lib.Handle(0)
+    try:
+        h.set_blocking(True, terms=4)
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec, handle=h, order=order,
+                             tiles=tiles)
+        rows, nt = H.blocking_launch(1)
+        assert nt == 1024 and rows > (4500 if prec == "fp32" else 2000)
+        a = tbpm.corr_dos(H, bes, psi, 10)
+        sa = tbpm.cheb_step(H, bes, psi, True)
+        h.set_blocking(False)
+        b = tbpm.corr_dos(H, bes, psi, 10)
+        np.testing.assert_array_equal(sa, tbpm.cheb_step(H, bes, psi, True))
+        _same_correlations(a, b, exact=False)
+        ref = oracle.orc_tbpm_dos(ip, ix, dt, bes, psi, 10)
+        assert np.max(np.abs(a - ref)) < (1e-12 if prec == "fp64" else 2e-5)
+        H.close()
+    finally:
+        h.close()
+
+
 def test_order_must_be_a_permutation(gpu):
     s = graphene_sample(4)
     ip, ix, dt = _csr(s)
