@@ -14,8 +14,12 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libtbplas_amd.so")
-SOURCES = ["tbg_prop.hip", "tbg_block.hip", "tbg_csr.hip", "tbg_cond.hip", "tbg_api.hip",
-           "tbg_mlp.hip", "tbg_hall.hip", "tbg_super.hip"]
+# the longest compiles first: the one-term kernels, then the tile-pass instances
+SOURCES = ["tbg_prop.hip", "tbg_pass_f32_t4_real.hip", "tbg_pass_f32_t4_cplx.hip",
+           "tbg_pass_f64_t4_real.hip", "tbg_pass_f64_t4_cplx.hip", "tbg_pass_t2.hip",
+           "tbg_pass_f32_t3_real.hip", "tbg_pass_f32_t3_cplx.hip", "tbg_pass_f64_t3_real.hip",
+           "tbg_pass_f64_t3_cplx.hip", "tbg_super.hip", "tbg_api.hip", "tbg_cond.hip",
+           "tbg_block.hip", "tbg_mlp.hip", "tbg_csr.hip", "tbg_hall.hip"]
 ARCH = os.environ.get("TBG_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -128,4 +128,12 @@ void launch_pass(int mode, int T, bool real_vals, const PassArgs& a, const Block
 // once, before any launch).
 void prepare_term3(int max_lds);
 
+// The instances of real type R with T terms per pass and real (RV) or
+// complex values (tbg_block_pass.h, instantiated in tbg_pass_*.hip):
+// launch, and the LDS permission.
+template <typename R, int T, bool RV>
+void launch_pass_rt(int mode, const PassArgs& a, const Blocking& b, hipStream_t s);
+template <typename R, int T, bool RV>
+void allow_lds_rt(int bytes);
+
 }  // namespace tbg

@@ -1,0 +1,10 @@
+// Tile-pass kernel instances: three terms per pass, double, complex values.
+#include "tbg_block_pass.h"
+
+namespace tbg {
+
+template void launch_pass_rt<double, 3, false>(int, const PassArgs&, const Blocking&,
+                                                hipStream_t);
+template void allow_lds_rt<double, 3, false>(int);
+
+}  // namespace tbg
