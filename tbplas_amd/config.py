@@ -22,8 +22,9 @@ look at:
                        Chebyshev terms per pass; same results, 0 keeps the
                        sample's order, 'auto' sizes the tiles for the
                        states per row (sample.auto_tile_nm)   default 'auto'
-    gpu['cond_tile_nm'] the same for AC and DC runs (more vectors per row,
-                       so smaller tiles fit the LDS)          default 1.5
+    gpu['cond_tile_nm'] the same for AC and dynamic-polarisation runs (more
+                       vectors per row, so smaller tiles)     default 1.5
+    gpu['dc_tile_nm']  the same for DC runs                   default 2.0
 """
 import pickle
 
@@ -69,7 +70,8 @@ class Config:
                     'state_kind': 'phase',
                     'fp32_norm_thr': 1.0e-4,
                     'tile_nm': 'auto',
-                    'cond_tile_nm': 1.5}
+                    'cond_tile_nm': 1.5,
+                    'dc_tile_nm': 2.0}
         self._legal_params = dict()
         self.set_legal_params()
 

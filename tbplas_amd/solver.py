@@ -411,7 +411,7 @@ class Solver:
         indptr, indices, hop, dx, dy = self._sample.build_ham_dxy_device()
         num_sample = self._dist_sample()
         self._echo_time_step_fs(time_step)
-        order, tiles = self._row_order(gpu, "cond_tile_nm")
+        order, tiles = self._row_order(gpu, "dc_tile_nm")
         H = Hamiltonian(self._sample.num_orb, indptr, indices, hop, dx, dy,
                         cur_scale=self._sample.rescale, precision=gpu["precision"],
                         handle=h, order=order, tiles=tiles)
