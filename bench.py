@@ -187,15 +187,19 @@ def layout_name(args, shape):
     return base + passes_name(shape)
 
 
+def passes_terms(shape):
+    """Terms per pass of a tuner shape (1: one term per pass)."""
+    return max(1, int(shape[2])) if len(shape) > 2 else 1
+
+
 def passes_name(shape):
-    t = int(shape[2]) if len(shape) > 2 else 1
     return {1: ", one term per pass", 3: ", three terms per pass",
-            4: ", four terms per pass"}[t]
+            4: ", four terms per pass"}.get(passes_terms(shape), ", several terms per pass")
 
 
 def shape_json(shape):
     return {"blocks_per_cu": shape[0], "lds_stage": bool(shape[1]),
-            "terms_per_pass": int(shape[2]) if len(shape) > 2 else 1}
+            "terms_per_pass": passes_terms(shape)}
 
 
 def term_bytes(n, nnz, K, s, real=False):
@@ -512,7 +516,7 @@ def main():
                      "term_avg_us": round(t_step / nterm * 1e6, 2)},
     }
     layout = layout_name(args, shape32)
-    if shape32[2] > 1:
+    if passes_terms(shape32) > 1:
         out["roofline"]["note"] = (
             "achieved = the one-term-per-pass algorithmic bytes per term over the "
             "measured time per term; several terms per pass read the state vectors "
