@@ -66,22 +66,31 @@ def graphene_hops(L1, L2, t=GRAPHENE_T, pbc=(True, True), vacancies=None,
     return pos[:n].copy(), hi[:m].copy(), hj[:m].copy(), hv[:m].copy(), dr[:m].copy()
 
 
-def auto_tile_nm(vectors, precision="fp32"):
+GRAPHENE_ORB_DENSITY = 38.2   # orbitals per nm^2 (2 per 0.0524 nm^2 cell)
+
+
+def auto_tile_nm(vectors, precision="fp32", density=GRAPHENE_ORB_DENSITY):
     """Edge (nm) of the square tiles of ``Sample.tile_order`` for runs of
-    ``vectors`` states per row: 5.0 nm at 64 bytes of states per row (8
-    complex64 states), scaled by sqrt(64 / row bytes) so that a tile with
-    its three halo rings about fills the LDS of one workgroup, within
-    [1.5, 5.0] nm.  Tiles too large for the LDS are split by the library.
-    Measured on 8.4M-orbital graphene (step·vectors/s, four terms per
-    pass): complex64 K=8 1606 at 5.0 nm, 1458 at 3.5 nm; K=16 1576 at 3.5
-    nm, 1438 at 5.0 nm."""
+    ``vectors`` states per row on a sample of ``density`` orbitals per nm^2:
+    the largest tile whose rows with four terms per pass (LDS areas A and
+    B: twice the tile, plus its first ring twice, its second twice and its
+    third once) fit 150 KB of LDS, with 2 rho a^2 tile rows and a ring of
+    2.96 sqrt(rho) a rows (nearest-neighbour lattices; fitted on graphene,
+    17.6 a + 2 per ring), within [1.5, 5.0] nm.  A tile any larger is split
+    by the library into strips, which costs more than it saves.  Measured on
+    8.4M-orbital graphene (step·vectors/s): complex64 K=8 1607 at 5.0 nm,
+    1458 at 3.5 nm; K=16 1643 at 3.0 nm, 1436 at 3.54 nm (split)."""
     p2, p3 = 1, 3            # vector slots of the device layout
     while p2 < vectors:
         p2 <<= 1
     while p3 < vectors:
         p3 <<= 1
     row = min(p2, p3) * (8 if precision == "fp32" else 16)
-    return float(min(5.0, max(1.5, 5.0 * math.sqrt(64.0 / row))))
+    rows = 150.0 * 1024 / row                     # LDS rows of one workgroup
+    a2 = 2.0 * density                            # tile rows per nm^2, A and B
+    a1 = 5.0 * 2.96 * math.sqrt(density)          # ring rows per nm of edge
+    edge = (-a1 + math.sqrt(a1 * a1 + 4.0 * a2 * rows)) / (2.0 * a2)
+    return float(min(5.0, max(1.5, edge)))
 
 
 def random_vacancies(n_orb, fraction, seed):

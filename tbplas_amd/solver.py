@@ -138,13 +138,23 @@ class Solver:
 
     def _row_order(self, gpu, key, vectors=1):
         """(row order, tile offsets) of square tiles of gpu[key] nm ('auto':
-        sized for `vectors` states per row, ``auto_tile_nm``) when the
-        sample has orbital positions, else (None, None)."""
+        sized for `vectors` states per row and the sample's orbitals per nm^2
+        in the x-y plane, ``auto_tile_nm``) when the sample has orbital
+        positions, else (None, None)."""
+        pos = self._sample.orb_pos
+        if pos is None:
+            return None, None
         tile = gpu.get(key, 0)
         if tile == "auto":
-            tile = auto_tile_nm(vectors, gpu["precision"])
+            # orbitals per nm^2: the median over occupied 2 nm cells (edge
+            # cells and a sheared cell's bounding box would understate it)
+            xy = np.asarray(pos)[:, :2]
+            cell = np.floor((xy - xy.min(axis=0)) / 2.0).astype(np.int64)
+            key = cell[:, 1] * (int(cell[:, 0].max()) + 1) + cell[:, 0]
+            density = float(np.median(np.unique(key, return_counts=True)[1])) / 4.0
+            tile = auto_tile_nm(vectors, gpu["precision"], max(density, 1.0))
         tile = float(tile or 0)
-        if tile > 0 and self._sample.orb_pos is not None:
+        if tile > 0:
             return self._sample.tile_order(tile, return_tiles=True)
         return None, None
 

@@ -59,14 +59,25 @@ class _S:
 
 def test_auto_tile_size():
     """Tiles sized for the bytes of states per row (vector slots: the
-    smaller of the next power of two and the next 3 * 2^j): 5.0 nm at 64
-    bytes, sqrt-scaled, within [1.5, 5.0]."""
-    assert tb.auto_tile_nm(8, "fp32") == 5.0            # 64 B
+    smaller of the next power of two and the next 3 * 2^j): the edge a
+    whose LDS rows 2 rho a^2 + 5 * 2.96 sqrt(rho) a fill 150 KB, within
+    [1.5, 5.0] nm; on graphene 5.0 nm at 64 bytes a row, 3.41 at 128."""
+    rho = 38.2
+
+    def rows(a):
+        return 2 * rho * a * a + 5 * 2.96 * math.sqrt(rho) * a
+
+    for vec, prec, row in ((16, "fp32", 128), (10, "fp32", 96), (24, "fp32", 192),
+                           (16, "fp64", 256)):
+        a = tb.auto_tile_nm(vec, prec)
+        assert 1.5 < a < 5.0
+        assert math.isclose(rows(a) * row, 150 * 1024, rel_tol=1e-9)
+    assert abs(tb.auto_tile_nm(16, "fp32") - 3.41) < 0.01
+    assert tb.auto_tile_nm(8, "fp32") == 5.0            # 64 B: 5.02, capped
     assert tb.auto_tile_nm(1, "fp64") == 5.0            # 16 B, capped
-    assert math.isclose(tb.auto_tile_nm(16, "fp32"), 5.0 / math.sqrt(2))
-    assert math.isclose(tb.auto_tile_nm(10, "fp32"), 5.0 * math.sqrt(64 / 96))  # 12 slots
-    assert math.isclose(tb.auto_tile_nm(24, "fp32"), 5.0 * math.sqrt(64 / 192))
     assert tb.auto_tile_nm(64, "fp64") == 1.5           # 1024 B, floor
+    # a denser lattice takes smaller tiles
+    assert tb.auto_tile_nm(16, "fp32", density=4 * rho) < tb.auto_tile_nm(16, "fp32")
 
 
 def test_analyzer_dos_matches_loop_restatement():
