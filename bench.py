@@ -459,8 +459,9 @@ def main():
          "latency": latency_workload}[args.workload](args)
         return 0
     if args.gpus != 1:
-        print(f"bench.py drives one GPU; --gpus {args.gpus} is not supported by this build "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print(f"bench.py drives one GPU; --gpus {args.gpus} is not supported "
+              "(states are independent: several GPUs would each propagate a share "
+              "of them and sum the correlation partials once per run).", file=sys.stderr)
         return 2
     cpu = None if (args.no_cpu or args.only_fp32) else cpu_baseline(args)
 

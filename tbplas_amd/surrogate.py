@@ -1,4 +1,4 @@
-"""Learned DOS surrogate with TBPM as fallback and judge.
+"""Learned DOS surrogate with TBPM as fallback and arbiter.
 
 A dense network maps (vacancy fraction, Anderson disorder W, uniaxial
 strain) of a graphene sample to its DOS on a fixed 1024-point grid in
