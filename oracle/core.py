@@ -18,8 +18,9 @@ __all__ = [
     "ref_fermi_coef", "ref_tbpm_ldos", "ref_tbpm_psi_t", "ref_tbpm_eigenstates",
     "bessel_series", "build_ham_csr", "build_ham_dxy_loop",
     "build_ham_dxy_stable", "graphene_hops_loop", "graphene_hops_vec", "get_rescale_loop",
-    "set_mag_field_loop", "kspace_graphene_corr", "dense_corr",
+    "set_mag_field_loop", "kspace_graphene_corr", "kspace_graphene_corr_multi", "dense_corr",
     "analyzer_calc_dos", "analyzer_calc_ac_cond", "analyzer_calc_dc_cond",
+    "analyzer_calc_diff_coeff",
     "window_hanning", "window_exp",
 ]
 
@@ -995,17 +996,31 @@ def graphene_hops_vec(L1, L2, t=-2.7, pbc=(True, True), a=0.246):
 
 
 def get_rescale_loop(orb_eng, hop_i, hop_j, hop_v):
-    """get_rescale (sample.pyx:75-114): max row sum of |H| (on-site signed)."""
+    """get_rescale (sample.pyx:75-114): max over rows of the signed on-site
+    energy plus the moduli of the row's hopping terms, each modulus formed
+    as sqrt(re**2 + im**2) (sample.pyx:110), not hypot.
+
+    Pinned by the closed form max_i(orb_eng_i + sum_{j != i} |H_ij|) on a
+    dense H with signed on-site energies and complex hoppings, and by a row
+    whose negative on-site energy must not count as |orb_eng|."""
     s = np.array(orb_eng, dtype=np.float64).copy()
     for ih in range(len(hop_i)):
-        v = abs(hop_v[ih])
+        z = complex(hop_v[ih])
+        v = math.sqrt(z.real ** 2 + z.imag ** 2)
         s[hop_i[ih]] += v
         s[hop_j[ih]] += v
     return float(np.max(s))
 
 
 def set_mag_field_loop(hop_i, hop_j, hop_v, dr, orb_pos, intensity, gauge=0):
-    """set_mag_field (sample.pyx:18-72), in place on a copy of hop_v."""
+    """set_mag_field (sample.pyx:18-72), in place on a copy of hop_v.
+
+    Pinned by the flux through every closed loop: around each hexagon of an
+    open graphene flake, traversed counter-clockwise, the product of the
+    amplitudes H[i_{k+1}, i_k] carries the phase 2 pi B A_hex / Phi_0
+    (Phi_0 = h/e = 4135.666734 T nm^2) in each of the three gauges, and the
+    spectrum of the flake is the same in all three (a gauge change is a
+    diagonal unitary)."""
     factor = math.pi / 4135.666734
     out = np.array(hop_v, dtype=np.complex128).copy()
     for ih in range(len(hop_i)):
@@ -1056,6 +1071,35 @@ def kspace_graphene_corr(L1, L2, hop, psi0, tau, steps, fwd=True):
         th = sgn * tau * k * e
         out.append(np.sum(wu * np.exp(1j * th)) + np.sum(wd * np.exp(-1j * th)))
     return np.array(out, dtype=np.complex128)
+
+
+def kspace_graphene_corr_multi(L1, L2, hop, psi0s, tau, steps, fwd=True):
+    """kspace_graphene_corr for several states (rows of ``psi0s``) at once:
+    the band energies and the phase factors exp(-+i E tau k) are shared, so
+    each sampled step costs one complex exponential over the k grid and one
+    (states x k) product.  Returns (states, len(steps)) complex128."""
+    psi0s = np.asarray(psi0s).reshape(-1, L1, L2, 2)
+    S = psi0s.shape[0]
+    k1 = 2 * np.pi * np.fft.fftfreq(L1)[:, None]
+    k2 = 2 * np.pi * np.fft.fftfreq(L2)[None, :]
+    f = hop + np.conj(hop) * (np.exp(-1j * k1) + np.exp(-1j * k2))
+    e = np.abs(f)
+    ph = np.where(e > 0, f / np.where(e > 0, e, 1), 1.0)
+    N = L1 * L2
+    wu = np.empty((S, N))
+    wd = np.empty((S, N))
+    for s in range(S):
+        A = np.fft.fft2(psi0s[s, :, :, 0])
+        B = np.fft.fft2(psi0s[s, :, :, 1])
+        wu[s] = (np.abs(A + ph * B) ** 2).ravel() / (2 * N)
+        wd[s] = (np.abs(A - ph * B) ** 2).ravel() / (2 * N)
+    e = e.ravel()
+    sgn = -1.0 if fwd else 1.0
+    out = np.empty((S, len(list(steps))), dtype=np.complex128)
+    for c, k in enumerate(steps):
+        z = np.exp(1j * (sgn * tau * k) * e)
+        out[:, c] = wu @ z + wd @ np.conj(z)
+    return out
 
 
 def dense_corr(indptr, indices, data, psi0, tau, steps):
@@ -1167,3 +1211,31 @@ def analyzer_calc_dc_cond(corr_dos, corr_dc, tnr, en_range, energy_limits,
     if correct_spin:
         dc = 2. * dc
     return energies, dc
+
+
+def analyzer_calc_diff_coeff(corr_dc, tnr, en_range, energy_limits, window_dc=window_exp):
+    """Analyzer.calc_diff_coeff (analysis.py:460-511), loop for loop, with
+    numpy's trapezoid for the removed scipy.integrate.trapz.
+
+    Pinned by closed forms: a correlation that is constant after the phase
+    shift grows linearly in time (rectangle window), and under a geometric
+    window r^i the running trapezoid sum is a geometric series."""
+    t_step = 2 * math.pi / en_range
+    energies = np.array([0.5 * i * en_range / tnr - en_range / 2.
+                         for i in range(tnr * 2)], dtype=float)
+    qe = np.where((energies >= energy_limits[0]) & (energies <= energy_limits[1]))[0]
+    energies = energies[qe]
+    time = np.linspace(0, tnr - 1, tnr) * t_step
+    diff = np.zeros((2, len(qe), tnr))
+    for i in range(2):
+        for j in range(len(qe)):
+            en = energies.item(j)
+            temp = np.zeros(tnr)
+            for k in range(tnr):
+                w = window_dc(k + 1, tnr)
+                phi = k * t_step * en
+                c_exp = math.cos(phi) - 1j * math.sin(phi)
+                temp[k] = w * (c_exp * corr_dc.item(i, j, k)).real
+            for k2 in range(tnr):
+                diff[i, j, k2] = np.trapezoid(temp[:k2 + 1], time[:k2 + 1])
+    return time, diff

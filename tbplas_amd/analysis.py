@@ -178,3 +178,30 @@ class Analyzer:
         if self._config.generic['correct_spin']:
             dc = 2. * dc
         return energies, dc
+
+    def calc_diff_coeff(self, corr_dc, window_dc=window_exp):
+        """analysis.py:460-511: (time (T,), diff_coeff (2, n_E, T)) -- the
+        running trapezoidal integral over time of the windowed, phase-shifted
+        DC correlation, in nm^2/(h_bar/eV), at the quasi-eigenstate energies
+        inside DC_conductivity['energy_limits'].  Real float64, as the
+        reference fills it; the reference's scipy.integrate.trapz (removed
+        from scipy) becomes a cumulative sum of the same trapezoids."""
+        tnr = self._config.generic['nr_time_steps']
+        en_range = self._sample.energy_range
+        t_step = 2 * math.pi / en_range
+        lim = self._config.DC_conductivity['energy_limits']
+        energies = np.array([0.5 * i * en_range / tnr - en_range / 2.
+                             for i in range(tnr * 2)], dtype=float)
+        qe = np.where((energies >= lim[0]) & (energies <= lim[1]))[0]
+        energies = energies[qe]
+        time = np.linspace(0, tnr - 1, tnr) * t_step
+        k = np.arange(tnr)
+        w = window_dc(k + 1, tnr)
+        phi = np.outer(energies, k * t_step)                    # (E, k)
+        c_exp = np.cos(phi) - 1j * np.sin(phi)
+        corr_dc = np.asarray(corr_dc)[:, :len(qe), :tnr]
+        temp = w[None, None, :] * (c_exp[None] * corr_dc).real   # (2, E, k)
+        diff_coeff = np.zeros((2, len(qe), tnr))
+        trap = 0.5 * np.diff(time)[None, None, :] * (temp[:, :, 1:] + temp[:, :, :-1])
+        diff_coeff[:, :, 1:] = np.cumsum(trap, axis=2)
+        return time, diff_coeff

@@ -233,12 +233,18 @@ class DOSSurrogate:
 
     def calibrate(self, params, dos_ref, corr_short, en_ranges, tol=0.02):
         """Threshold on the gate distance: the largest value such that every
-        calibration point below it has L1 error <= tol."""
+        calibration point at or below it has L1 error <= tol.  When no
+        calibration point exceeds tol the threshold is the largest distance
+        seen (nothing beyond the checked range is accepted)."""
         pred = self.predict(params)
         err = l1_error(pred, dos_ref)
         d = np.array([self.gate_distance(p, c, e) for p, c, e in zip(pred, corr_short, en_ranges)])
         bad = d[err > tol]
-        self.gate_thr = float(np.min(bad)) * 0.999 if len(bad) else float(np.max(d)) * 1.5
+        if len(bad):
+            good = d[(err <= tol) & (d < np.min(bad))]
+            self.gate_thr = float(np.max(good)) if len(good) else 0.0
+        else:
+            self.gate_thr = float(np.max(d))
         return d, err
 
     def save(self, path):

@@ -159,3 +159,56 @@ def test_one_hip_runtime_when_library_loads_first():
                        timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     assert r.stdout.startswith("1 "), r.stdout
+
+
+def test_anderson_disorder_seeded_and_bounded():
+    """orb_eng += U[-W/2, W/2]: added to the existing on-site energies,
+    inside the bounds, reproducible from the seed, different for another
+    seed, and with the uniform distribution's mean 0 and variance W^2/12."""
+    W = 1.7
+    a = tb.make_graphene_sample(64)
+    a.orb_eng = np.full(a.num_orb, 0.25)
+    b = tb.make_graphene_sample(64)
+    b.orb_eng = np.full(b.num_orb, 0.25)
+    tb.add_anderson_disorder(a, W, 7)
+    tb.add_anderson_disorder(b, W, 7)
+    np.testing.assert_array_equal(a.orb_eng, b.orb_eng)
+    d = a.orb_eng - 0.25
+    assert d.min() >= -0.5 * W and d.max() <= 0.5 * W
+    assert abs(d.mean()) < 0.02 and abs(d.var() - W * W / 12) < 0.02
+    c = tb.make_graphene_sample(64)
+    tb.add_anderson_disorder(c, W, 8)
+    assert not np.allclose(c.orb_eng, d)
+    e = tb.make_graphene_sample(8)
+    tb.add_anderson_disorder(e, 0.0, 3)
+    assert np.all(e.orb_eng == 0.0)
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_uniaxial_strain_closed_form(axis):
+    """Stretching by (1 + eps) along one axis scales that component of every
+    position and bond vector; a bond at angle theta to the axis gets length
+    d0 sqrt((1 + eps)^2 cos^2 theta + sin^2 theta), and its hopping becomes
+    t exp(-3.37 (d - d0)/d0) with d0 = a / sqrt(3) (unstrained: unchanged)."""
+    eps = 0.04
+    s = tb.make_graphene_sample(6)
+    pos0, dr0, hv0 = s.orb_pos.copy(), s.dr.copy(), s.hop_v.copy()
+    tb.apply_uniaxial_strain(s, eps, axis=axis)
+    d0 = 0.246 / math.sqrt(3.0)
+    other = 1 - axis
+    np.testing.assert_allclose(s.orb_pos[:, axis], pos0[:, axis] * (1 + eps), rtol=1e-15)
+    np.testing.assert_array_equal(s.orb_pos[:, other], pos0[:, other])
+    np.testing.assert_allclose(s.dr[:, axis], dr0[:, axis] * (1 + eps), rtol=1e-15)
+    np.testing.assert_array_equal(s.dr[:, other], dr0[:, other])
+    cos_t = dr0[:, axis] / np.linalg.norm(dr0, axis=1)
+    d = d0 * np.sqrt((1 + eps) ** 2 * cos_t ** 2 + (1 - cos_t ** 2))
+    np.testing.assert_allclose(s.hop_v, -2.7 * np.exp(-3.37 * (d - d0) / d0), rtol=1e-13)
+    # graphene's bonds lie at 30, 150 and 90 degrees to x: stretched along y
+    # the 90-degree bond loses exp(-3.37 eps); along x it keeps -2.7
+    if axis == 1:
+        assert np.min(np.abs(s.hop_v)) == pytest.approx(2.7 * math.exp(-3.37 * eps), rel=1e-12)
+    else:
+        assert np.max(np.abs(s.hop_v)) == pytest.approx(2.7, rel=1e-12)
+    u = tb.make_graphene_sample(6)
+    tb.apply_uniaxial_strain(u, 0.0, axis=axis)
+    np.testing.assert_allclose(u.hop_v, hv0, rtol=1e-13)

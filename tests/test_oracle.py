@@ -102,6 +102,27 @@ def test_orc_dos_vs_kspace_exact():
     assert np.max(np.abs(c - k)) < 1e-12
 
 
+def test_kspace_multi_equals_single_and_dense():
+    """The several-state k-space oracle (used at 8.4M orbitals) equals the
+    one-state form and the dense eigendecomposition, forward and backward,
+    on a non-square lattice (L1 != L2 catches a transposed k grid)."""
+    L1, L2 = 6, 4
+    hi, hj, hv, dr, pos = oracle.graphene_hops_vec(L1, L2)
+    ip, ix, dt = oracle.build_ham_csr(np.zeros(len(pos)), hi, hj, hv * (1 / 9.0))
+    tau = math.pi
+    rng = np.random.default_rng(8)
+    psi = np.exp(2j * np.pi * rng.random((3, len(pos))))
+    psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+    steps = [0, 1, 5, 40]
+    for fwd in (True, False):
+        m = oracle.kspace_graphene_corr_multi(L1, L2, hv[0] / 9.0, psi, tau, steps, fwd)
+        for s in range(3):
+            one = oracle.kspace_graphene_corr(L1, L2, hv[0] / 9.0, psi[s], tau, steps, fwd)
+            assert np.max(np.abs(m[s] - one)) < 1e-13
+            d = oracle.dense_corr(ip, ix, dt, psi[s], tau if fwd else -tau, steps)
+            assert np.max(np.abs(m[s] - d)) < 1e-12
+
+
 def test_orc_dos_vs_dense_eig_disordered():
     L = 6
     orb, hi, hj, hv, dr, pos = _graphene(L)
