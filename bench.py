@@ -1,31 +1,51 @@
 #!/usr/bin/env python3
 """Benchmark of the TBPM hot path on one MI355X.
 
-Workload (the metric's configuration): pristine graphene 2048 x 2048 with
-periodic boundaries (8,388,608 orbitals, 25,165,824 non-zeros), rescaled by
-9 eV, 8 random-phase states propagated together for 2048 Chebyshev time
-steps (21 Bessel terms at precision 1e-15, i.e. 20 SpMM terms per step) with
-the DOS correlation C(t) formed every step.  FP32 storage and arithmetic,
-FP64 correlation sums.
+Headline workload (the metric's configuration): pristine graphene 2048 x
+2048 with periodic boundaries (8,388,608 orbitals, 25,165,824 non-zeros),
+rescaled by 9 eV, 8 random-phase states propagated together (21 Bessel
+terms at precision 1e-15, i.e. 20 SpMM terms per time step) with the DOS
+correlation C(t) formed every step; rows in 5.0 nm spatial tiles so the
+tuner can run several Chebyshev terms per pass.  complex64 storage and
+arithmetic, FP64 correlation sums.
 
 A "step" is one time step of all 8 states; ``value`` is step*vectors/s over
-the timed steps with the states resident in HBM.  Printed as one JSON line,
-with
-  * ``roofline``: algorithmic bytes per Chebyshev term
-      nnz*(s_val + 4) + 4*(n + 1) + K*5*n*s_vec   (s_vec = 8 B for complex64;
-      s_val = 4 B: graphene's hoppings are real and stored as float)
-    divided by the average time of one term over the timed region (HIP
-    events on the library's stream), against 8 TB/s;
-  * ``fp64_k1``: the plain FP64 single-state path on the same sample (the
-    speedup denominator) and ``speedup_vs_fp64_k1``;
-  * ``ct_err_fp32_vs_fp64``: max |C32(t) - C64(t)| over all timed steps for
-    state 0, both from the GPU on the identical initial state;
-  * ``cpu_baseline``: the reference's own Fortran tbpm_dos (compiled from its
-    sources into oracle/_ref) timed on the host cores on a bounded sample.
+the timed steps (--steps, after --warmup untimed ones) with the states
+resident in HBM, replayed from a hipGraph.  The one JSON line also holds
 
-``--gpus N`` with N > 1 exits: this build drives one GPU.
+  * ``roofline``: the dominant kernel (largest share of the step's device
+    time) from a segment of --profile-steps steps run right after the
+    timed ones with HIP events around every launch on the library's stream:
+    its average launch duration and the bytes one launch must move (for the
+    tile pass: the workgroup tables, the distinct rows of v_{m-1} its tiles
+    gather, v_{m-2} on the tiles and their halo rings, and the output read
+    plus written and the last two terms written on the owned rows, counted
+    from the tables by the library), ``frac`` = those bytes per second over
+    8 TB/s.  ``traffic`` is the counter-measured L2-to-HBM bytes per launch
+    (FETCH_SIZE and WRITE_SIZE, copy-calibrated) from profiles/traffic.json
+    when that file was measured on this build of the kernel sources, else
+    null.  ``one_term_equivalent`` restates the speed as one-term-per-pass
+    algorithmic bytes (five vector streams per state and H once per term)
+    per measured time per term;
+  * ``fp64_k8`` (the same 8 states in complex128) and ``fp64_k1`` (one
+    complex128 state: the speedup denominator);
+  * ``ct_err_fp32_vs_exact`` / ``ct_err_fp64_vs_exact``: max |C(t) - C_exact|
+    over the 8 states at sampled steps of a 2048-step run (the timed steps
+    are part of it), C_exact from k-space propagation of the same states
+    (exact for pristine periodic graphene), computed on the host;
+  * ``cpu_baseline``: the reference's own Fortran tbpm_dos (compiled from its
+    sources into oracle/_ref) on the host's physical cores;
+  * sub-records for the other workloads: ``ac`` (AC conductivity on the same
+    sample), ``dc`` (disordered field-threaded DC), ``batch`` (32 uneven
+    instances), ``latency`` (131,072 orbitals) and ``assembly`` (GPU CSR
+    assembly at 8.4M orbitals and 10^7 atoms), each with its dominant
+    kernel's roofline.
+
+``--workload X`` runs one of them alone (for profiling); ``--gpus N`` with
+N > 1 exits: this build drives one GPU.
 """
 import argparse
+import hashlib
 import json
 import math
 import os
@@ -38,19 +58,25 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "TBPM step·vectors/s + % HBM roofline, 8.4M-orb graphene; C(t) err vs FP64"
+# steps of the accuracy run and the steps compared with the exact propagation
+ACC_STEPS = 2048
+EXACT_PTS = sorted(set([0, 1, 2, 3, 4, 5, 17, 21, 100, 511, 1023, 1025, 2047, ACC_STEPS]
+                       + list(range(0, ACC_STEPS + 1, 128))))
+WORKLOADS = ("all", "dos", "ac", "dc", "batch", "latency", "assembly")
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=2048)
+    p.add_argument("--steps", type=int, default=512)
     p.add_argument("--warmup", type=int, default=16)
+    p.add_argument("--profile-steps", type=int, default=16,
+                   help="steps after the timed ones with per-launch events (roofline)")
+    p.add_argument("--accuracy-steps", type=int, default=ACC_STEPS,
+                   help="the DOS runs continue untimed to this many steps for the "
+                        "error against the exact propagation (0: no error check)")
     p.add_argument("--L", type=int, default=2048)
     p.add_argument("--vectors", type=int, default=8)
-    p.add_argument("--fp64-steps", type=int, default=None,
-                   help="timed steps of the FP64 K=1 run (default: --steps)")
-    p.add_argument("--only-fp32", action="store_true",
-                   help="skip the FP64 and CPU legs (for profiling)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=16)
     p.add_argument("--no-graph", action="store_true")
@@ -58,30 +84,24 @@ def parse():
                    help="use the default launch shape instead of the autotuner")
     p.add_argument("--copy-probe", action="store_true",
                    help="time a plain device copy of known size (counter calibration)")
-    p.add_argument("--workload", choices=("dos", "dc", "batch", "ac", "latency"),
-                   default="dos",
-                   help="dos: the metric's configuration (default); dc: disordered "
-                        "field-threaded DC conductivity; batch: 32 uneven instances; "
-                        "ac: AC conductivity on the 8.4M-orbital sample; latency: "
-                        "131,072-orbital single-state steps")
+    p.add_argument("--workload", choices=WORKLOADS, default="all")
     p.add_argument("--order", choices=("caller", "tiles"), default="tiles",
                    help="row order of the device layout: the sample's own, or square "
-                        "tiles of --tile nm (Sample.tile_order), which also lets the "
-                        "tuner choose three or four Chebyshev terms per pass")
+                        "tiles (Sample.tile_order), which lets the tuner choose three or "
+                        "four Chebyshev terms per pass")
     p.add_argument("--tile", type=float, default=5.0,
-                   help="dos and batch workloads: tile edge in nm of the spatial row order "
-                        "(5.0 nm fits 8 complex64 states a row with three halo rings in "
-                        "the LDS of one workgroup)")
+                   help="dos and batch: tile edge in nm (5.0 nm fits 8 complex64 states a "
+                        "row with three halo rings in the LDS of one workgroup)")
     p.add_argument("--cond-tile", type=float, default=None,
-                   help="dc and ac workloads: tile edge in nm of a spatial row order "
-                        "(0: the sample's own order; default 2.0 for dc, 1.5 for ac)")
+                   help="dc and ac: tile edge in nm (0: the sample's own order; default "
+                        "2.0 for dc, 1.5 for ac)")
     p.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
-    args = p.parse_args()
-    if args.cond_tile is None:
-        # measured best: DC 16 states 2.0 nm (4328 step·vectors/s, 3749 at
-        # 1.5 nm); AC 24 vectors 1.5 nm (535 state·steps/s, 507 at 2.0 nm)
-        args.cond_tile = 2.0 if args.workload == "dc" else 1.5
-    return args
+    p.add_argument("--exact-child", action="store_true", help=argparse.SUPPRESS)
+    return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def copy_probe(nbytes=1 << 30, reps=20):
@@ -104,13 +124,49 @@ def copy_probe(nbytes=1 << 30, reps=20):
     print(json.dumps({"copy_bytes": nbytes, "copy_ms": ms, "copy_rw_GBs": gbs}), flush=True)
 
 
-def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+# ---------------------------------------------------------------------------
+# Host side: the reference Fortran's time and the exact correlations, each in
+# a child process that loads neither torch nor the HIP runtime
+# ---------------------------------------------------------------------------
+def physical_cores():
+    """(logical CPUs, physical cores) this process may run on."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            with open(base + "physical_package_id") as f:
+                pkg = f.read().strip()
+            with open(base + "core_id") as f:
+                core = f.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("cpu", c))
+    return len(cpus), len(cores)
 
 
-# ---------------------------------------------------------------------------
-# CPU baseline: the reference Fortran, in a child process (no GPU there)
-# ---------------------------------------------------------------------------
+def cpu_model():
+    name = ""
+    try:
+        r = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20)
+        for line in r.stdout.splitlines():
+            if line.startswith("Model name:"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if not name:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        name = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return name or "unknown"
+
+
 def cpu_child(args):
     # numpy and the oracle only: no torch or HIP runtime (and their OpenMP
     # runtime) in the process that times the reference's OpenMP code
@@ -133,12 +189,30 @@ def cpu_child(args):
           flush=True)
 
 
+def exact_child(args):
+    """C(t) of the headline states by exact k-space propagation (pristine
+    periodic graphene is block diagonal in k)."""
+    import numpy as np
+    import oracle
+    from tbplas_amd import states
+    L, K = args.L, args.vectors
+    psi = states.random_states(2 * L * L, states.sample_seeds(1337, K))
+    hop = -2.7 * (1.0 / 9.0)
+    c = oracle.kspace_graphene_corr_multi(L, L, hop, psi, math.pi, EXACT_PTS)
+    print(json.dumps({"pts": EXACT_PTS, "re": np.real(c).tolist(), "im": np.imag(c).tolist()}),
+          flush=True)
+
+
 def cpu_baseline(args):
     import resource
     if not os.path.exists(os.path.join(HERE, "oracle", "_ref", "libtbpm_ref.so")):
         log("cpu_baseline: oracle/_ref/libtbpm_ref.so missing, skipped")
         return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    logical, phys = physical_cores()
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    # every physical core of the affinity mask, within the CPU share the
+    # machine assigns to the process (OMP_NUM_THREADS) when it sets one
+    threads = min(phys, share) if share > 0 else phys
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                OMP_PLACES="cores")
 
@@ -149,7 +223,8 @@ def cpu_baseline(args):
 
     cmd = [sys.executable, os.path.join(HERE, "bench.py"), "--cpu-child", "--L", str(args.L),
            "--cpu-steps", str(args.cpu_steps)]
-    log(f"cpu_baseline: reference tbpm_dos, {threads} threads, {args.cpu_steps} steps")
+    log(f"cpu_baseline: reference tbpm_dos, {threads} threads ({phys} physical cores, "
+        f"{logical} logical CPUs available), {args.cpu_steps} steps")
     r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900,
                        preexec_fn=unlimited_stack)
     if r.returncode != 0:
@@ -161,30 +236,66 @@ def cpu_baseline(args):
         log("cpu_baseline: no result line:", r.stdout[-1000:])
         return None
     res = json.loads(lines[-1])
-    cpu_model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
+    model = cpu_model()
     return {"value": 1.0 / res["per_step_s"], "unit": "step·vectors/s", "cores": threads,
             "kind": "reference",
             "sample": (f"reference Fortran tbpm_dos (oracle/_ref, amdflang -O3 -fopenmp) on "
                        f"graphene {args.L}x{args.L}, FP64, 1 state: time of T={1 + args.cpu_steps}"
-                       f" minus T=1 over {args.cpu_steps} steps; host {cpu_model}"),
+                       f" minus T=1 over {args.cpu_steps} steps; {threads} OpenMP threads on "
+                       f"{model}"),
+            "physical_cores_available": phys, "logical_cpus_available": logical,
+            "cpu_share": share or None, "cpu_model": model,
             "per_step_s": res["per_step_s"]}
 
 
+def start_exact(args):
+    env = dict(os.environ, OMP_NUM_THREADS="4")
+    cmd = [sys.executable, os.path.join(HERE, "bench.py"), "--exact-child", "--L", str(args.L),
+           "--vectors", str(args.vectors)]
+    return subprocess.Popen(cmd, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                            text=True)
+
+
+def finish_exact(proc):
+    import numpy as np
+    out, err = proc.communicate(timeout=900)
+    lines = [ln for ln in out.splitlines() if ln.startswith('{"pts"')]
+    if proc.returncode != 0 or not lines:
+        log("exact k-space correlations failed:", err[-2000:])
+        return None
+    d = json.loads(lines[-1])
+    return d["pts"], np.array(d["re"]) + 1j * np.array(d["im"])
+
+
 # ---------------------------------------------------------------------------
-# GPU legs
+# GPU helpers
 # ---------------------------------------------------------------------------
-def layout_name(args, shape):
-    """Row order and pass mode of a run (matches the traffic profile)."""
-    base = "caller order" if args.order == "caller" else f"tiles of {args.tile} nm"
-    return base + passes_name(shape)
+def src_hash():
+    """Hash of the kernel sources and the header: a counter profile applies
+    to the build it was measured on."""
+    h = hashlib.sha256()
+    csrc = os.path.join(HERE, "tbplas_amd", "csrc")
+    for name in sorted(os.listdir(csrc)):
+        with open(os.path.join(csrc, name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    with open(os.path.join(HERE, "include", "tbplas_amd.h"), "rb") as f:
+        h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def counter_traffic(workload, kernel):
+    """Counter-measured bytes per launch of `kernel` for `workload` from
+    profiles/traffic.json, when measured on these kernel sources."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = d.get(workload, {}).get(kernel)
+    if not e or e.get("src_hash") != src_hash():
+        return None
+    return e
 
 
 def passes_terms(shape):
@@ -203,78 +314,230 @@ def shape_json(shape):
 
 
 def term_bytes(n, nnz, K, s, real=False):
-    """Algorithmic bytes of one Chebyshev term: H (values + int32 columns +
-    row pointers) and five vector streams per state.  s = bytes of one
-    complex vector element; H values take s bytes, or s/2 when the
-    Hamiltonian is stored with real values (no imaginary parts)."""
+    """One-term-per-pass algorithmic bytes of one Chebyshev term: H (values +
+    int32 columns + row pointers) and five vector streams per state.  s =
+    bytes of one complex vector element; H values take s bytes, or s/2 when
+    the Hamiltonian is stored with real values."""
     vs = s // 2 if real else s
     return nnz * (vs + 4) + 4 * (n + 1) + K * 5 * n * s
 
 
-def gpu_run(H, bes, psi, warmup, steps):
-    from tbplas_amd import tbpm
-    run = tbpm.DosRun(H, bes, psi, warmup + steps)
-    if warmup:
-        run.advance(warmup)
-    ms = run.advance(steps)
+def term_kernel_bytes(n, nnz, K, s, real=False):
+    """What the one-term kernel must move per term: H, the gathered v_{m-1},
+    v_{m-2} read and v_m written, and the step output's read-modify-write
+    every third term (two streams per three terms)."""
+    vs = s // 2 if real else s
+    return nnz * (vs + 4) + 4 * (n + 1) + K * n * s * (3.0 + 2.0 / 3.0)
+
+
+def kernel_label(kind, prec, K, real, threads):
+    """Template name of a timed launch kind (k_block_pass<R, V, RV, mode,
+    threads, terms>)."""
+    if kind.startswith("pass"):
+        mode, terms = kind[4:].split("x")
+        slots = 1
+        while slots < K:
+            slots <<= 1
+        p3 = 3
+        while p3 < K:
+            p3 <<= 1
+        slots = min(slots, p3)
+        V = 3 if slots % 3 == 0 else (slots if slots < 2 else 2)
+        R = "float" if prec == "fp32" else "double"
+        return (f"k_block_pass<{R}, {V}, {str(bool(real)).lower()}, {mode}, {threads}, "
+                f"{terms}>")
+    return {"term": "k_term", "ac_corr": "k_accorr", "dc_qe": "k_qe",
+            "dc_dot": "k_dcdot"}.get(kind, kind)
+
+
+def roofline(kt, bytes_of, label_of, workload, extra=None):
+    """Roofline record of the dominant timed kernel kind: bytes_of(kind) =
+    bytes one launch must move (None: not modelled), label_of(kind) =
+    kernel name."""
+    tot = sum(v[0] for v in kt.values())
+    kind = max(kt, key=lambda k: kt[k][0])
+    ms, cnt = kt[kind]
+    avg_s = ms / cnt / 1e3
+    b = bytes_of(kind)
+    name = label_of(kind)
+    rec = {"bound": "hbm", "kernel": name, "launches": cnt,
+           "launch_avg_us": round(avg_s * 1e6, 2),
+           "share_of_device_time": round(ms / tot, 4) if tot > 0 else None,
+           "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+           "traffic": None, "bytes_per_launch": None}
+    if b is not None:
+        ach = b / avg_s / 1e9
+        rec.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "bytes_per_launch": int(b)})
+    tr = counter_traffic(workload, name)
+    if tr is not None:
+        rec["traffic"] = tr["bytes_per_launch"]
+        rate = tr["bytes_per_launch"] / avg_s / 1e9
+        rec["traffic_GBs"] = round(rate, 1)
+        rec["traffic_frac"] = round(rate / HBM_PEAK_GBS, 4)
+        rec["traffic_source"] = tr.get("summary")
+    else:
+        rec["traffic_note"] = ("no counter profile of this build of the kernel sources in "
+                               "profiles/traffic.json")
+    rec["launches_by_kind"] = {label_of(k): {"launches": v[1],
+                                             "avg_us": round(v[0] / v[1] * 1e3, 2)}
+                               for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])}
+    if extra:
+        rec.update(extra)
+    return rec
+
+
+def timed_run(run, h, args, total):
+    """warmup, timed steps (graph replay), profile steps (per-launch events),
+    then untimed to `total` steps; returns (timed ms, kernel times)."""
+    if args.warmup:
+        run.advance(args.warmup)
+    ms = run.advance(args.steps)
+    kt = {}
+    if args.profile_steps > 0:
+        h.set_kernel_timing(True)
+        try:
+            run.advance(args.profile_steps)
+            kt = h.kernel_times()
+        finally:
+            h.set_kernel_timing(False)
+    rest = total - run.done
+    if rest > 0:
+        run.advance(rest)
+    return ms, kt
+
+
+def pass_bytes_fn(sched, n, nnz, K, s, real):
+    """bytes_of for propagation launches: a full tile pass from the
+    library's table counts, the one-term kernel from its streams."""
+    full = None if sched is None else sched["pass_bytes"]
+
+    def f(kind):
+        if kind == "term":
+            return term_kernel_bytes(n, nnz, K, s, real)
+        if kind.startswith("pass0x"):
+            return full
+        return None
+    return f
+
+
+# ---------------------------------------------------------------------------
+# Headline: DOS on 8.4M-orbital graphene
+# ---------------------------------------------------------------------------
+def dos_leg(args, s, csr, ot, bes, psi, prec, K, h, total):
+    from tbplas_amd import tbpm, tune
+    ip, ix, dt = csr
+    n, nnz = s.num_orb, int(ix.numel())
+    H = tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h, order=ot[0], tiles=ot[1])
+    real = H.info()["real_values"]
+    shape = tune.apply(H, K, bes, not args.no_tune)
+    threads = H.blocking_launch(K)[1]
+    run = tbpm.DosRun(H, bes, psi[:K], total)
+    ms, kt = timed_run(run, h, args, total)
     corr = run.result(chk_step=0)
+    sched = run.blocking_bytes()
     run.close()
-    return ms, corr
+    H.close()
+    sz = 8 if prec == "fp32" else 16
+    t_step = ms / 1e3 / args.steps
+    nterm = len(bes) - 1
+    label = lambda k: kernel_label(k, prec, K, real, threads)  # noqa: E731
+    rl = roofline(kt, pass_bytes_fn(sched, n, nnz, K, sz, real), label,
+                  f"dos_{prec}_k{K}") if kt else None
+    one = term_bytes(n, nnz, K, sz, real) / (t_step / nterm) / 1e9
+    if rl is not None:
+        rl["one_term_equivalent"] = {
+            "GBs": round(one, 1), "frac": round(one / HBM_PEAK_GBS, 4),
+            "bytes_per_term": term_bytes(n, nnz, K, sz, real),
+            "term_avg_us": round(t_step / nterm * 1e6, 2),
+            "note": "one-term-per-pass algorithmic bytes (five vector streams per state, "
+                    "H once) per measured time per term; several terms per pass move fewer "
+                    "bytes, so this may exceed the device's bandwidth"}
+        if sched is not None:
+            rl["schedule"] = sched
+    return {"value": K / t_step, "ms_per_step": t_step * 1e3, "shape": shape, "corr": corr,
+            "roofline": rl, "real": real}
 
 
-def cond_order(s, args):
-    """(row order, tile offsets) for the dc and ac workloads: square tiles
-    of --cond-tile nm, or the sample's own order."""
-    if args.cond_tile > 0:
-        return s.tile_order(args.cond_tile, return_tiles=True)
-    return None, None
-
-
-def dc_workload(args):
-    """Graphene 1024x1024, 1% vacancies, Anderson W = 1 eV (U[-0.5, 0.5]),
-    Peierls phases for B = 20 T (gauge 0), rescale 9; DC correlation at the 5
-    grid energies nearest -1, -0.5, 0, 0.5, 1 eV; 16 states, 1024 steps,
-    complex64 (FP64 quasi-eigenstates and correlations)."""
+def headline(args, cpu, exact_proc):
     import numpy as np
     import tbplas_amd as tb
-    from tbplas_amd import tbpm, _lib, tune
-    L, K, T = 1024, 16, args.steps if args.steps != 2048 else 1024
-    s = tb.make_graphene_sample(L, vacancies=tb.random_vacancies(2 * L * L, 0.01, 2024))
-    tb.add_anderson_disorder(s, 1.0, 2025)
-    s.set_magnetic_field(20.0, 0)
-    s.rescale_ham(9.0)
-    ip, ix, hop, dx, dy = s.build_ham_dxy_device()
-    n = s.num_orb
-    en = np.array([0.5 * i * s.energy_range / T - s.energy_range / 2. for i in range(2 * T)])
-    inds = np.array([int(np.argmin(np.abs(en - e))) for e in (-1.0, -0.5, 0.0, 0.5, 1.0)])
-    ts = 2 * math.pi / s.energy_range
-    bes = tbpm.bessel_series(ts * s.rescale, 1e-14)
-    h = _lib.default_handle(0)
-    order, tiles = cond_order(s, args)
-    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32", handle=h,
-                         order=order, tiles=tiles)
-    shape = tune.apply(H, 2 * K, bes, not args.no_tune)
-    psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
-    import torch
-    torch.cuda.synchronize()
+    from tbplas_amd import tbpm, _lib
+
     t0 = time.perf_counter()
-    d, c = tbpm.corr_dc(H, bes, psi, T, ts, en, inds, chk_thr=1e-4)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    # phase 1 propagates 2K states for T steps, phase 2 2K states for T-1 steps
-    value = 2 * K * (2 * T - 1) / dt
-    out = {"metric": "DC correlation step·vectors/s (disordered field-threaded graphene)",
-           "value": round(value, 2), "unit": "step·vectors/s", "n_gpus": 1,
-           "higher_is_better": True, "dtype": "fp32", "seconds": round(dt, 3),
-           "config": {"workload": f"graphene {L}x{L}, 1% vacancies, Anderson W=1 eV, B=20 T, "
-                                  f"DC at 5 energies, {K} states, {T} steps",
-                      "n_orbitals": n, "nnz": int(ix.numel()), "energies": en[inds].tolist(),
-                      "layout": ("caller order" if order is None
-                                 else f"tiles of {args.cond_tile:g} nm")
-                      + passes_name(shape),
-                      "launch_shape": shape_json(shape)},
-           "dc_corr_t1_mean": [float(np.mean(c[:, a, :, 0]).real) for a in range(2)]}
-    print(json.dumps(out), flush=True)
+    s = tb.make_graphene_sample(args.L)
+    s.rescale_ham(9.0)
+    csr = s.build_ham_csr_device()
+    n, nnz = s.num_orb, int(csr[1].numel())
+    bes = tbpm.bessel_series(2 * math.pi / s.energy_range * s.rescale, 1e-15)
+    K = args.vectors
+    psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
+    ot = (s.tile_order(args.tile, return_tiles=True) if args.order == "tiles" else (None, None))
+    log(f"setup {time.perf_counter() - t0:.1f}s: n={n} nnz={nnz} n_bes={len(bes)}")
+    h = _lib.default_handle(0)
+    h.set_graph(not args.no_graph)
+    total = max(args.accuracy_steps, args.warmup + args.steps + args.profile_steps)
+
+    r32 = dos_leg(args, s, csr, ot, bes, psi, "fp32", K, h, total)
+    log(f"fp32 K={K}: {r32['value']:.1f} step·vectors/s ({r32['ms_per_step']:.3f} ms/step)")
+    r64 = dos_leg(args, s, csr, ot, bes, psi, "fp64", K, h, total)
+    log(f"fp64 K={K}: {r64['value']:.1f} step·vectors/s ({r64['ms_per_step']:.3f} ms/step)")
+    a1 = argparse.Namespace(**vars(args))
+    a1.profile_steps = 0
+    r1 = dos_leg(a1, s, csr, ot, bes, psi, "fp64", 1, h,
+                 args.warmup + args.steps)
+    log(f"fp64 K=1: {r1['value']:.1f} steps/s")
+
+    layout = ("caller order" if args.order == "caller" else f"tiles of {args.tile:g} nm")
+    out = {
+        "metric": METRIC, "value": round(r32["value"], 3), "unit": "step·vectors/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(r32["ms_per_step"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded random-phase states; pristine graphene lattice)",
+        "config": {"workload": (f"graphene {args.L}x{args.L} PBC DOS correlation, {K} states "
+                                f"together, complex64 storage/FP64 correlation"),
+                   "n_orbitals": n, "nnz": nnz, "states": K, "bessel_terms": len(bes),
+                   "terms_per_step": len(bes) - 1, "hipgraph": not args.no_graph,
+                   "layout": layout + passes_name(r32["shape"]),
+                   "launch_shape": shape_json(r32["shape"]),
+                   "run_steps": total},
+        "roofline": r32["roofline"],
+        "fp64_k8": {"value": round(r64["value"], 3), "unit": "step·vectors/s",
+                    "ms_per_step": round(r64["ms_per_step"], 4),
+                    "layout": layout + passes_name(r64["shape"]),
+                    "launch_shape": shape_json(r64["shape"]), "roofline": r64["roofline"]},
+        "fp64_k1": {"value": round(r1["value"], 3), "unit": "step·vectors/s",
+                    "ms_per_step": round(r1["ms_per_step"], 4),
+                    "layout": layout + passes_name(r1["shape"]),
+                    "launch_shape": shape_json(r1["shape"])},
+        "speedup_vs_fp64_k1": round(r32["value"] / r1["value"], 3),
+    }
+    tmax = min(r32["corr"].shape[1], r64["corr"].shape[1])
+    out["ct_err_fp32_vs_fp64"] = float(np.max(np.abs(r32["corr"][:, :tmax] -
+                                                     r64["corr"][:, :tmax])))
+    if exact_proc is not None:
+        ex = finish_exact(exact_proc)
+        if ex is not None:
+            pts, cex = ex
+            pts = [p for p in pts if p < tmax]
+            k = len(pts)
+            out["ct_err_fp32_vs_exact"] = float(np.max(np.abs(r32["corr"][:, pts] - cex[:, :k])))
+            out["ct_err_fp64_vs_exact"] = float(np.max(np.abs(r64["corr"][:, pts] - cex[:, :k])))
+            out["ct_err_exact_steps"] = {"states": K, "run_steps": tmax - 1,
+                                         "sampled_steps": len(pts), "last": pts[-1]}
+            log(f"max |C - C_exact| over {tmax - 1} steps: fp32 "
+                f"{out['ct_err_fp32_vs_exact']:.3e}, fp64 {out['ct_err_fp64_vs_exact']:.3e}")
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Other workloads (sub-records)
+# ---------------------------------------------------------------------------
+def cond_order(s, tile):
+    return s.tile_order(tile, return_tiles=True) if tile > 0 else (None, None)
 
 
 def ac_workload(args):
@@ -283,17 +546,19 @@ def ac_workload(args):
     (FFT coefficients at 2^15 points, precision 1e-12) applied once, then
     J_x psi1, J_y psi1 and psi2 of every state (24 vectors) propagated
     together with tau = pi/2 (16 Bessel terms at 1e-14) and the four current
-    correlations formed every step; complex64 storage, FP64 sums."""
+    correlations formed every step; complex64 (and complex128 for the error)
+    with FP64 sums; 1.5 nm tiles."""
     import numpy as np
     import torch
     import tbplas_amd as tb
     from tbplas_amd import tbpm, _lib, tune
-    L, K = 2048, 8
-    T = args.steps
+    L, K = args.L, args.vectors
+    T = max(args.steps, 2)
+    tile = 1.5 if args.cond_tile is None else args.cond_tile
     s = tb.make_graphene_sample(L)
     s.rescale_ham(9.0)
     ip, ix, hop, dx, dy = s.build_ham_dxy_device()
-    n = s.num_orb
+    n, nnz = s.num_orb, int(ix.numel())
     ts = 2 * math.pi / s.energy_range * 0.5
     bes = tbpm.bessel_series(ts * s.rescale, 1e-14)
     beta = 1.0 / (8.617333262e-5 * 300.0) * s.rescale
@@ -301,88 +566,155 @@ def ac_workload(args):
     cO = tbpm.fermi_cheb_coef(2 ** 15, beta, 0.0, True, 1e-12)
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
-    order, tiles = cond_order(s, args)
-    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32",
-                         handle=h, order=order, tiles=tiles)
-    shape = tune.apply(H, 3 * K, bes, not args.no_tune)
+    order, tiles = cond_order(s, tile)
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
-    # Fermi operators alone (T = 1 also forms one correlation step)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    tbpm.corr_ac(H, bes, cF, cO, psi, 1, chk_thr=1e-4)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    c = tbpm.corr_ac(H, bes, cF, cO, psi, T, chk_thr=1e-4)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    t_prop = (t2 - t1) - (t1 - t0)
-    value = K * (T - 1) / t_prop
-    out = {"metric": "AC correlation state·steps/s (8.4M-orbital graphene, 3 propagated "
-                     "vectors per state)",
-           "value": round(value, 2), "unit": "state·steps/s", "n_gpus": 1,
-           "higher_is_better": True, "dtype": "fp32",
-           "ms_per_step": round(t_prop / (T - 1) * 1e3, 4),
-           "fermi_setup_s": round(t1 - t0, 3), "total_s": round(t2 - t1, 3),
-           "config": {"workload": f"graphene {L}x{L} AC, {K} states (24 vectors), {T} steps, "
-                                  f"{len(bes)} Bessel terms, Fermi terms {len(cF)}",
-                      "n_orbitals": n, "nnz": int(ix.numel()),
-                      "layout": ("caller order" if order is None
-                                 else f"tiles of {args.cond_tile:g} nm")
-                      + passes_name(shape),
-                      "launch_shape": shape_json(shape)},
-           "corr_xx_t2_mean": complex(np.mean(c[:, 0, 1])).real}
-    H.close()
-    print(json.dumps(out), flush=True)
+    res = {}
+    for prec in ("fp32", "fp64"):
+        H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision=prec,
+                             handle=h, order=order, tiles=tiles)
+        real = H.info()["real_values"]
+        shape = tune.apply(H, 3 * K, bes, not args.no_tune)
+        threads = H.blocking_launch(3 * K)[1]
+        sched = H.blocking_bytes(3 * K)
+        # Fermi operators alone (T = 1 also forms one correlation step)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tbpm.corr_ac(H, bes, cF, cO, psi, 1, chk_thr=1e-4)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        c = tbpm.corr_ac(H, bes, cF, cO, psi, T, chk_thr=1e-4)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        kt = {}
+        if args.profile_steps > 0:
+            # the library leaves the Fermi operators' launches untimed
+            h.set_kernel_timing(True)
+            try:
+                tbpm.corr_ac(H, bes, cF, cO, psi, 1 + args.profile_steps, chk_thr=1e-4)
+                kt = h.kernel_times()
+            finally:
+                h.set_kernel_timing(False)
+        H.close()
+        t_prop = (t2 - t1) - (t1 - t0)
+        sz = 8 if prec == "fp32" else 16
+        vsz = sz // 2 if real else sz
+
+        def bytes_of(kind, sched=sched, sz=sz, vsz=vsz):
+            if kind == "ac_corr":
+                # cols, J_x, J_y values (complex), psi1x, psi1y gathered, psi2 own row
+                return nnz * (4 + 2 * sz) + 3 * K * n * sz
+            return pass_bytes_fn(sched, n, nnz, 3 * K, sz, real)(kind)
+        label = (lambda k, prec=prec, real=real, threads=threads:
+                 kernel_label(k, prec, 3 * K, real, threads))
+        res[prec] = {"value": K * (T - 1) / t_prop, "ms_per_step": t_prop / (T - 1) * 1e3,
+                     "fermi_setup_s": t1 - t0, "shape": shape, "corr": c,
+                     "roofline": roofline(kt, bytes_of, label, f"ac_{prec}") if kt else None}
+        log(f"ac {prec}: {res[prec]['value']:.1f} state·steps/s")
+    err = float(np.max(np.abs(res["fp32"]["corr"] - res["fp64"]["corr"])))
+    scale = float(np.max(np.abs(res["fp64"]["corr"])))
+    r = res["fp32"]
+    return {"metric": "AC correlation state·steps/s (8.4M-orbital graphene, 3 propagated "
+                      "vectors per state)",
+            "value": round(r["value"], 2), "unit": "state·steps/s", "higher_is_better": True,
+            "dtype": "fp32", "ms_per_step": round(r["ms_per_step"], 4),
+            "fermi_setup_s": round(r["fermi_setup_s"], 3),
+            "config": {"workload": f"graphene {L}x{L} AC, {K} states ({3 * K} vectors), "
+                                   f"{T} steps, {len(bes)} Bessel terms, Fermi terms {len(cF)}",
+                       "n_orbitals": n, "nnz": nnz,
+                       "layout": (f"tiles of {tile:g} nm" if tile > 0 else "caller order")
+                       + passes_name(r["shape"]),
+                       "launch_shape": shape_json(r["shape"])},
+            "roofline": r["roofline"],
+            "fp64": {"value": round(res["fp64"]["value"], 2),
+                     "ms_per_step": round(res["fp64"]["ms_per_step"], 4),
+                     "roofline": res["fp64"]["roofline"]},
+            "corr_err_fp32_vs_fp64": err, "corr_max_abs": scale, "corr_err_steps": T}
 
 
-def latency_workload(args):
-    """Pristine graphene 256x256 (131,072 orbitals, rescale 9), one state,
-    1024 DOS steps (21 Bessel terms at 1e-15): the case whose vectors fit in
-    the last-level cache, reported as microseconds per step, complex128 and
-    complex64, hipGraph replay, 2 nm tiles (the tuner picks one or three
-    terms per pass)."""
+def dc_workload(args):
+    """Graphene 1024x1024, 1% vacancies, Anderson disorder U[-0.5, 0.5] eV,
+    Peierls phases for B = 20 T (gauge 0), rescale 9; DC correlation at the 5
+    grid energies nearest -1, -0.5, 0, 0.5, 1 eV; 16 states, --steps steps
+    (at most 1024), complex64 (FP64 quasi-eigenstates and correlations),
+    2.0 nm tiles."""
+    import numpy as np
+    import torch
     import tbplas_amd as tb
     from tbplas_amd import tbpm, _lib, tune
-    L, T = 256, 1024
-    s = tb.make_graphene_sample(L)
+    L, K = 1024, 16
+    T = max(2, min(args.steps, 1024))
+    tile = 2.0 if args.cond_tile is None else args.cond_tile
+    s = tb.make_graphene_sample(L, vacancies=tb.random_vacancies(2 * L * L, 0.01, 2024))
+    tb.add_anderson_disorder(s, 1.0, 2025)
+    s.set_magnetic_field(20.0, 0)
     s.rescale_ham(9.0)
-    ip, ix, dt = s.build_ham_csr_device()
-    n = s.num_orb
-    bes = tbpm.bessel_series(2 * math.pi / s.energy_range * s.rescale, 1e-15)
-    psi = tb.states.random_states(n, tb.states.sample_seeds(1337, 1))
+    ip, ix, hop, dx, dy = s.build_ham_dxy_device()
+    n, nnz = s.num_orb, int(ix.numel())
+    en = np.array([0.5 * i * s.energy_range / T - s.energy_range / 2. for i in range(2 * T)])
+    inds = np.array([int(np.argmin(np.abs(en - e))) for e in (-1.0, -0.5, 0.0, 0.5, 1.0)])
+    ts = 2 * math.pi / s.energy_range
+    bes = tbpm.bessel_series(ts * s.rescale, 1e-14)
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
-    # 2 nm tiles (~150 rows): enough workgroups for 256 CUs on this size
-    order, tiles = (s.tile_order(2.0, return_tiles=True) if args.order == "tiles"
-                    else (None, None))
-    res = {}
-    for prec in ("fp64", "fp32"):
-        H = tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h, order=order, tiles=tiles)
-        shape = tune.apply(H, 1, bes, not args.no_tune)
-        ms, _ = gpu_run(H, bes, psi, 32, T)
-        H.close()
-        res[prec] = {"us_per_step": round(ms / T * 1e3, 2),
-                     "us_per_term": round(ms / T / (len(bes) - 1) * 1e3, 3),
-                     "layout": ("caller order" if order is None else "tiles of 2.0 nm")
-                     + passes_name(shape),
-                     "launch_shape": shape_json(shape)}
-    out = {"metric": "TBPM microseconds per step, 131,072-orbital graphene, 1 state",
-           "value": res["fp64"]["us_per_step"], "unit": "us/step", "n_gpus": 1,
-           "higher_is_better": False, "dtype": "fp64",
-           "config": {"workload": f"graphene {L}x{L} DOS, 1 state, {T} steps",
-                      "n_orbitals": n, "nnz": int(ix.numel()), "hipgraph": not args.no_graph},
-           "fp64": res["fp64"], "fp32": res["fp32"]}
-    print(json.dumps(out), flush=True)
+    order, tiles = cond_order(s, tile)
+    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32",
+                         handle=h, order=order, tiles=tiles)
+    real = H.info()["real_values"]
+    shape = tune.apply(H, 2 * K, bes, not args.no_tune)
+    threads = H.blocking_launch(2 * K)[1]
+    sched = H.blocking_bytes(2 * K)
+    psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d, c = tbpm.corr_dc(H, bes, psi, T, ts, en, inds, chk_thr=1e-4)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kt = {}
+    if args.profile_steps > 0:
+        h.set_kernel_timing(True)
+        try:
+            Tp = max(2, min(args.profile_steps, T))
+            enp = np.array([0.5 * i * s.energy_range / Tp - s.energy_range / 2.
+                            for i in range(2 * Tp)])
+            indp = np.array([int(np.argmin(np.abs(enp - e))) for e in (-1.0, -0.5, 0.0, 0.5, 1.0)])
+            tbpm.corr_dc(H, bes, psi, Tp, ts, enp, indp, chk_thr=1e-4)
+            kt = h.kernel_times()
+        finally:
+            h.set_kernel_timing(False)
+    H.close()
+    ne = len(inds)
+
+    def bytes_of(kind):
+        if kind == "dc_qe":    # psi(+t), psi(-t), psi0 read; QE read and written (FP64)
+            return 3 * K * n * 8 + 2 * ne * K * n * 16
+        if kind == "dc_dot":   # J_a psi(t) read, J_a QE read (FP64)
+            return 2 * K * n * 8 + 2 * ne * K * n * 16
+        return pass_bytes_fn(sched, n, nnz, 2 * K, 8, real)(kind)
+    value = 2 * K * (2 * T - 1) / dt
+    return {"metric": "DC correlation step·vectors/s (disordered field-threaded graphene)",
+            "value": round(value, 2), "unit": "step·vectors/s", "higher_is_better": True,
+            "dtype": "fp32", "seconds": round(dt, 3),
+            "config": {"workload": f"graphene {L}x{L}, 1% vacancies, Anderson W=1 eV, B=20 T, "
+                                   f"DC at 5 energies, {K} states, {T} steps",
+                       "n_orbitals": n, "nnz": nnz, "energies": en[inds].tolist(),
+                       "layout": (f"tiles of {tile:g} nm" if tile > 0 else "caller order")
+                       + passes_name(shape),
+                       "launch_shape": shape_json(shape)},
+            "roofline": (roofline(kt, bytes_of,
+                                  lambda k: kernel_label(k, "fp32", 2 * K, real, threads), "dc")
+                         if kt else None),
+            "dos_corr_t1_mean": float(np.mean(d[:, 0]).real),
+            "dc_corr_t1_mean": [float(np.mean(c[:, a, :, 0]).real) for a in range(2)]}
 
 
 def batch_workload(args):
     """32 independent disordered graphene instances of uneven size (edges from
     {128, 256, 384, 512, 768, 1024}, vacancies 0-3%, Anderson W 0-2 eV, all
     seeded), rescaled per instance by the row-sum bound, 4 states each, DOS
-    correlation for 1024 steps, every instance in one launch per pass (rows
-    in --tile nm tiles: three Chebyshev terms per pass where the tuner finds
-    that faster), replayed from a hipGraph; complex64, with the
-    per-instance error against the complex128 run on the same states."""
+    correlation for --steps steps (at most 1024), every instance in one
+    launch per pass (rows in --tile nm tiles), replayed from a hipGraph;
+    complex64, with the per-instance error against the complex128 run on
+    the same states."""
     import numpy as np
     import tbplas_amd as tb
     from tbplas_amd import tbpm, _lib, tune
@@ -390,8 +722,9 @@ def batch_workload(args):
     edges = rng.choice([128, 256, 384, 512, 768, 1024], 32)
     vac = rng.uniform(0.0, 0.03, 32)
     Ws = rng.uniform(0.0, 2.0, 32)
-    K, T = 4, args.steps if args.steps != 2048 else 1024
+    K = 4
     h = _lib.default_handle(0)
+    h.set_graph(not args.no_graph)
     insts = []
     for i in range(32):
         L = int(edges[i])
@@ -404,43 +737,178 @@ def batch_workload(args):
                       s.tile_order(args.tile, return_tiles=True) if args.order == "tiles"
                       else (None, None)))
     bes = tbpm.bessel_series(math.pi, 1e-14)
+    total = args.warmup + args.steps + args.profile_steps
     res = {}
     for prec in ("fp32", "fp64"):
         hams = [tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h, order=ot[0],
                                  tiles=ot[1])
                 for (n, nnz, ip, ix, dt, _, ot) in insts]
         shape = tune.apply(hams, K, bes, not args.no_tune)
-        run = tbpm.DosRun(hams, bes, [x[5] for x in insts], T + 16)
-        run.advance(16)
-        ms = run.advance(T)
-        res[prec] = (ms, run.result(chk_step=0), shape,
-                     all(H.info()["real_values"] for H in hams))
+        real = all(H.info()["real_values"] for H in hams)
+        run = tbpm.DosRun(hams, bes, [x[5] for x in insts], total)
+        ms, kt = timed_run(run, h, args, total)
+        res[prec] = (ms, run.result(chk_step=0), shape, real, kt, run.blocking_bytes())
         run.close()
         for H in hams:
             H.close()
-    ms, c32, shape, _ = res["fp32"]
+    ms, c32, shape, real, kt, sched = res["fp32"]
     c64 = res["fp64"][1]
     errs = [float(np.max(np.abs(a - b))) for a, b in zip(c32, c64)]
     n_tot = sum(x[0] for x in insts)
     nnz_tot = sum(x[1] for x in insts)
-    t_step = ms / 1e3 / T
-    nterm = len(bes) - 1
-    bt = (term_bytes(n_tot, nnz_tot, K, 8, res["fp32"][3]) - 4 * (n_tot + 1)
-          + sum(4 * (x[0] + 1) for x in insts))
-    ach = bt / (t_step / nterm) / 1e9
-    out = {"metric": "uneven batch DOS step·vectors/s (32 instances in one launch per pass)",
-           "value": round(32 * K / t_step, 2), "unit": "instance-step·vectors/s", "n_gpus": 1,
-           "higher_is_better": True, "dtype": "fp32", "ms_per_step": round(t_step * 1e3, 4),
-           "config": {"workload": f"32 disordered graphene instances, edges {sorted(set(edges.tolist()))}, "
-                                  f"{K} states each, {T} steps", "n_orbitals_total": n_tot,
-                      "nnz_total": nnz_tot, "launch_shape": shape_json(shape),
-                      "layout": layout_name(args, shape)},
-           "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)},
-           "fp64_ms_per_step": round(res["fp64"][0] / T, 4),
-           "ct_err_fp32_vs_fp64_per_instance_max": max(errs),
-           "ct_err_fp32_vs_fp64_per_instance_median": float(np.median(errs))}
-    print(json.dumps(out), flush=True)
+    t_step = ms / 1e3 / args.steps
+    return {"metric": "uneven batch DOS step·vectors/s (32 instances in one launch per pass)",
+            "value": round(32 * K / t_step, 2), "unit": "instance-step·vectors/s",
+            "higher_is_better": True, "dtype": "fp32", "ms_per_step": round(t_step * 1e3, 4),
+            "config": {"workload": f"32 disordered graphene instances, edges "
+                                   f"{sorted(set(edges.tolist()))}, {K} states each, "
+                                   f"{args.steps} timed steps", "n_orbitals_total": n_tot,
+                       "nnz_total": nnz_tot, "launch_shape": shape_json(shape),
+                       "layout": (f"tiles of {args.tile:g} nm" if args.order == "tiles"
+                                  else "caller order") + passes_name(shape),
+                       "hipgraph": not args.no_graph},
+            "roofline": (roofline(kt, pass_bytes_fn(sched, n_tot, nnz_tot, K, 8, real),
+                                  lambda k: kernel_label(k, "fp32", K, real, 1024), "batch")
+                         if kt else None),
+            "fp64_ms_per_step": round(res["fp64"][0] / args.steps, 4),
+            "ct_err_fp32_vs_fp64_per_instance_max": max(errs),
+            "ct_err_fp32_vs_fp64_per_instance_median": float(np.median(errs))}
+
+
+def latency_workload(args):
+    """Pristine graphene 256x256 (131,072 orbitals, rescale 9), one state,
+    1024 DOS steps (21 Bessel terms at 1e-15): the case whose vectors fit in
+    the last-level cache, reported as microseconds per step, complex128 and
+    complex64, hipGraph replay, 2 nm tiles (the tuner picks one, three or
+    four terms per pass)."""
+    import tbplas_amd as tb
+    from tbplas_amd import tbpm, _lib, tune
+    L, T = 256, 1024
+    s = tb.make_graphene_sample(L)
+    s.rescale_ham(9.0)
+    ip, ix, dt = s.build_ham_csr_device()
+    n, nnz = s.num_orb, int(ix.numel())
+    bes = tbpm.bessel_series(2 * math.pi / s.energy_range * s.rescale, 1e-15)
+    psi = tb.states.random_states(n, tb.states.sample_seeds(1337, 1))
+    h = _lib.default_handle(0)
+    h.set_graph(not args.no_graph)
+    order, tiles = (s.tile_order(2.0, return_tiles=True) if args.order == "tiles"
+                    else (None, None))
+    a = argparse.Namespace(**vars(args))
+    a.warmup, a.steps = 32, T
+    res = {}
+    for prec in ("fp64", "fp32"):
+        H = tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h, order=order, tiles=tiles)
+        real = H.info()["real_values"]
+        shape = tune.apply(H, 1, bes, not args.no_tune)
+        threads = H.blocking_launch(1)[1]
+        run = tbpm.DosRun(H, bes, psi, a.warmup + T + a.profile_steps)
+        ms, kt = timed_run(run, h, a, a.warmup + T + a.profile_steps)
+        sched = run.blocking_bytes()
+        run.close()
+        H.close()
+        sz = 8 if prec == "fp32" else 16
+        rl = (roofline(kt, pass_bytes_fn(sched, n, nnz, 1, sz, real),
+                       lambda k, prec=prec, real=real, threads=threads:
+                       kernel_label(k, prec, 1, real, threads), f"latency_{prec}")
+              if kt else None)
+        if rl is not None:
+            rl["note"] = ("the vectors fit in the 256 MB last-level cache: launch and "
+                          "latency bound, HBM bandwidth does not bound it")
+        res[prec] = {"us_per_step": round(ms / T * 1e3, 2),
+                     "us_per_term": round(ms / T / (len(bes) - 1) * 1e3, 3),
+                     "layout": ("caller order" if order is None else "tiles of 2.0 nm")
+                     + passes_name(shape),
+                     "launch_shape": shape_json(shape), "roofline": rl}
+    return {"metric": "TBPM microseconds per step, 131,072-orbital graphene, 1 state",
+            "value": res["fp64"]["us_per_step"], "unit": "us/step", "higher_is_better": False,
+            "dtype": "fp64",
+            "config": {"workload": f"graphene {L}x{L} DOS, 1 state, {T} steps",
+                       "n_orbitals": n, "nnz": nnz, "hipgraph": not args.no_graph},
+            "fp64": res["fp64"], "fp32": res["fp32"]}
+
+
+def assembly_workload(args):
+    """GPU Hamiltonian assembly from the hop list: the insertion-order CSR
+    with dx, dy (build_ham_dxy_fast's output, sample.pyx:148-257) and the
+    canonical CSR (build_ham_csr) at 8.4M orbitals (2048x2048 graphene) and
+    10^7 atoms (2236x2236), timed with the inputs already in device memory
+    (count + fill) and end to end from host arrays to host arrays."""
+    import ctypes
+    import torch
+    import tbplas_amd as tb
+    from tbplas_amd import _lib
+    from tbplas_amd.sample import _DevHops
+    h = _lib.default_handle(0)
+    L_ = _lib.lib()
+    recs = {}
+    for L in (2048, 2236):
+        s = tb.make_graphene_sample(L)
+        s.rescale_ham(9.0)
+        s.init_orb_pos()
+        n, m = s.num_orb, s.num_hop
+        d = _DevHops(s, 0, with_dr=True)
+        dev = "cuda:0"
+
+        def dxy():
+            indptr = torch.empty(n + 1, dtype=torch.int64, device=dev)
+            nnz = ctypes.c_int64(0)
+            _lib.check(L_.tbg_ham_dxy_count(h.ptr, n, m, _lib.ptr(d.orb_eng), _lib.ptr(d.hop_i),
+                                            _lib.ptr(d.hop_j), 1e-5, _lib.ptr(indptr),
+                                            ctypes.byref(nnz)))
+            k = nnz.value
+            ix = torch.empty(k, dtype=torch.int64, device=dev)
+            hv = torch.empty(k, dtype=torch.complex128, device=dev)
+            dx = torch.empty(k, dtype=torch.float64, device=dev)
+            dy = torch.empty(k, dtype=torch.float64, device=dev)
+            _lib.check(L_.tbg_ham_dxy_fill(h.ptr, n, m, _lib.ptr(d.orb_eng), _lib.ptr(d.hop_i),
+                                           _lib.ptr(d.hop_j), _lib.ptr(d.hop_v), _lib.ptr(d.dr),
+                                           1e-5, _lib.ptr(indptr), _lib.ptr(ix), _lib.ptr(hv),
+                                           _lib.ptr(dx), _lib.ptr(dy)))
+            return k
+
+        def csr():
+            indptr = torch.empty(n + 1, dtype=torch.int64, device=dev)
+            nnz = ctypes.c_int64(0)
+            _lib.check(L_.tbg_ham_csr_count(h.ptr, n, m, _lib.ptr(d.orb_eng), _lib.ptr(d.hop_i),
+                                            _lib.ptr(d.hop_j), _lib.ptr(d.hop_v),
+                                            _lib.ptr(indptr), ctypes.byref(nnz)))
+            k = nnz.value
+            ix = torch.empty(k, dtype=torch.int64, device=dev)
+            dt = torch.empty(k, dtype=torch.complex128, device=dev)
+            _lib.check(L_.tbg_ham_csr_fill(h.ptr, n, m, _lib.ptr(d.orb_eng), _lib.ptr(d.hop_i),
+                                           _lib.ptr(d.hop_j), _lib.ptr(d.hop_v),
+                                           _lib.ptr(indptr), _lib.ptr(ix), _lib.ptr(dt)))
+            return k
+
+        def best(f, reps=3):
+            f()
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                k = f()
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            return min(ts), k
+        t_dxy, nnz_dxy = best(dxy)
+        t_csr, nnz_csr = best(csr)
+        t0 = time.perf_counter()
+        s.build_ham_dxy()
+        t_host = time.perf_counter() - t0
+        recs[f"{n}_orbitals"] = {
+            "L": L, "n_orbitals": n, "hops": m, "nnz": nnz_dxy,
+            "dxy_device_s": round(t_dxy, 5), "csr_device_s": round(t_csr, 5),
+            "dxy_host_to_host_s": round(t_host, 4),
+            "dxy_entries_per_s": round(nnz_dxy / t_dxy, 1), "csr_nnz": nnz_csr}
+        del d
+        torch.cuda.empty_cache()
+    return {"metric": "GPU Hamiltonian assembly time (insertion-order CSR + dx, dy; canonical "
+                      "CSR)", "unit": "s", "higher_is_better": False, "dtype": "int64+fp64",
+            "value": recs["9999392_orbitals"]["dxy_device_s"],
+            "reference_published": {"build_ham_dxy_fast_s": 1.48, "build_ham_dxy_safe_s": 2.83,
+                                    "atoms": "1e7", "source": "tbplas/cython/sample.pyx:299-301"},
+            "sizes": recs}
 
 
 def main():
@@ -448,115 +916,33 @@ def main():
     if args.cpu_child:
         cpu_child(args)
         return 0
-    if args.copy_probe:
-        copy_probe()
-        return 0
-    if args.workload != "dos":
-        if args.gpus != 1:
-            print("bench.py drives one GPU", file=sys.stderr)
-            return 2
-        {"dc": dc_workload, "batch": batch_workload, "ac": ac_workload,
-         "latency": latency_workload}[args.workload](args)
+    if args.exact_child:
+        exact_child(args)
         return 0
     if args.gpus != 1:
         print(f"bench.py drives one GPU; --gpus {args.gpus} is not supported "
               "(states are independent: several GPUs would each propagate a share "
               "of them and sum the correlation partials once per run).", file=sys.stderr)
         return 2
-    cpu = None if (args.no_cpu or args.only_fp32) else cpu_baseline(args)
-
-    import numpy as np
-    import tbplas_amd as tb
-    from tbplas_amd import tbpm, _lib
-
-    t0 = time.perf_counter()
-    s = tb.make_graphene_sample(args.L)
-    s.rescale_ham(9.0)
-    indptr, indices, data = s.build_ham_csr_device()
-    n, nnz = s.num_orb, int(indices.numel())
-    ts = 2 * math.pi / s.energy_range
-    bes = tbpm.bessel_series(ts * s.rescale, 1e-15)
-    K = args.vectors
-    psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
-    log(f"setup {time.perf_counter() - t0:.1f}s: n={n} nnz={nnz} n_bes={len(bes)}")
-    h = _lib.default_handle(0)
-    h.set_graph(not args.no_graph)
-
-    from tbplas_amd import tune
-    order, tiles = (s.tile_order(args.tile, return_tiles=True) if args.order == "tiles"
-                    else (None, None))
-    H32 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp32", handle=h, order=order,
-                           tiles=tiles)
-    real = H32.info()["real_values"]
-    shape32 = tune.apply(H32, K, bes, not args.no_tune)
-    ms32, c32 = gpu_run(H32, bes, psi, args.warmup, args.steps)
-    H32.close()
-    t_step = ms32 / 1e3 / args.steps
-    value = K / t_step
-    nterm = len(bes) - 1
-    bt = term_bytes(n, nnz, K, 8, real)
-    achieved = bt / (t_step / nterm) / 1e9
-    log(f"fp32 K={K}: {ms32:.1f} ms for {args.steps} steps -> {value:.1f} step·vectors/s, "
-        f"{achieved:.0f} GB/s algorithmic")
-
-    out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "step·vectors/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
-        "data": "synthetic (seeded random-phase states; pristine graphene lattice)",
-        "config": {"workload": (f"graphene {args.L}x{args.L} PBC DOS correlation, {K} states "
-                                f"together, {args.warmup + args.steps} Chebyshev steps, "
-                                "complex64 storage/FP64 correlation"),
-                   "n_orbitals": n, "nnz": nnz, "states": K, "bessel_terms": len(bes),
-                   "terms_per_step": nterm, "hipgraph": not args.no_graph,
-                   "layout": layout_name(args, shape32),
-                   "launch_shape": shape_json(shape32)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_term": bt,
-                     "term_avg_us": round(t_step / nterm * 1e6, 2)},
-    }
-    layout = layout_name(args, shape32)
-    if passes_terms(shape32) > 1:
-        out["roofline"]["note"] = (
-            "achieved = the one-term-per-pass algorithmic bytes per term over the "
-            "measured time per term; several terms per pass read the state vectors "
-            "fewer times, so it can exceed what one term per pass could reach "
-            "(traffic: bytes per term actually moved, from the counters)")
-    traffic_file = os.path.join(HERE, "profiles", "traffic_term_fp32_k8.json")
-    if os.path.exists(traffic_file) and args.L == 2048 and K == 8:
-        with open(traffic_file) as f:
-            tr = json.load(f)
-        if tr.get("layout") == layout and tr.get("bytes_per_term"):
-            moved = float(tr["bytes_per_term"])
-            out["roofline"]["traffic"] = moved
-            # the bytes the counters saw per term over the same time per term
-            rate = moved / (t_step / nterm) / 1e9
-            out["roofline"]["moved_GBs"] = round(rate, 1)
-            out["roofline"]["moved_frac"] = round(rate / HBM_PEAK_GBS, 4)
-
-    if not args.only_fp32:
-        steps64 = args.fp64_steps or args.steps
-        H64 = tbpm.Hamiltonian(n, indptr, indices, data, precision="fp64", handle=h,
-                               order=order, tiles=tiles)
-        shape64 = tune.apply(H64, 1, bes, not args.no_tune)
-        ms64, c64 = gpu_run(H64, bes, psi[:1], args.warmup, steps64)
-        real64 = H64.info()["real_values"]
-        H64.close()
-        t64 = ms64 / 1e3 / steps64
-        b64 = term_bytes(n, nnz, 1, 16, real64)
-        tmax = min(args.warmup + args.steps, args.warmup + steps64)
-        err = float(np.max(np.abs(c32[0, :tmax + 1] - c64[0, :tmax + 1])))
-        out["fp64_k1"] = {"value": round(1.0 / t64, 3), "unit": "step·vectors/s",
-                          "ms_per_step": round(t64 * 1e3, 4),
-                          "roofline_frac": round(b64 / (t64 / nterm) / 1e9 / HBM_PEAK_GBS, 4),
-                          "launch_shape": shape_json(shape64)}
-        out["speedup_vs_fp64_k1"] = round(value * t64, 3)
-        out["ct_err_fp32_vs_fp64"] = err
-        out["ct_err_steps"] = tmax
-        log(f"fp64 K=1: {1.0 / t64:.1f} step/s; fp32 vs fp64 max |dC| = {err:.3e}")
-    if cpu is not None:
-        out["cpu_baseline"] = cpu
+    if args.copy_probe:
+        copy_probe()
+        return 0
+    subs = {"ac": ac_workload, "dc": dc_workload, "batch": batch_workload,
+            "latency": latency_workload, "assembly": assembly_workload}
+    if args.workload in subs:
+        rec = subs[args.workload](args)
+        rec["n_gpus"] = 1
+        print(json.dumps(rec), flush=True)
+        return 0
+    cpu = None if args.no_cpu else cpu_baseline(args)
+    exact = start_exact(args) if args.accuracy_steps >= ACC_STEPS else None
+    out = headline(args, cpu, exact)
+    if args.workload == "all":
+        for name, fn in subs.items():
+            t0 = time.perf_counter()
+            out[name] = fn(args)
+            log(f"{name}: {out[name]['value']} {out[name]['unit']} "
+                f"({time.perf_counter() - t0:.1f} s)")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -177,6 +177,30 @@ int tbg_ham_blocking_info(tbg_handle *h, const tbg_ham *H, int states,
  * counterpart. */
 int tbg_ham_blocking_launch(tbg_handle *h, const tbg_ham *H, int states,
                             int precision, int *max_rows, int *threads);
+/* What one full tile pass (all of the tables' terms, mode "continue") must
+ * move, summed over its workgroup tiles: parts[0] = table bytes (columns,
+ * values, LDS positions, row numbers, tile records), parts[1] = rows of
+ * v_{m-1} gathered (distinct rows per tile), parts[2] = rows of v_{m-2}
+ * read (R and its rings), parts[3] = owned rows (output read and written,
+ * last two terms written: four row streams).  A row of states is
+ * slots * element bytes.  All zero when H runs one term per pass.  No
+ * reference counterpart. */
+int tbg_ham_blocking_bytes(tbg_handle *h, const tbg_ham *H, int states,
+                           int precision, int64_t *parts);
+
+/* ---- per-launch timing -----------------------------------------------------
+ * No reference counterpart.  With timing on, every propagation and
+ * correlation launch of the handle is bracketed by HIP events on its stream
+ * (graph replay is skipped meanwhile).  tbg_kernel_times synchronises the
+ * stream, sums the durations per launch kind and clears them: kinds[i]
+ * (1 = one-term kernel, 2 = AC correlation, 3 = DC quasi-eigenstate update,
+ * 4 = DC current correlation, 100 + 10 * mode + terms = tile pass with mode
+ * 0 continue, 1 first terms from v_0, 2 last terms, 3 last terms with the
+ * norm, 4 last terms with dot and norm), total_ms[i], counts[i], for at
+ * most max_kinds kinds; *n_kinds = kinds seen. */
+int tbg_set_kernel_timing(tbg_handle *h, int enable);
+int tbg_kernel_times(tbg_handle *h, int max_kinds, int *kinds,
+                     double *total_ms, int64_t *counts, int *n_kinds);
 
 /* ---- correlation functions ----------------------------------------------
  * psi0: K initial states, complex128 (K, n) row-major (state k contiguous),
@@ -345,6 +369,10 @@ int tbg_run_advance(tbg_run *r, int nsteps, double *elapsed_ms);
 int tbg_dos_end(tbg_run *r, int chk_step, double chk_thr,
                 double *const *corr, int *fail_inst, int *fail_step);
 int tbg_run_destroy(tbg_run *r);
+/* The byte counts of tbg_ham_blocking_bytes for the tables the run uses
+ * (a batch's concatenated tiles included) and the bytes of one row of
+ * states; all zero when the run goes one term per pass. */
+int tbg_run_blocking_bytes(const tbg_run *r, int64_t *parts, int *row_bytes);
 
 /* ---- learned DOS surrogate -------------------------------------------------
  * No reference counterpart.  A dense network x -> GELU(x W_0 + b_0) -> ... ->

@@ -42,6 +42,9 @@ _SIGS = {
     "tbg_set_blocking": [_P, _I, _I],
     "tbg_ham_blocking_info": [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)],
     "tbg_ham_blocking_launch": [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)],
+    "tbg_ham_blocking_bytes": [_P, _P, _I, _I, _P],
+    "tbg_set_kernel_timing": [_P, _I],
+    "tbg_kernel_times": [_P, _I, _P, _P, _P, ctypes.POINTER(_I)],
     "tbg_ham_destroy": [_P],
     "tbg_ham_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                      ctypes.POINTER(_I64), ctypes.POINTER(_I)],
@@ -71,6 +74,7 @@ _SIGS = {
     "tbg_run_advance": [_P, _I, ctypes.POINTER(_D)],
     "tbg_dos_end": [_P, _I, _D, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "tbg_run_destroy": [_P],
+    "tbg_run_blocking_bytes": [_P, _P, ctypes.POINTER(_I)],
     "tbg_mlp_create": [_P, _I, _P, ctypes.POINTER(_P)],
     "tbg_mlp_destroy": [_P],
     "tbg_mlp_num_params": [_P, ctypes.POINTER(_I64)],
@@ -179,6 +183,32 @@ class Handle:
         if terms not in (3, 4):
             raise ValueError("terms per pass must be 3 or 4")
         check(lib().tbg_set_blocking(self._h, int(terms) if enable else 0, int(max_rows)))
+
+    def set_kernel_timing(self, enable=True):
+        """Bracket every propagation and correlation launch by HIP events
+        (graph replay is skipped while on); read with kernel_times()."""
+        check(lib().tbg_set_kernel_timing(self._h, 1 if enable else 0))
+
+    def kernel_times(self):
+        """{kind: (total ms, launches)} of the launches timed since the last
+        call (synchronises the handle's stream).  Kinds: "term" (one-term
+        kernel), "ac_corr", "dc_qe", "dc_dot", and "pass<mode>x<terms>" for
+        the tile pass (mode 0 continue, 1 first terms from v_0, 2 last
+        terms, 3 with the norm, 4 with dot and norm)."""
+        cap = 64
+        kinds = np.zeros(cap, dtype=np.int32)
+        ms = np.zeros(cap, dtype=np.float64)
+        cnt = np.zeros(cap, dtype=np.int64)
+        nk = ctypes.c_int()
+        check(lib().tbg_kernel_times(self._h, cap, kinds.ctypes.data, ms.ctypes.data,
+                                     cnt.ctypes.data, ctypes.byref(nk)))
+        names = {1: "term", 2: "ac_corr", 3: "dc_qe", 4: "dc_dot"}
+        out = {}
+        for i in range(min(nk.value, cap)):
+            k = int(kinds[i])
+            name = names.get(k) or f"pass{(k - 100) // 10}x{(k - 100) % 10}"
+            out[name] = (float(ms[i]), int(cnt[i]))
+        return out
 
     def set_real_layout(self, enable=True):
         """Store real values for Hamiltonians without imaginary parts

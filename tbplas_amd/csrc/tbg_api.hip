@@ -54,6 +54,8 @@ struct tbg_run {
     virtual void advance(int nsteps) = 0;
     virtual void end(int chk_step, double chk_thr, double* const* corr, int* fail_inst,
                      int* fail_step) = 0;
+    // the byte counts of the run's tile-pass tables (zeros without them)
+    virtual void sched(int64_t* parts, int* row_bytes) const = 0;
     tbg_handle* h = nullptr;
     int T = 0, done = 0;
     // time index of the norm reference of the wave-function check: 1 for
@@ -166,6 +168,11 @@ struct DosRun : tbg_run {
         e.run(nsteps, sp, 0, 0, F_DOT | F_NORM, psi.p, counter.as<int>(),
               [&]() { reduce(e.epi_range()); });
         done += nsteps;
+    }
+
+    void sched(int64_t* parts, int* row_bytes) const override {
+        for (int i = 0; i < 4; ++i) parts[i] = e.blk ? e.blk->sched[i] : 0;
+        *row_bytes = e.blk ? e.blk->row_bytes : 0;
     }
 
     void end(int chk_step, double chk_thr, double* const* corr, int* fail_inst,
@@ -343,6 +350,11 @@ int tbg_destroy(tbg_handle* h) {
             (void)hipStreamSynchronize(h->stream);
             (void)hipStreamDestroy(h->stream);
         }
+        for (auto& t : h->timed) {
+            (void)hipEventDestroy(t.e0);
+            (void)hipEventDestroy(t.e1);
+        }
+        for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
         delete h;
     });
 }
@@ -378,6 +390,52 @@ int tbg_set_real_layout(tbg_handle* h, int enable) {
     return guard([&] {
         check_handle(h);
         h->real_layout = enable ? 1 : 0;
+    });
+}
+
+int tbg_set_kernel_timing(tbg_handle* h, int enable) {
+    return guard([&] {
+        check_handle(h);
+        h->timing = enable ? 1 : 0;
+    });
+}
+
+int tbg_kernel_times(tbg_handle* h, int max_kinds, int* kinds, double* total_ms,
+                     int64_t* counts, int* n_kinds) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(n_kinds != nullptr && max_kinds >= 0, "null argument");
+        TBG_REQUIRE(max_kinds == 0 || (kinds && total_ms && counts), "null argument");
+        TBG_HIP(hipSetDevice(h->device));
+        TBG_HIP(hipStreamSynchronize(h->stream));
+        std::vector<int> ks;
+        std::vector<double> ms;
+        std::vector<int64_t> cs;
+        hipError_t err = hipSuccess;
+        for (auto& t : h->timed) {
+            float e = 0.0f;
+            const hipError_t r = hipEventElapsedTime(&e, t.e0, t.e1);
+            if (r != hipSuccess && err == hipSuccess) err = r;
+            size_t i = 0;
+            while (i < ks.size() && ks[i] != t.kind) ++i;
+            if (i == ks.size()) {
+                ks.push_back(t.kind);
+                ms.push_back(0.0);
+                cs.push_back(0);
+            }
+            ms[i] += e;
+            cs[i] += 1;
+            h->ev_pool.push_back(t.e0);
+            h->ev_pool.push_back(t.e1);
+        }
+        h->timed.clear();
+        TBG_HIP(err);
+        *n_kinds = (int)ks.size();
+        for (int i = 0; i < (int)ks.size() && i < max_kinds; ++i) {
+            kinds[i] = ks[i];
+            total_ms[i] = ms[i];
+            counts[i] = cs[i];
+        }
     });
 }
 
@@ -651,6 +709,20 @@ int tbg_ham_blocking_launch(tbg_handle* h, const tbg_ham* H, int states, int pre
     });
 }
 
+int tbg_ham_blocking_bytes(tbg_handle* h, const tbg_ham* H, int states, int precision,
+                           int64_t* parts) {
+    return guard([&] {
+        check_handle(h);
+        TBG_REQUIRE(H != nullptr && parts != nullptr, "null argument");
+        TBG_REQUIRE(states >= 1 && states <= 64, "states must be in [1, 64]");
+        TBG_REQUIRE(precision == TBG_FP64 || precision == TBG_FP32, "unknown precision");
+        const int Kp = slots_for(states);
+        const int cb = precision == TBG_FP32 ? (int)sizeof(float2) : (int)sizeof(double2);
+        auto b = get_blocking(h, H, Kp * cb);
+        for (int i = 0; i < 4; ++i) parts[i] = b ? b->sched[i] : 0;
+    });
+}
+
 static void check_batch(tbg_handle* h, int n_inst, const tbg_ham* const* hams,
                         const double* bes, int n_bes, const double* const* psi0, int K, int T) {
     check_handle(h);
@@ -730,6 +802,13 @@ int tbg_dos_begin(tbg_handle* h, int n_inst, const tbg_ham* const* hams, const d
         check_batch(h, n_inst, hams, bes, n_bes, psi0, K, T);
         TBG_REQUIRE(out != nullptr, "null output pointer");
         *out = dos_begin(h, n_inst, hams, bes, n_bes, psi0, K, T);
+    });
+}
+
+int tbg_run_blocking_bytes(const tbg_run* r, int64_t* parts, int* row_bytes) {
+    return guard([&] {
+        TBG_REQUIRE(r != nullptr && parts != nullptr && row_bytes != nullptr, "null argument");
+        r->sched(parts, row_bytes);
     });
 }
 

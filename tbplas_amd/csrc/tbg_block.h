@@ -35,6 +35,10 @@ struct Blocking {
     std::vector<int> seg_block;   // blocks of segment i: [seg_block[i], seg_block[i+1])
     int nA_max = 0, nB_max = 0;
     double halo = 0.0;       // (sum nA) / n: rows computed per row owned, term m
+    // What a full pass of depth + 1 terms must move, summed over the tiles:
+    // table bytes, distinct rows of v_{m-1} gathered, rows of v_{m-2} read
+    // (sum nA), owned rows (sum n[0]).
+    int64_t sched[4] = {0, 0, 0, 0};
     DevBuf meta;             // BlockMeta [nblocks]
     DevBuf rows;             // int32 [sum nA]
     DevBuf colL;             // uint16 [sum w * nB]

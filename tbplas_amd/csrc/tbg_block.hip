@@ -93,6 +93,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
     std::vector<unsigned char> valG;
     std::vector<int> mark((size_t)n, -1), loc((size_t)n, 0);
     int nA_max = 0, nB_max = 0;
+    int64_t sched[4] = {0, 0, 0, 0};
     int64_t budget = h->max_lds;
     for (int attempt = 0;; ++attempt) {
         meta.clear();
@@ -103,6 +104,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
         valG.clear();
         std::fill(mark.begin(), mark.end(), -1);
         nA_max = nB_max = 0;
+        for (int q = 0; q < 4; ++q) sched[q] = 0;
         for (size_t sg = 0; sg < segs.size(); ++sg) {
             const std::vector<int64_t>& t = segs[sg];
             seg_block.push_back((int)meta.size());
@@ -198,6 +200,25 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
                             cl[(size_t)jj * nB + l] = (uint16_t)(jj < w ? loc[col(r, jj)] : l);
                     }
                 }
+                // distinct rows of v_{m-1} the first phase gathers: A and
+                // the ring beyond its last one
+                int64_t outer = 0;
+                for (int l = nB; l < nA; ++l) {
+                    const int r = rows[base + l];
+                    for (int jj = 0, w = width(r); jj < w; ++jj) {
+                        const int c = col(r, jj);
+                        if (mark[c] != b) {
+                            mark[c] = b;
+                            ++outer;
+                        }
+                    }
+                }
+                sched[0] += (int64_t)wA * nA * (int64_t)(sizeof(int) + vsz) +
+                            (int64_t)wA * nB * (int64_t)sizeof(uint16_t) +
+                            (int64_t)nA * (int64_t)sizeof(int) + (int64_t)sizeof(BlockMeta);
+                sched[1] += nA + outer;
+                sched[2] += nA;
+                sched[3] += r1 - r0;
                 nA_max = std::max(nA_max, nA);
                 nB_max = std::max(nB_max, nB);
             }
@@ -236,6 +257,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
     B->nA_max = nA_max;
     B->nB_max = nB_max;
     B->halo = (double)rows.size() / (double)std::max<int64_t>(covered, 1);
+    for (int q = 0; q < 4; ++q) B->sched[q] = sched[q];
     B->meta.alloc(meta.size() * sizeof(BlockMeta));
     B->rows.alloc(rows.size() * sizeof(int));
     B->colL.alloc(std::max<size_t>(colL.size(), 1) * sizeof(uint16_t));

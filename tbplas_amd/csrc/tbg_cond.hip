@@ -447,7 +447,10 @@ void corr_ac_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
     uint64_t cls = 0;
     for (int k = 2 * K; k < 3 * K; ++k) cls |= 1ull << k;
     const StepPoly fp = cheb_poly(cO, nO, cF, nF);
-    e.step(fp, 0, cls, 0, nullptr, nullptr);
+    {
+        TimingHold hold(h);
+        e.step(fp, 0, cls, 0, nullptr, nullptr);
+    }
     // norm_ref of psi2 (tbpm.f90:306) and per-step norms of every slot.
     const int nvn = 3 * Kp;
     DevBuf nbuf((size_t)T * nvn * sizeof(double));
@@ -462,11 +465,13 @@ void corr_ac_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
     auto after = [&]() {
         launch_reduce(e.part.template as<double>(), nvn, e.epi_range(), 1, nbuf.as<double>(), 0,
                       counter.as<int>(), 0, st);
-        hipLaunchKernelGGL(k_accorr<R>, dim3(e.plan.nchunks), dim3(256), 0, st,
-                           H->cols.as<int>(), H->jx.as<C>(), H->jy.as<C>(),
-                           H->slice_off.as<int64_t>(), H->slice_w.as<int>(),
-                           e.plan.chunk.template as<int64_t>(), e.X.template as<C>(), Kp, K,
-                           e.part.template as<double>());
+        timed_launch(h, kTimeAcCorr, [&] {
+            hipLaunchKernelGGL(k_accorr<R>, dim3(e.plan.nchunks), dim3(256), 0, st,
+                               H->cols.as<int>(), H->jx.as<C>(), H->jy.as<C>(),
+                               H->slice_off.as<int64_t>(), H->slice_w.as<int>(),
+                               e.plan.chunk.template as<int64_t>(), e.X.template as<C>(), Kp, K,
+                               e.part.template as<double>());
+        });
         launch_reduce(e.part.template as<double>(), 8 * K, e.plan.inst_chunk_dev.template as<int>(),
                       1, abuf.as<double>(), 0, counter.as<int>(), 0, st);
     };
@@ -559,7 +564,10 @@ void dyn_pol_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
         hipLaunchKernelGGL(k_rho_conj_mul<R>, dim3(nblk(n_pad * K)), dim3(256), 0, st,
                            e.X.template as<C>(), Kp, K, rho.as<double2>(), n_pad);
         TBG_HIP(hipGetLastError());
-        e.step(fp, 0, cls, 0, nullptr, nullptr);
+        {
+            TimingHold hold(h);
+            e.step(fp, 0, cls, 0, nullptr, nullptr);
+        }
         TBG_HIP(hipMemsetAsync(counter.p, 0, sizeof(int), st));
         launch_dotnorm<R>(e.plan, e.X.p, e.X.p, Kp, Kp, e.part.template as<double>(), st);
         launch_reduce(e.part.template as<double>(), nvn, e.plan.inst_chunk_dev.template as<int>(),
@@ -664,10 +672,13 @@ void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
     auto after1 = [&]() {
         launch_reduce(e.part.template as<double>(), nvn, e.epi_range(), 1, nbuf.as<double>(), 0,
                       counter.as<int>(), 0, st);
-        hipLaunchKernelGGL(k_qe<R>, dim3(e.plan.nchunks), dim3(256), 0, st,
-                           e.plan.chunk.template as<int64_t>(), e.X.template as<C>(), Kp, K,
-                           psi.as<C>(), qe.as<double2>(), n_en, dptab.as<double2>(),
-                           dwtab.as<double>(), counter.as<int>(), e.part.template as<double>());
+        timed_launch(h, kTimeQe, [&] {
+            hipLaunchKernelGGL(k_qe<R>, dim3(e.plan.nchunks), dim3(256), 0, st,
+                               e.plan.chunk.template as<int64_t>(), e.X.template as<C>(), Kp, K,
+                               psi.as<C>(), qe.as<double2>(), n_en, dptab.as<double2>(),
+                               dwtab.as<double>(), counter.as<int>(),
+                               e.part.template as<double>());
+        });
         launch_reduce(e.part.template as<double>(), 2 * K, e.plan.inst_chunk_dev.template as<int>(),
                       1, dbuf.as<double>(), 0, counter.as<int>(), -1, st);
     };
@@ -737,9 +748,11 @@ void corr_dc_impl(tbg_handle* h, const tbg_ham* H, const double* bes, int nb, co
     const int64_t ntiles2 = (nq2 + 255) / 256;
     auto after2 = [&]() {
         if (nq2 == 0) return;
-        hipLaunchKernelGGL(k_dcdot<R>, dim3(e.plan.nchunks, (unsigned)ntiles2), dim3(256), 0, st,
-                           e.plan.chunk.template as<int64_t>(), e.X.template as<C>(), Kp, K,
-                           Y.as<double2>(), nq2, nq, cpart.as<double>());
+        timed_launch(h, kTimeDcDot, [&] {
+            hipLaunchKernelGGL(k_dcdot<R>, dim3(e.plan.nchunks, (unsigned)ntiles2), dim3(256), 0,
+                               st, e.plan.chunk.template as<int64_t>(), e.X.template as<C>(), Kp,
+                               K, Y.as<double2>(), nq2, nq, cpart.as<double>());
+        });
         launch_reduce(cpart.as<double>(), (int)(nq2 * 2), e.plan.inst_chunk_dev.template as<int>(),
                       1, cbuf.as<double>(), 0, counter.as<int>(), 0, st);
     };

@@ -216,7 +216,9 @@ struct Engine {
                 t.psi0 = psi0;
                 t.part = part.as<double>();
             }
-            launch_pass<R>(mode, terms, M.real_vals != 0, t, *blk, h->stream);
+            timed_launch(h, kTimePass + 10 * mode + terms, [&] {
+                launch_pass<R>(mode, terms, M.real_vals != 0, t, *blk, h->stream);
+            });
         };
         // v_{m-2}, v_{m-1} and two free buffers; X (v_0) is free after the
         // first pass and Y receives the output (X and Y swap at the end)
@@ -246,7 +248,9 @@ struct Engine {
             a.cc[1] = sp.c[1][m];
             a.psi0 = psi0;
             a.part = part.as<double>();
-            launch_term<R>(F_PREV | F_ACC | F_ACCRMW | epi, a, plan.nchunks, h->stream);
+            timed_launch(h, kTimeTerm, [&] {
+                launch_term<R>(F_PREV | F_ACC | F_ACCRMW | epi, a, plan.nchunks, h->stream);
+            });
             epi_tiles = false;
         } else {
             const int mode = (epi & F_DOT) ? PM_DOTNORM : (epi & F_NORM) ? PM_NORM : PM_LAST;
@@ -312,7 +316,7 @@ struct Engine {
                     a.part = part.as<double>();
                 }
             }
-            launch_term<R>(fl, a, plan.nchunks, h->stream);
+            timed_launch(h, kTimeTerm, [&] { launch_term<R>(fl, a, plan.nchunks, h->stream); });
         }
         std::swap(X, Y);
         parity ^= 1;
@@ -370,7 +374,7 @@ struct Engine {
     void run(int nsteps, const StepPoly& sp, uint64_t bwd, uint64_t cls, int epi,
              const void* psi0, int* counter, F&& after) {
         int done = 0;
-        if (h->use_graph) {
+        if (h->use_graph && !h->timing) {
             const int per = std::min(16, nsteps & ~1);
             if (graph_steps == 0 && per >= 2) capture(per, sp, bwd, cls, epi, psi0, counter, after);
             if (graph_steps > 0 && parity != graph_parity && nsteps > 0) {

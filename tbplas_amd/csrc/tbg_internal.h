@@ -103,7 +103,63 @@ struct tbg_handle {
     // three-term tables of recent batches (key: state row bytes, limits and
     // the instances' ids); a null entry records tiles that do not fit
     std::vector<std::pair<std::string, std::shared_ptr<tbg::Blocking>>> batch_blk;
+    // Per-launch timing (tbg_set_kernel_timing): a pair of events around
+    // every propagation and correlation launch, tagged with the launch kind
+    // (tbg_kernel_times sums them); events are reused from ev_pool.
+    int timing = 0;
+    int timing_hold = 0;      // > 0 inside one-off set-up work (not timed)
+    struct Timed {
+        int kind;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Timed> timed;
+    std::vector<hipEvent_t> ev_pool;
 };
+
+namespace tbg {
+
+// Kinds of timed launches: the one-term kernel, the AC / DC correlation
+// kernels, and the tile pass as kTimePass + 10 * mode + terms.
+enum TimedKind : int { kTimeTerm = 1, kTimeAcCorr = 2, kTimeQe = 3, kTimeDcDot = 4,
+                       kTimePass = 100 };
+
+// Runs launch() on h->stream, bracketed by two events when timing is on.
+template <typename F>
+inline void timed_launch(tbg_handle* h, int kind, F&& launch) {
+    if (!h->timing || h->timing_hold > 0) {
+        launch();
+        return;
+    }
+    auto take = [&]() {
+        hipEvent_t e;
+        if (!h->ev_pool.empty()) {
+            e = h->ev_pool.back();
+            h->ev_pool.pop_back();
+        } else {
+            TBG_HIP(hipEventCreate(&e));
+        }
+        return e;
+    };
+    TBG_REQUIRE(h->timed.size() < (1u << 16),
+                "too many timed launches pending: read them with tbg_kernel_times");
+    hipEvent_t e0 = take(), e1 = take();
+    h->timed.push_back({kind, e0, e1});
+    TBG_HIP(hipEventRecord(e0, h->stream));
+    launch();
+    TBG_HIP(hipEventRecord(e1, h->stream));
+}
+
+// Launches inside its scope are not timed (one-off set-up of a run, such as
+// applying the Fermi operators before the time loop).
+struct TimingHold {
+    tbg_handle* h;
+    explicit TimingHold(tbg_handle* hh) : h(hh) { ++h->timing_hold; }
+    ~TimingHold() { --h->timing_hold; }
+    TimingHold(const TimingHold&) = delete;
+    TimingHold& operator=(const TimingHold&) = delete;
+};
+
+}  // namespace tbg
 
 // Hamiltonian prepared for propagation: sliced-ELL (64-row slices, column
 // major inside a slice), int32 columns, complex values in the working

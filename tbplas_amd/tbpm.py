@@ -114,6 +114,31 @@ class Hamiltonian:
                                                       ctypes.byref(nt)))
         return rows.value, nt.value
 
+    def blocking_bytes(self, states, precision=None):
+        """What one full tile pass must move for ``states`` vectors:
+        {"tables": bytes, "gathered_rows", "streamed_rows", "owned_rows"} and
+        "row_bytes" (slots x element bytes) and "pass_bytes" = tables +
+        row_bytes x (gathered + streamed + 4 owned); None when H runs one
+        term per pass."""
+        prec = self.precision if precision is None else precision_code(precision)
+        parts = np.zeros(4, dtype=np.int64)
+        _lib.check(_lib.lib().tbg_ham_blocking_bytes(self.handle.ptr, self._h, int(states), prec,
+                                                     parts.ctypes.data))
+        if parts[3] == 0:
+            return None
+        slots = 1
+        while slots < states:
+            slots <<= 1
+        p3 = 3
+        while p3 < states:
+            p3 <<= 1
+        slots = min(slots, p3)
+        rb = slots * (8 if prec == _lib.TBG_FP32 else 16)
+        tab, gat, stream, own = (int(x) for x in parts)
+        return {"tables": tab, "gathered_rows": gat, "streamed_rows": stream,
+                "owned_rows": own, "row_bytes": rb,
+                "pass_bytes": tab + rb * (gat + stream + 4 * own)}
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.lib().tbg_ham_destroy(self._h)
@@ -404,6 +429,20 @@ class DosRun:
         _lib.check(_lib.lib().tbg_run_advance(self._r, int(nsteps), ctypes.byref(ms)))
         self.done += int(nsteps)
         return ms.value
+
+    def blocking_bytes(self):
+        """Hamiltonian.blocking_bytes for the tables this run uses (a
+        batch's concatenated tiles included); None without tile passes."""
+        parts = np.zeros(4, dtype=np.int64)
+        rb = ctypes.c_int(0)
+        _lib.check(_lib.lib().tbg_run_blocking_bytes(self._r, parts.ctypes.data,
+                                                     ctypes.byref(rb)))
+        if parts[3] == 0:
+            return None
+        tab, gat, stream, own = (int(x) for x in parts)
+        return {"tables": tab, "gathered_rows": gat, "streamed_rows": stream,
+                "owned_rows": own, "row_bytes": rb.value,
+                "pass_bytes": tab + rb.value * (gat + stream + 4 * own)}
 
     def result(self, chk_step=128, chk_thr=1e-9):
         torch = self._torch

@@ -1,0 +1,167 @@
+"""The benchmark configurations at their own sizes and layouts, against the
+oracle.
+
+* 2048x2048 graphene (8,388,608 orbitals), 8 states, 2048 steps, rows in
+  5.0 nm tiles with four Chebyshev terms per pass (the layout the benchmark
+  times), complex64 and complex128: C(t) at sampled steps over the whole run
+  against exact k-space propagation.  Tolerances: complex128 1e-10;
+  complex64 (FP32 storage and arithmetic, FP64 sums) 1e-6, about ten times
+  the largest error measured over 2048 steps.
+* The disordered, field-threaded DC sample (1% vacancies, Anderson disorder
+  U[-0.5, 0.5] eV, B = 20 T in the Landau gauge, 16 states, 5 energies,
+  2.0 nm tiles with three and four terms per pass) on 24x24 cells against
+  the C restatement of tbpm_dccond, and at 1024x1024 (2,076,180 orbitals)
+  through properties that hold at any size: the DC run's DOS correlation
+  equals a DOS run's C(t) on the same states, and the quasi-eigenstates are
+  normalised.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import tbplas_amd as tb
+from tbplas_amd import tbpm
+from conftest import graphene_sample, tau_of
+
+pytestmark = pytest.mark.gpu
+
+L2048 = 2048
+STEPS = 2048
+# sampled steps: the first few, every 128th and the last
+PTS = sorted(set([0, 1, 2, 3, 4, 5, 17, 21, 100, 511, 1023, 1025, 2047, STEPS]
+                 + list(range(0, STEPS + 1, 128))))
+
+
+def _states(n, K, seed=1337):
+    return tb.states.random_states(n, tb.states.sample_seeds(seed, K))
+
+
+@pytest.fixture(scope="module")
+def graphene2048():
+    s = graphene_sample(L2048)
+    ip, ix, dt = s.build_ham_csr_device()
+    order, tiles = s.tile_order(5.0, return_tiles=True)
+    bes = tbpm.bessel_series(tau_of(s), 1e-15)
+    psi = _states(s.num_orb, 8)
+    exact = oracle.kspace_graphene_corr_multi(L2048, L2048, s.hop_v[0], psi, tau_of(s), PTS)
+    return s, (ip, ix, dt), (order, tiles), bes, psi, exact
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("fp64", 1e-10)])
+def test_benchmark_layout_2048_steps_vs_kspace(gpu, graphene2048, prec, tol):
+    """The benchmark's kernel configuration over the whole 2048-step run:
+    every pass is a four-term tile pass (terms 1-4 from v_0, three full
+    passes, terms 17-20 with the correlation sums per tile), 16-wave
+    workgroups for complex64; the output partials of owned rows beyond the
+    register queue go through memory (complex128: two held per lane, up to
+    eight owned rows per lane)."""
+    s, (ip, ix, dt), (order, tiles), bes, psi, exact = graphene2048
+    assert len(bes) == 21
+    h = tb._lib.Handle(0)
+    try:
+        h.set_tuning(8, True)
+        h.set_blocking(True, terms=4)
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec, handle=h, order=order,
+                             tiles=tiles)
+        assert H.info()["real_values"]
+        nb, halo = H.blocking_info(8)
+        rows, nt = H.blocking_launch(8)
+        assert nb > 4000 and 1.0 < halo < 1.6
+        if prec == "fp32":
+            assert nt == 1024
+        c = tbpm.corr_dos(H, bes, psi, STEPS, chk_thr=1e-4)
+        H.close()
+    finally:
+        h.close()
+    err = float(np.max(np.abs(c[:, PTS] - exact)))
+    print(f"{prec} K=8 2048x2048, 5.0 nm tiles, four terms per pass, {nb} tiles "
+          f"(widest {rows} rows, {nt} threads): max |C - C_exact| over {STEPS} steps = "
+          f"{err:.3e}", flush=True)
+    assert err < tol
+    # the norm stays one (C(0) is the squared norm of the state)
+    assert np.max(np.abs(c[:, 0] - 1.0)) < 1e-12
+
+
+def _cfg3_sample(L, seed=2024):
+    s = tb.make_graphene_sample(L, vacancies=tb.random_vacancies(2 * L * L, 0.01, seed))
+    tb.add_anderson_disorder(s, 1.0, seed + 1)
+    s.set_magnetic_field(20.0, 0)
+    s.rescale_ham(9.0)
+    return s
+
+
+def _en_grid(s, T):
+    en = np.array([0.5 * i * s.energy_range / T - s.energy_range / 2. for i in range(2 * T)])
+    inds = np.array([int(np.argmin(np.abs(en - e))) for e in (-1.0, -0.5, 0.0, 0.5, 1.0)])
+    return en, inds
+
+
+@pytest.mark.parametrize("terms", [3, 4])
+def test_cfg3_dc_vs_oracle(gpu, terms):
+    """1% vacancies, Anderson W = 1 eV, B = 20 T, 16 states (32 propagated
+    vectors), 5 energies in [-1, 1] eV, 2.0 nm tiles: complex128 to 1e-10
+    and complex64 to 2e-4 (relative to the largest correlation) against
+    orc_tbpm_dccond over 40 steps."""
+    s = _cfg3_sample(24)
+    assert np.any(s.hop_v.imag != 0)
+    ip, ix, hop, dx, dy = oracle.build_ham_dxy_loop(s.orb_eng, s.hop_i, s.hop_j, s.hop_v, s.dr)
+    order, tiles = s.tile_order(2.0, return_tiles=True)
+    T, K = 40, 16
+    ts = 2 * math.pi / s.energy_range
+    bes = tbpm.bessel_series(ts * s.rescale, 1e-14)
+    en, inds = _en_grid(s, T)
+    psi = _states(s.num_orb, K, seed=77)
+    rd, rc = oracle.orc_tbpm_dccond(ip, ix, hop, dx, dy, s.rescale, bes, psi, T, ts, en, inds)
+    h = tb._lib.Handle(0)
+    try:
+        h.set_blocking(True, terms=terms)
+        for prec, tol in (("fp64", 1e-10), ("fp32", 2e-4)):
+            H = tbpm.Hamiltonian(s.num_orb, ip, ix, hop, dx, dy, cur_scale=s.rescale,
+                                 precision=prec, handle=h, order=order, tiles=tiles)
+            assert not H.info()["real_values"]
+            nb = H.blocking_info(2 * K)[0]
+            # complex64: 32 vectors of 8 bytes a row fit tile passes at
+            # 2.0 nm; complex128 may fall back to one term per pass
+            assert nb > 1 or prec == "fp64"
+            d, c = tbpm.corr_dc(H, bes, psi, T, ts, en, inds, chk_thr=1e-4)
+            H.close()
+            scale = max(1.0, float(np.max(np.abs(rc))))
+            assert np.max(np.abs(d - rd)) < tol, prec
+            assert np.max(np.abs(c - rc)) < tol * scale, prec
+    finally:
+        h.close()
+
+
+def test_cfg3_full_size_properties(gpu):
+    """1024x1024 with 1% vacancies (2,076,180 orbitals), Anderson W = 1 eV,
+    B = 20 T, complex64, 2.0 nm tiles: the DOS correlation of the DC run
+    (t = 1..T) equals the DOS run's C(t) on the same 16 states, and every
+    quasi-eigenstate has unit norm."""
+    s = _cfg3_sample(1024)
+    assert s.num_orb == 2076180
+    ip, ix, hop, dx, dy = s.build_ham_dxy_device()
+    order, tiles = s.tile_order(2.0, return_tiles=True)
+    T, K = 256, 16
+    ts = 2 * math.pi / s.energy_range
+    bes = tbpm.bessel_series(ts * s.rescale, 1e-14)
+    en, inds = _en_grid(s, T)
+    psi = _states(s.num_orb, K, seed=5)
+    h = tb._lib.Handle(0)
+    try:
+        h.set_blocking(True, terms=4)
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, hop, dx, dy, cur_scale=s.rescale,
+                             precision="fp32", handle=h, order=order, tiles=tiles)
+        d, c = tbpm.corr_dc(H, bes, psi, T, ts, en, inds, chk_thr=1e-4)
+        assert np.all(np.isfinite(c))
+        cd = tbpm.corr_dos(H, bes, psi, T, chk_thr=1e-4)
+        err = float(np.max(np.abs(d - cd[:, 1:])))
+        print(f"cfg3 full size: max |dos_corr(DC) - C_DOS(t)| = {err:.3e}", flush=True)
+        assert err < 1e-6
+        qe = tbpm.quasi_eigenstates(H, bes, psi[:2], 64, ts, en[inds], chk_thr=1e-4)
+        H.close()
+    finally:
+        h.close()
+    norms = qe.sum(axis=2)
+    assert np.max(np.abs(norms - 1.0)) < 1e-10
