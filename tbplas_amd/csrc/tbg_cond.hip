@@ -254,8 +254,9 @@ k_qe_stats(const int64_t* __restrict__ chunk, const double2* __restrict__ qe, in
     sh[threadIdx.x * 3 + 1] = di;
     sh[threadIdx.x * 3 + 2] = nn;
     __syncthreads();
-    if (threadIdx.x < nqt * 3) {
-        const int q2 = threadIdx.x / 3, k = threadIdx.x % 3;
+    // nqt * 3 sums (up to 768) over 256 threads
+    for (int o = threadIdx.x; o < nqt * 3; o += blockDim.x) {
+        const int q2 = o / 3, k = o % 3;
         double acc = 0.0;
         for (int g = 0; g < ng; ++g) acc += sh[(g * nqt + q2) * 3 + k];
         part[((int64_t)c * nq + qb + q2) * 3 + k] = acc;
@@ -361,8 +362,9 @@ k_dcdot(const int64_t* __restrict__ chunk, const typename CplxT<R>::T* __restric
     sh[threadIdx.x * 2 + 0] = dr;
     sh[threadIdx.x * 2 + 1] = di;
     __syncthreads();
-    if (threadIdx.x < nqt * 2) {
-        const int qq = threadIdx.x / 2, k = threadIdx.x % 2;
+    // nqt * 2 sums (up to 512) over 256 threads
+    for (int o = threadIdx.x; o < nqt * 2; o += blockDim.x) {
+        const int qq = o / 2, k = o % 2;
         double acc = 0.0;
         for (int g = 0; g < ng; ++g) acc += sh[(g * nqt + qq) * 2 + k];
         part[((int64_t)c * nq2 + qb + qq) * 2 + k] = acc;

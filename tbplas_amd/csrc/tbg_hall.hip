@@ -7,12 +7,12 @@
 // T_i(H~) J_x psi (i < N) and mu(i, j) = g_i g_j <P_i | W_j> with the Jackson
 // weights g (kpm.f90:8-29).  The Chebyshev vectors come from the term
 // kernel (T_j = 2 H~ T_{j-1} - T_{j-2}, in place over T_{j-2}); the
-// currents from the slice-ELL SpMV; the N x N Gram matrix P^H W is a plain
-// complex128 GEMM (rocBLAS zgemm), issued per block of 256 rows of mu while
-// the next block of P is formed.  Complex128 only: the moments are sums of
-// N^2 products that complex64 would leave at ~1e-7 of their size.
-#include <rocblas/rocblas.h>
-
+// currents from the slice-ELL SpMV; the N x N Gram matrix P^H W is formed
+// per block of up to 256 rows of mu on the FP64 matrix cores (k_gram: a
+// complex product as four real 16x16x4 MFMAs, the long n dimension split
+// over workgroups and summed in a fixed order by k_gram_sum).  Complex128
+// only: the moments are sums of N^2 products that complex64 would leave at
+// ~1e-7 of their size.
 #include <cmath>
 #include <vector>
 
@@ -34,20 +34,109 @@ __global__ void k_jackson_scale(double2* __restrict__ mu, int N, const double* _
     mu[t] = v;
 }
 
-struct Blas {
-    rocblas_handle b = nullptr;
-    explicit Blas(hipStream_t s) {
-        if (rocblas_create_handle(&b) != rocblas_status_success)
-            throw Error{TBG_EHIP, "rocblas_create_handle failed"};
-        if (rocblas_set_stream(b, s) != rocblas_status_success) {
-            rocblas_destroy_handle(b);
-            throw Error{TBG_EHIP, "rocblas_set_stream failed"};
+// Gram matrix of the Hall moments on the FP64 matrix cores:
+// part[z][m][j] = sum_{k in split z} conj(P[m][k]) W[j][k] for m < rows,
+// j < N, with P and W column-major (column m of P at P + m * np).  A
+// workgroup (4 waves) owns a 64 x 64 tile of the output and one split of
+// k; it stages 16 rows of k of both operands in LDS (real and imaginary
+// parts apart) and each wave accumulates a 32 x 32 sub-tile as 2 x 2 blocks
+// of v_mfma_f64_16x16x4f64: re += Pr Wr + Pi Wi, im += Pr Wi - Pi Wr.
+// Operand layout of the instruction: lane i + 16 k holds A[i][k] and
+// B[k][i]; the result D[i][j] sits in lane j + 16 (i % 4), element i / 4
+// (checked on the device with known operands).
+constexpr int kGramTile = 64;   // output rows / columns per workgroup
+constexpr int kGramK = 16;      // rows of k per LDS stage
+constexpr int kGramLd = 80;     // LDS row stride in doubles: the MFMA operand
+                                // reads of rows k and k + 1 land 32 banks apart
+typedef double gram_d4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256)
+k_gram(const double2* __restrict__ P, const double2* __restrict__ W, int64_t np, int rows,
+       int N, int64_t kchunk, double2* __restrict__ part) {
+    __shared__ double lds[4][kGramK][kGramLd];   // Pr, Pi, Wr, Wi
+    const int m0 = blockIdx.y * kGramTile, j0 = blockIdx.x * kGramTile;
+    const int64_t kb = (int64_t)blockIdx.z * kchunk;
+    const int64_t ke = kb + kchunk < np ? kb + kchunk : np;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    gram_d4 cre[2][2], cim[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            cre[a][b] = gram_d4{0.0, 0.0, 0.0, 0.0};
+            cim[a][b] = gram_d4{0.0, 0.0, 0.0, 0.0};
         }
+    const int sk = t & 15;
+    for (int64_t k0 = kb; k0 < ke; k0 += kGramK) {
+        const int64_t k = k0 + sk;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int sm = (t >> 4) + 16 * q;
+            double2 p = make_double2(0.0, 0.0), x = make_double2(0.0, 0.0);
+            if (k < ke && m0 + sm < rows) p = P[(int64_t)(m0 + sm) * np + k];
+            if (k < ke && j0 + sm < N) x = W[(int64_t)(j0 + sm) * np + k];
+            lds[0][sk][sm] = p.x;
+            lds[1][sk][sm] = p.y;
+            lds[2][sk][sm] = x.x;
+            lds[3][sk][sm] = x.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGramK; kk += 4) {
+            double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                ar[a] = lds[0][kk + lk][wm + 16 * a + li];
+                ai[a] = lds[1][kk + lk][wm + 16 * a + li];
+                br[a] = lds[2][kk + lk][wn + 16 * a + li];
+                bi[a] = lds[3][kk + lk][wn + 16 * a + li];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    cre[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cre[a][b], 0,
+                                                                     0, 0);
+                    cre[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bi[b], cre[a][b], 0,
+                                                                     0, 0);
+                    cim[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], cim[a][b], 0,
+                                                                     0, 0);
+                    cim[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], br[b], cim[a][b],
+                                                                     0, 0, 0);
+                }
+        }
+        __syncthreads();
     }
-    ~Blas() {
-        if (b) rocblas_destroy_handle(b);
+    double2* out = part + (int64_t)blockIdx.z * rows * N;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = m0 + wm + 16 * a + lk + 4 * v;
+                const int j = j0 + wn + 16 * b + li;
+                if (m < rows && j < N)
+                    out[(int64_t)m * N + j] = make_double2(cre[a][b][v], cim[a][b][v]);
+            }
+}
+
+// mu[m + j * ld] = sum_z part[z][m][j] in the order z = 0, 1, ...
+__global__ void k_gram_sum(const double2* __restrict__ part, int nsplit, int rows, int N,
+                           double2* __restrict__ mu, int ld) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)rows * N) return;
+    const int m = (int)(q / N), j = (int)(q % N);
+    double re = 0.0, im = 0.0;
+    for (int z = 0; z < nsplit; ++z) {
+        const double2 v = part[(int64_t)z * rows * N + q];
+        re += v.x;
+        im += v.y;
     }
-};
+    mu[(int64_t)j * ld + m] = make_double2(re, im);
+}
 
 // Haydock step i on one vector (Kp = 1): buf -= a_i cur + b_{i-1} prev with
 // a_i = dots[3i] + i dots[3i+1] and b_{i-1} = sqrt(nrm[3(i-1)+2]) read on
@@ -170,8 +259,13 @@ void hall_mu_impl(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, in
         a.beta = -1.0;
         launch_term<double>(first ? F_STORE : (F_PREV | F_STORE), a, e.plan.nchunks, st);
     };
-    Blas blas(st);
-    const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+    // k split of the Gram blocks: enough workgroups for two per CU
+    const int tiles = ((B + kGramTile - 1) / kGramTile) * ((N + kGramTile - 1) / kGramTile);
+    int64_t nsplit = std::max<int64_t>(1, (2 * (int64_t)h->num_cus + tiles - 1) / tiles);
+    nsplit = std::min<int64_t>(nsplit, std::max<int64_t>(1, np / (4 * kGramK)));
+    const int64_t kchunk = ((np + nsplit - 1) / nsplit + kGramK - 1) / kGramK * kGramK;
+    nsplit = (np + kchunk - 1) / kchunk;
+    DevBuf gpart((size_t)nsplit * B * N * sizeof(double2));
     for (int s = 0; s < K; ++s) {
         launch_fill_zero(psi.p, vb, st);
         launch_pack<double>(psi0 + (size_t)s * n * 2, 1, n, psi.p, 1, 0, H->perm_ptr(), st);
@@ -211,15 +305,18 @@ void hall_mu_impl(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, in
             p1 = pi;
             if (i % B == B - 1 || i == N - 1) {
                 const int i0 = i - i % B, rows = i % B + 1;
-                if (rocblas_zgemm(blas.b, rocblas_operation_conjugate_transpose,
-                                  rocblas_operation_none, rows, N, (rocblas_int)np, &one,
-                                  reinterpret_cast<const rocblas_double_complex*>(Pb.p),
-                                  (rocblas_int)np,
-                                  reinterpret_cast<const rocblas_double_complex*>(W.p),
-                                  (rocblas_int)np, &zero,
-                                  reinterpret_cast<rocblas_double_complex*>(mus) + i0, N) !=
-                    rocblas_status_success)
-                    throw Error{TBG_EHIP, "rocblas_zgemm failed"};
+                const dim3 grid((unsigned)((N + kGramTile - 1) / kGramTile),
+                                (unsigned)((rows + kGramTile - 1) / kGramTile), (unsigned)nsplit);
+                hipLaunchKernelGGL(k_gram, grid, dim3(256), 0, st,
+                                   reinterpret_cast<const double2*>(Pb.p),
+                                   reinterpret_cast<const double2*>(W.p), np, rows, N, kchunk,
+                                   gpart.as<double2>());
+                TBG_HIP(hipGetLastError());
+                const int64_t nq = (int64_t)rows * N;
+                hipLaunchKernelGGL(k_gram_sum, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                                   st, gpart.as<double2>(), (int)nsplit, rows, N,
+                                   reinterpret_cast<double2*>(mus) + i0, N);
+                TBG_HIP(hipGetLastError());
             }
         }
         const int64_t nn = (int64_t)N * N;

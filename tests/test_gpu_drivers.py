@@ -127,6 +127,23 @@ def test_quasi_eigenstates_vs_oracle(gpu, prec, layout):
     H.close()
 
 
+def test_quasi_eigenstates_many_states_and_energies(gpu):
+    """12 states x 9 energies = 108 quasi-eigenstates: their norms and
+    overlaps are summed in tiles of up to 256 per workgroup, more sums
+    than a workgroup has threads."""
+    s = _sample(L=8)
+    H, (ip, ix, dt) = _ham(s, "fp64", "caller")
+    t_step = 2 * math.pi / s.energy_range
+    bes = tbpm.bessel_series(t_step * s.rescale, 1e-14)
+    psi = states.random_states(s.num_orb, states.sample_seeds(19, 12))
+    T, en = 24, list(np.linspace(-2.0, 2.0, 9))
+    g = tbpm.quasi_eigenstates(H, bes, psi, T, t_step, en)
+    r = oracle.orc_quasi_eigenstates(ip, ix, dt, bes, psi, T, t_step, en)
+    assert np.max(np.abs(g - r)) < 1e-11 * max(1.0, np.max(r))
+    assert np.allclose(g.sum(axis=2), 1.0, atol=1e-12)
+    H.close()
+
+
 def _config(T, S, precision="fp64"):
     c = tb.Config()
     c.generic["nr_time_steps"] = T
@@ -256,6 +273,19 @@ def test_hall_mu_vs_oracle(gpu, N, field, direction):
     g = tbpm.hall_mu(H, psi, N, direction)
     r = oracle.orc_tbpm_kbdc(ip, ix, dt, dx, dy, s.rescale, psi, N, direction)
     assert g.shape == (2, N, N)
+    assert np.max(np.abs(g - r)) < 1e-12 * max(1.0, np.max(np.abs(r)))
+    H.close()
+
+
+def test_hall_mu_gram_split_over_workgroups(gpu):
+    """40x40 cells (3,136 orbitals): the Gram matrix's n dimension is split over ~50
+    workgroups per output tile (fixed-order sum), 96 moments (a partial
+    64-wide tile in both directions)."""
+    s = _sample(L=40, field=12.0)
+    H, (ip, ix, dt, dx, dy) = _dxy_ham(s)
+    psi = states.random_states(s.num_orb, states.sample_seeds(23, 1))
+    g = tbpm.hall_mu(H, psi, 96, 1)
+    r = oracle.orc_tbpm_kbdc(ip, ix, dt, dx, dy, s.rescale, psi, 96, 1)
     assert np.max(np.abs(g - r)) < 1e-12 * max(1.0, np.max(np.abs(r)))
     H.close()
 

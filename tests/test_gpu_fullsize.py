@@ -68,9 +68,13 @@ def test_benchmark_layout_2048_steps_vs_kspace(gpu, graphene2048, prec, tol):
         assert H.info()["real_values"]
         nb, halo = H.blocking_info(8)
         rows, nt = H.blocking_launch(8)
-        assert nb > 4000 and 1.0 < halo < 1.6
+        assert nb > 4000
         if prec == "fp32":
-            assert nt == 1024
+            # 64-byte rows: whole 5.0 nm tiles fit the LDS
+            assert nt == 1024 and 1.0 < halo < 1.4
+        else:
+            # 128-byte rows: the tiles are split to fit
+            assert 1.0 < halo < 2.0
         c = tbpm.corr_dos(H, bes, psi, STEPS, chk_thr=1e-4)
         H.close()
     finally:
@@ -80,8 +84,8 @@ def test_benchmark_layout_2048_steps_vs_kspace(gpu, graphene2048, prec, tol):
           f"(widest {rows} rows, {nt} threads): max |C - C_exact| over {STEPS} steps = "
           f"{err:.3e}", flush=True)
     assert err < tol
-    # the norm stays one (C(0) is the squared norm of the state)
-    assert np.max(np.abs(c[:, 0] - 1.0)) < 1e-12
+    # C(0) is the squared norm of the state as stored (complex64 rounds it)
+    assert np.max(np.abs(c[:, 0] - 1.0)) < (1e-9 if prec == "fp32" else 1e-12)
 
 
 def _cfg3_sample(L, seed=2024):
