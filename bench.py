@@ -62,6 +62,9 @@ METRIC = "TBPM step·vectors/s + % HBM roofline, 8.4M-orb graphene; C(t) err vs 
 ACC_STEPS = 2048
 EXACT_PTS = sorted(set([0, 1, 2, 3, 4, 5, 17, 21, 100, 511, 1023, 1025, 2047, ACC_STEPS]
                        + list(range(0, ACC_STEPS + 1, 128))))
+# the AC accuracy run and its sampled steps (t >= 2; column t = 1 is zero)
+AC_ACC_STEPS = 256
+AC_PTS = [2, 3, 4, 17, 64, 128, 255, AC_ACC_STEPS]
 WORKLOADS = ("all", "dos", "ac", "dc", "batch", "latency", "assembly")
 
 
@@ -190,17 +193,25 @@ def cpu_child(args):
 
 
 def exact_child(args):
-    """C(t) of the headline states by exact k-space propagation (pristine
-    periodic graphene is block diagonal in k)."""
+    """DOS and AC correlations of the headline states by exact k-space
+    propagation (pristine periodic graphene is block diagonal in k; the AC
+    Fermi operators are the same Chebyshev series, summed on the band
+    energies)."""
     import numpy as np
     import oracle
     from tbplas_amd import states
     L, K = args.L, args.vectors
     psi = states.random_states(2 * L * L, states.sample_seeds(1337, K))
-    hop = -2.7 * (1.0 / 9.0)
-    c = oracle.kspace_graphene_corr_multi(L, L, hop, psi, math.pi, EXACT_PTS)
-    print(json.dumps({"pts": EXACT_PTS, "re": np.real(c).tolist(), "im": np.imag(c).tolist()}),
-          flush=True)
+    hi, hj, hv, dr, pos = oracle.graphene_hops_vec(L, L)
+    hv = hv * (1.0 / 9.0)
+    c = oracle.kspace_graphene_corr_multi(L, L, hv[0], psi, math.pi, EXACT_PTS)
+    beta = 1.0 / (8.617333262e-5 * 300.0) * 9.0
+    cF = oracle.orc_fermi_coef(2 ** 15, beta, 0.0, False, 1e-12)
+    cO = oracle.orc_fermi_coef(2 ** 15, beta, 0.0, True, 1e-12)
+    a = oracle.kspace_graphene_ac(L, L, hv[:3], dr[:3], 9.0, cF, cO, psi, math.pi / 2, AC_PTS)
+    print(json.dumps({"pts": EXACT_PTS, "re": np.real(c).tolist(), "im": np.imag(c).tolist(),
+                      "ac_pts": AC_PTS, "ac_re": np.real(a).tolist(),
+                      "ac_im": np.imag(a).tolist()}), flush=True)
 
 
 def cpu_baseline(args):
@@ -257,6 +268,7 @@ def start_exact(args):
 
 
 def finish_exact(proc):
+    """{"dos": (steps, (K, steps) exact C(t)), "ac": (steps, (K, 4, steps))}."""
     import numpy as np
     out, err = proc.communicate(timeout=900)
     lines = [ln for ln in out.splitlines() if ln.startswith('{"pts"')]
@@ -264,7 +276,8 @@ def finish_exact(proc):
         log("exact k-space correlations failed:", err[-2000:])
         return None
     d = json.loads(lines[-1])
-    return d["pts"], np.array(d["re"]) + 1j * np.array(d["im"])
+    return {"dos": (d["pts"], np.array(d["re"]) + 1j * np.array(d["im"])),
+            "ac": (d["ac_pts"], np.array(d["ac_re"]) + 1j * np.array(d["ac_im"]))}
 
 
 # ---------------------------------------------------------------------------
@@ -516,21 +529,20 @@ def headline(args, cpu, exact_proc):
     tmax = min(r32["corr"].shape[1], r64["corr"].shape[1])
     out["ct_err_fp32_vs_fp64"] = float(np.max(np.abs(r32["corr"][:, :tmax] -
                                                      r64["corr"][:, :tmax])))
-    if exact_proc is not None:
-        ex = finish_exact(exact_proc)
-        if ex is not None:
-            pts, cex = ex
-            pts = [p for p in pts if p < tmax]
-            k = len(pts)
-            out["ct_err_fp32_vs_exact"] = float(np.max(np.abs(r32["corr"][:, pts] - cex[:, :k])))
-            out["ct_err_fp64_vs_exact"] = float(np.max(np.abs(r64["corr"][:, pts] - cex[:, :k])))
-            out["ct_err_exact_steps"] = {"states": K, "run_steps": tmax - 1,
-                                         "sampled_steps": len(pts), "last": pts[-1]}
-            log(f"max |C - C_exact| over {tmax - 1} steps: fp32 "
-                f"{out['ct_err_fp32_vs_exact']:.3e}, fp64 {out['ct_err_fp64_vs_exact']:.3e}")
+    ex = finish_exact(exact_proc) if exact_proc is not None else None
+    if ex is not None:
+        pts, cex = ex["dos"]
+        pts = [p for p in pts if p < tmax]
+        k = len(pts)
+        out["ct_err_fp32_vs_exact"] = float(np.max(np.abs(r32["corr"][:, pts] - cex[:, :k])))
+        out["ct_err_fp64_vs_exact"] = float(np.max(np.abs(r64["corr"][:, pts] - cex[:, :k])))
+        out["ct_err_exact_steps"] = {"states": K, "run_steps": tmax - 1,
+                                     "sampled_steps": len(pts), "last": pts[-1]}
+        log(f"max |C - C_exact| over {tmax - 1} steps: fp32 "
+            f"{out['ct_err_fp32_vs_exact']:.3e}, fp64 {out['ct_err_fp64_vs_exact']:.3e}")
     if cpu is not None:
         out["cpu_baseline"] = cpu
-    return out
+    return out, ex
 
 
 # ---------------------------------------------------------------------------
@@ -540,7 +552,7 @@ def cond_order(s, tile):
     return s.tile_order(tile, return_tiles=True) if tile > 0 else (None, None)
 
 
-def ac_workload(args):
+def ac_workload(args, exact=None):
     """Pristine graphene 2048x2048 (rescale 9), AC conductivity correlation
     of 8 states at 300 K, mu = 0: the Fermi and one-minus-Fermi polynomials
     (FFT coefficients at 2^15 points, precision 1e-12) applied once, then
@@ -594,6 +606,12 @@ def ac_workload(args):
                 kt = h.kernel_times()
             finally:
                 h.set_kernel_timing(False)
+        acc = None
+        if exact is not None and L == args.L and K == exact[1].shape[0]:
+            # a 256-step run against the exact k-space correlations
+            pts, cex = exact
+            ca = tbpm.corr_ac(H, bes, cF, cO, psi, AC_ACC_STEPS, chk_thr=1e-4)
+            acc = float(np.max(np.abs(ca[:, :, [t - 1 for t in pts]] - cex)))
         H.close()
         t_prop = (t2 - t1) - (t1 - t0)
         sz = 8 if prec == "fp32" else 16
@@ -608,8 +626,9 @@ def ac_workload(args):
                  kernel_label(k, prec, 3 * K, real, threads))
         res[prec] = {"value": K * (T - 1) / t_prop, "ms_per_step": t_prop / (T - 1) * 1e3,
                      "fermi_setup_s": t1 - t0, "shape": shape, "corr": c,
-                     "roofline": roofline(kt, bytes_of, label, f"ac_{prec}") if kt else None}
-        log(f"ac {prec}: {res[prec]['value']:.1f} state·steps/s")
+                     "roofline": roofline(kt, bytes_of, label, f"ac_{prec}") if kt else None,
+                     "err_vs_exact": acc}
+        log(f"ac {prec}: {res[prec]['value']:.1f} state·steps/s, max |C - C_exact| {acc}")
     err = float(np.max(np.abs(res["fp32"]["corr"] - res["fp64"]["corr"])))
     scale = float(np.max(np.abs(res["fp64"]["corr"])))
     r = res["fp32"]
@@ -628,7 +647,11 @@ def ac_workload(args):
             "fp64": {"value": round(res["fp64"]["value"], 2),
                      "ms_per_step": round(res["fp64"]["ms_per_step"], 4),
                      "roofline": res["fp64"]["roofline"]},
-            "corr_err_fp32_vs_fp64": err, "corr_max_abs": scale, "corr_err_steps": T}
+            "corr_err_fp32_vs_fp64": err, "corr_max_abs": scale, "corr_err_steps": T,
+            "ct_err_fp32_vs_exact": res["fp32"]["err_vs_exact"],
+            "ct_err_fp64_vs_exact": res["fp64"]["err_vs_exact"],
+            "ct_err_exact_steps": {"run_steps": AC_ACC_STEPS, "sampled": AC_PTS,
+                                   "states": K} if exact is not None else None}
 
 
 def dc_workload(args):
@@ -936,11 +959,11 @@ def main():
         return 0
     cpu = None if args.no_cpu else cpu_baseline(args)
     exact = start_exact(args) if args.accuracy_steps >= ACC_STEPS else None
-    out = headline(args, cpu, exact)
+    out, ex = headline(args, cpu, exact)
     if args.workload == "all":
         for name, fn in subs.items():
             t0 = time.perf_counter()
-            out[name] = fn(args)
+            out[name] = fn(args, ex["ac"] if ex is not None else None) if name == "ac" else fn(args)
             log(f"{name}: {out[name]['value']} {out[name]['unit']} "
                 f"({time.perf_counter() - t0:.1f} s)")
     print(json.dumps(out), flush=True)

@@ -19,6 +19,7 @@ __all__ = [
     "bessel_series", "build_ham_csr", "build_ham_dxy_loop",
     "build_ham_dxy_stable", "graphene_hops_loop", "graphene_hops_vec", "get_rescale_loop",
     "set_mag_field_loop", "kspace_graphene_corr", "kspace_graphene_corr_multi", "dense_corr",
+    "kspace_graphene_ac", "cheb_series_values",
     "analyzer_calc_dos", "analyzer_calc_ac_cond", "analyzer_calc_dc_cond",
     "analyzer_calc_diff_coeff",
     "window_hanning", "window_exp",
@@ -1100,6 +1101,84 @@ def kspace_graphene_corr_multi(L1, L2, hop, psi0s, tau, steps, fwd=True):
         z = np.exp(1j * (sgn * tau * k) * e)
         out[:, c] = wu @ z + wd @ np.conj(z)
     return out
+
+
+def cheb_series_values(coef, x):
+    """sum_n coef[n] T_n(x) for |x| <= 1 by Clenshaw's recurrence (the
+    polynomial that Fermi, propagation.f90:52-82, applies to H~)."""
+    x = np.asarray(x, dtype=np.float64)
+    b1 = np.zeros_like(x)
+    b2 = np.zeros_like(x)
+    for c in coef[:0:-1]:
+        b1, b2 = c + 2.0 * x * b1 - b2, b1
+    return coef[0] + x * b1 - b2
+
+
+def kspace_graphene_ac(L1, L2, hops, drs, rescale, coef_F, coef_omF, psi0s, tau, steps):
+    """Exact tbpm_accond correlations (tbpm.f90:193-351) of pristine periodic
+    graphene, per state: (S, 4, len(steps)) for the entries
+    <psi2|J_x psi1_x>, <psi2|J_y psi1_x>, <psi2|J_x psi1_y>, <psi2|J_y psi1_y>
+    at the listed time indices t >= 2, with psi1_b = (1 - F)(H~) J_b psi0,
+    psi2 = F(H~) psi0, both propagated t - 1 steps of exp(-i H~ tau), F the
+    Chebyshev series of the Fermi coefficients (evaluated exactly on the
+    band energies) and J_a = i rescale H~ d_a (funcs.f90:12-33).
+
+    ``hops`` / ``drs``: the rescaled values and bond vectors of the three
+    hops of cell (0, 0) in graphene_hops_loop order: orbital 0 -> 1 in the
+    cell, 1 -> 0 of cell +a1, 1 -> 0 of cell +a2.  Orbital order
+    (ia*L2+ib)*2+io.  In k the blocks are H~ = [[0, f], [f*, 0]] and
+    J_a = [[0, g_a], [g_a*, 0]]; in the band basis v_s = (1, s conj(f/|f|))
+    / sqrt2 the correlation is a sum over s, s' of F(s|f|) (1 - F)(s'|f|)
+    <v_s|psi0>* <v_s|J_a|v_s'> <v_s'|J_b psi0> e^{i (s - s')|f| tau (t - 1)},
+    with 1/N from Parseval (numpy's unnormalised FFT)."""
+    h0, h1, h2 = [complex(h) for h in hops]
+    d0, d1, d2 = [np.asarray(d, dtype=np.float64) for d in drs]
+    psi0s = np.asarray(psi0s).reshape(-1, L1, L2, 2)
+    S, N = psi0s.shape[0], L1 * L2
+    k1 = 2 * np.pi * np.fft.fftfreq(L1)[:, None]
+    k2 = 2 * np.pi * np.fft.fftfreq(L2)[None, :]
+    z1, z2 = np.exp(-1j * k1), np.exp(-1j * k2)
+    f = (h0 + np.conj(h1) * z1 + np.conj(h2) * z2).ravel()
+    g = [(1j * rescale * (h0 * d0[a] - np.conj(h1) * d1[a] * z1
+                          - np.conj(h2) * d2[a] * z2)).ravel() for a in (0, 1)]
+    e = np.abs(f)
+    ph = np.where(e > 0, f / np.where(e > 0, e, 1), 1.0)
+    # F and 1 - F on the band energies +-e: Clenshaw once per distinct value
+    ev, inv = np.unique(np.round(e, 14), return_inverse=True)
+    Fp = cheb_series_values(coef_F, ev)[inv]
+    Fm = cheb_series_values(coef_F, -ev)[inv]
+    Op = cheb_series_values(coef_omF, ev)[inv]
+    Om = cheb_series_values(coef_omF, -ev)[inv]
+    Fs = {1: Fp, -1: Fm}
+    Os = {1: Op, -1: Om}
+    # <v_s|J_a|v_s'> = (s' g_a conj(ph) + s ph conj(g_a)) / 2
+    M = {(a, s, sp): 0.5 * (sp * g[a] * np.conj(ph) + s * ph * np.conj(g[a]))
+         for a in (0, 1) for s in (1, -1) for sp in (1, -1)}
+    steps = list(steps)
+    pairs = [(0, 0), (1, 0), (0, 1), (1, 1)]   # (a, b): <psi2|J_a psi1_b>
+    # per (state, pair): the time-independent sum and the weights of
+    # e^{+-2i|f| tau (t-1)}, stacked so each step is one matrix product
+    w0 = np.zeros(S * 4, dtype=np.complex128)
+    wp = np.empty((S * 4, N), dtype=np.complex128)
+    wm = np.empty((S * 4, N), dtype=np.complex128)
+    for i in range(S):
+        A = np.fft.fft2(psi0s[i, :, :, 0]).ravel()
+        B = np.fft.fft2(psi0s[i, :, :, 1]).ravel()
+        al = {s: (A + s * ph * B) / np.sqrt(2) for s in (1, -1)}
+        JA = [g[b] * B for b in (0, 1)]           # components of J_b psi0
+        JB = [np.conj(g[b]) * A for b in (0, 1)]
+        be = {(b, s): (JA[b] + s * ph * JB[b]) / np.sqrt(2) for b in (0, 1) for s in (1, -1)}
+        for q, (a, b) in enumerate(pairs):
+            w = {(s, sp): Fs[s] * Os[sp] * np.conj(al[s]) * M[(a, s, sp)] * be[(b, sp)]
+                 for s in (1, -1) for sp in (1, -1)}
+            w0[i * 4 + q] = np.sum(w[(1, 1)] + w[(-1, -1)])
+            wp[i * 4 + q] = w[(1, -1)]
+            wm[i * 4 + q] = w[(-1, 1)]
+    out = np.zeros((S * 4, len(steps)), dtype=np.complex128)
+    for c, t in enumerate(steps):
+        zt = np.exp(2j * e * tau * (t - 1))
+        out[:, c] = (w0 + wp @ zt + wm @ np.conj(zt)) / N
+    return out.reshape(S, 4, len(steps))
 
 
 def dense_corr(indptr, indices, data, psi0, tau, steps):

@@ -7,6 +7,9 @@ oracle.
   against exact k-space propagation.  Tolerances: complex128 1e-10;
   complex64 (FP32 storage and arithmetic, FP64 sums) 1e-6, about ten times
   the largest error measured over 2048 steps.
+* AC on the same sample (8 states, 1.5 nm tiles, three terms per pass)
+  against exact k-space propagation with the product's Fermi polynomial
+  over 256 steps: complex128 1e-11, complex64 1e-6 (correlations ~4e-2).
 * The disordered, field-threaded DC sample (1% vacancies, Anderson disorder
   U[-0.5, 0.5] eV, B = 20 T in the Landau gauge, 16 states, 5 energies,
   2.0 nm tiles with three and four terms per pass) on 24x24 cells against
@@ -86,6 +89,53 @@ def test_benchmark_layout_2048_steps_vs_kspace(gpu, graphene2048, prec, tol):
     assert err < tol
     # C(0) is the squared norm of the state as stored (complex64 rounds it)
     assert np.max(np.abs(c[:, 0] - 1.0)) < (1e-9 if prec == "fp32" else 1e-12)
+
+
+AC_STEPS = 256
+AC_PTS = [2, 3, 4, 17, 64, 128, 255, AC_STEPS]
+
+
+@pytest.fixture(scope="module")
+def ac2048(graphene2048):
+    """The AC configuration of the benchmark sample: 300 K, mu = 0, 8 states,
+    tau = pi/2, the Fermi series of the product, and its exact k-space
+    correlations at sampled steps of a 256-step run."""
+    s, _, _, _, psi, _ = graphene2048
+    ip, ix, hop, dx, dy = s.build_ham_dxy_device()
+    tau = tau_of(s, 0.5)
+    bes = tbpm.bessel_series(tau, 1e-14)
+    beta = 1.0 / (8.617333262e-5 * 300.0) * s.rescale
+    cF = tbpm.fermi_cheb_coef(2 ** 15, beta, 0.0, False, 1e-12)
+    cO = tbpm.fermi_cheb_coef(2 ** 15, beta, 0.0, True, 1e-12)
+    exact = oracle.kspace_graphene_ac(L2048, L2048, s.hop_v[:3], s.dr[:3], s.rescale, cF, cO,
+                                      psi, tau, AC_PTS)
+    order, tiles = s.tile_order(1.5, return_tiles=True)
+    return s, (ip, ix, hop, dx, dy), (order, tiles), bes, (cF, cO), psi, exact
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("fp64", 1e-11)])
+def test_ac_benchmark_layout_vs_kspace(gpu, ac2048, prec, tol):
+    """AC on 8.4M-orbital graphene as the benchmark runs it (1.5 nm tiles,
+    three terms per pass, 24 vectors: J_x psi1, J_y psi1 and psi2 of 8
+    states) against exact k-space propagation with the same Fermi
+    polynomial, at sampled steps of 256 (the correlations are ~4e-2)."""
+    s, (ip, ix, hop, dx, dy), (order, tiles), bes, (cF, cO), psi, exact = ac2048
+    h = tb._lib.Handle(0)
+    try:
+        h.set_blocking(True, terms=3)
+        H = tbpm.Hamiltonian(s.num_orb, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision=prec,
+                             handle=h, order=order, tiles=tiles)
+        assert H.blocking_info(24)[0] > 10000
+        c = tbpm.corr_ac(H, bes, cF, cO, psi, AC_STEPS, chk_thr=1e-4)
+        H.close()
+    finally:
+        h.close()
+    got = c[:, :, [t - 1 for t in AC_PTS]]
+    err = float(np.max(np.abs(got - exact)))
+    print(f"AC {prec} 2048x2048, 8 states, {AC_STEPS} steps: max |C - C_exact| = {err:.3e} "
+          f"(max |C| {np.max(np.abs(exact)):.3e})", flush=True)
+    assert np.max(np.abs(c[:, :, 0])) == 0.0
+    assert err < tol
 
 
 def _cfg3_sample(L, seed=2024):

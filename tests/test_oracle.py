@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
+import tbplas_amd as tb
 from conftest import golden
 
 KB = 8.617333262e-5
@@ -630,3 +631,28 @@ def test_orc_supercell_geometry_invariants():
             exp = (np.asarray(h[:3], float) + frac[h[4]] - frac[o]) @ lat
             assert np.min(np.abs(dr[sel] - exp).max(axis=1)) < 1e-12
     assert count == len(hi)
+
+
+def test_kspace_ac_matches_restatement():
+    """The exact k-space AC correlations of periodic graphene (band basis,
+    Fermi polynomial on the band energies) against the C restatement of
+    tbpm_accond (itself pinned against the reference) on 12x12 cells, two
+    states, every stored step, both currents and both states: a swapped
+    current, a sign of J or a wrong time origin fails."""
+    s = tb.make_graphene_sample(12)
+    s.rescale_ham(9.0)
+    ip, ix, hop, dx, dy = oracle.build_ham_dxy_loop(s.orb_eng, s.hop_i, s.hop_j, s.hop_v, s.dr)
+    tau = math.pi / s.energy_range * s.rescale
+    bes = oracle.bessel_series(tau, 1e-14)
+    beta = 1.0 / (8.617333262e-5 * 300.0) * s.rescale
+    mu = 0.1 / s.rescale
+    cF = oracle.orc_fermi_coef(2 ** 15, beta, mu, False, 1e-12)
+    cO = oracle.orc_fermi_coef(2 ** 15, beta, mu, True, 1e-12)
+    psi = tb.states.random_states(s.num_orb, tb.states.sample_seeds(5, 2))
+    T = 20
+    r = oracle.orc_tbpm_accond(ip, ix, hop, dx, dy, s.rescale, beta, mu, bes, psi, T)
+    k = oracle.kspace_graphene_ac(12, 12, s.hop_v[:3], s.dr[:3], s.rescale, cF, cO, psi, tau,
+                                  range(2, T + 1))
+    assert np.max(np.abs(r[:, :, 0])) == 0.0
+    assert np.max(np.abs(r)) > 1e-2
+    assert np.max(np.abs(r[:, :, 1:] - k)) < 1e-13
