@@ -353,6 +353,25 @@ int tbg_corr_dos_batch(tbg_handle *h, int n_inst, const tbg_ham *const *hams,
                        double chk_thr, double *const *corr, int *fail_inst,
                        int *fail_step);
 
+/* tbg_corr_dos without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, outputs, norm check and status codes as tbg_corr_dos; the
+ * run's buffers (recurrence vectors, packed states, partial sums,
+ * correlation rows) come from a 64 MiB device array the library holds on
+ * every GPU it runs on until the process ends, so the run must fit it:
+ * about 4 * n_pad * slots * (8 or 16) bytes + (T + 1) * 24 * slots bytes
+ * (n_pad = n rounded up to 64: about one million orbitals with one
+ * complex128 state, 260,000 with 8 complex64 states);
+ * larger runs return TBG_EINVAL.  One term per pass (spatial tiles are not
+ * used), so the correlations equal tbg_corr_dos's bit for bit on a
+ * Hamiltonian without tiles.  Every call on a GPU shares its array and
+ * leaves its contents there: calls that overlap in time on one GPU race. */
+int tbg_corr_dos_noalloc(tbg_handle *h, const tbg_ham *H, const double *bes,
+                         int n_bes, const double *psi0, int K, int T,
+                         int chk_step, double chk_thr, double *corr,
+                         int *fail_step);
+
 /* ---- DOS run in pieces ----------------------------------------------------
  * tbg_corr_dos / tbg_corr_dos_batch split into set-up, propagation and
  * read-out, so a caller can time the propagation alone or stop early:

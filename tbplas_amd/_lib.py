@@ -50,6 +50,7 @@ _SIGS = {
                      ctypes.POINTER(_I64), ctypes.POINTER(_I)],
     "tbg_corr_dos": [_P, _P, _P, _I, _P, _I, _I, _I, _D, _P, ctypes.POINTER(_I)],
     "tbg_corr_ldos": [_P, _P, _P, _I, _P, _I, _I, _I, _D, _P, ctypes.POINTER(_I)],
+    "tbg_corr_dos_noalloc": [_P, _P, _P, _I, _P, _I, _I, _I, _D, _P, ctypes.POINTER(_I)],
     "tbg_psi_t": [_P, _P, _P, _I, _P, _P, _I, _I, _I, _D, _P, ctypes.POINTER(_I)],
     "tbg_quasi_eigenstates": [_P, _P, _P, _I, _P, _I, _I, _D, _P, _I, _I, _D, _P,
                               ctypes.POINTER(_I)],

@@ -19,7 +19,7 @@ SOURCES = ["tbg_prop.hip", "tbg_pass_f32_t4_real.hip", "tbg_pass_f32_t4_cplx.hip
            "tbg_pass_f64_t4_real.hip", "tbg_pass_f64_t4_cplx.hip", "tbg_pass_t2.hip",
            "tbg_pass_f32_t3_real.hip", "tbg_pass_f32_t3_cplx.hip", "tbg_pass_f64_t3_real.hip",
            "tbg_pass_f64_t3_cplx.hip", "tbg_super.hip", "tbg_api.hip", "tbg_cond.hip",
-           "tbg_block.hip", "tbg_mlp.hip", "tbg_csr.hip", "tbg_hall.hip"]
+           "tbg_block.hip", "tbg_mlp.hip", "tbg_csr.hip", "tbg_hall.hip", "tbg_static.hip"]
 ARCH = os.environ.get("TBG_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

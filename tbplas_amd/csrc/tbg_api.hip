@@ -337,6 +337,7 @@ int tbg_create(tbg_handle** out, int device) {
         h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         h->max_lds = prop.sharedMemPerBlock > 0 ? (int)prop.sharedMemPerBlock : 65536;
         prepare_term3(h->max_lds);
+        static_scratch_init(device);   // the allocation-free entry point's array
         TBG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         *out = h.release();
     });

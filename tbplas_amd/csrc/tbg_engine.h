@@ -9,6 +9,10 @@
 
 namespace tbg {
 
+// Looks up the device address of the allocation-free entry point's scratch
+// array on `device` (current), once per device (tbg_static.hip).
+void static_scratch_init(int device);
+
 // Matrix operand of a run: one prepared Hamiltonian or the block-diagonal
 // concatenation of several.
 struct MatView {

@@ -37,17 +37,31 @@ struct Error {
             throw ::tbg::Error{TBG_EINVAL, (msg)};                             \
     } while (0)
 
-// Device memory owned by the library (freed in the destructor).
+// Device memory owned by the library (freed in the destructor), or borrowed
+// from memory the library does not allocate (never freed here).
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool owned = true;
     DevBuf() = default;
     explicit DevBuf(size_t b) { alloc(b); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), owned(o.owned) {
+        o.p = nullptr;
+        o.bytes = 0;
+        o.owned = true;
+    }
     DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            owned = o.owned;
+            o.p = nullptr;
+            o.bytes = 0;
+            o.owned = true;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
@@ -61,12 +75,21 @@ struct DevBuf {
                                       hipGetErrorString(e)};
         }
         bytes = b;
+        owned = true;
+    }
+    // use b bytes at q (device memory the caller keeps alive)
+    void borrow(void* q, size_t b) {
+        release();
+        p = q;
+        bytes = b;
+        owned = false;
     }
     void ensure(size_t b) { if (b > bytes) alloc(b); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        owned = true;
     }
     template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
 };
