@@ -57,6 +57,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6      # dense FP64 matrix-core peak of one MI355X
 METRIC = "TBPM step·vectors/s + % HBM roofline, 8.4M-orb graphene; C(t) err vs FP64"
 # steps of the accuracy run and the steps compared with the exact propagation
 ACC_STEPS = 2048
@@ -65,7 +66,7 @@ EXACT_PTS = sorted(set([0, 1, 2, 3, 4, 5, 17, 21, 100, 511, 1023, 1025, 2047, AC
 # the AC accuracy run and its sampled steps (t >= 2; column t = 1 is zero)
 AC_ACC_STEPS = 256
 AC_PTS = [2, 3, 4, 17, 64, 128, 255, AC_ACC_STEPS]
-WORKLOADS = ("all", "dos", "ac", "dc", "batch", "latency", "assembly")
+WORKLOADS = ("all", "dos", "ac", "dc", "batch", "latency", "assembly", "hall")
 
 
 def parse():
@@ -357,7 +358,13 @@ def kernel_label(kind, prec, K, real, threads):
         slots = min(slots, p3)
         V = 3 if slots % 3 == 0 else (slots if slots < 2 else 2)
         R = "float" if prec == "fp32" else "double"
-        return (f"k_block_pass<{R}, {V}, {str(bool(real)).lower()}, {mode}, {threads}, "
+        # the launch narrows 16-wave workgroups where the instance needs more
+        # registers than a lane has at 16 waves (complex128 states with the
+        # dot epilogue after three or four terms, three states per lane)
+        wide = V == 1 or (V == 2 and (prec == "fp32" or
+                                      (real and not (mode == "4" and int(terms) >= 3))))
+        nt = threads if (wide or threads != 1024) else 512
+        return (f"k_block_pass<{R}, {V}, {str(bool(real)).lower()}, {mode}, {nt}, "
                 f"{terms}>")
     return {"term": "k_term", "ac_corr": "k_accorr", "dc_qe": "k_qe",
             "dc_dot": "k_dcdot"}.get(kind, kind)
@@ -934,6 +941,71 @@ def assembly_workload(args):
             "sizes": recs}
 
 
+def hall_workload(args):
+    """The reference's Hall example (tests/test_units/test_hall.py): graphene
+    500x200 with t = 3 eV (200,000 orbitals), B = 100 T (gauge 0), rescale 9,
+    Kubo-Bastin moments with 2048 Chebyshev kernels in direction x for one
+    state, complex128.  The n_kernel^2 Gram matrix runs on the FP64 matrix
+    cores (k_gram, eight blocks of 256 rows): its roofline is FP64 MFMA,
+    8 flops per complex multiply-add over the sample's orbitals."""
+    import torch
+    import tbplas_amd as tb
+    from tbplas_amd import tbpm, _lib
+    N = 2048
+    s = tb.make_graphene_sample(500, 200, t=3.0)
+    s.set_magnetic_field(100.0, 0)
+    s.rescale_ham(9.0)
+    ip, ix, hop, dx, dy = s.build_ham_dxy_device()
+    n = s.num_orb
+    h = _lib.default_handle(0)
+    H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp64",
+                         handle=h)
+    psi = tb.states.random_states(n, tb.states.sample_seeds(1337, 1))
+    tbpm.hall_mu(H, psi, 256, 1)          # code objects and buffers warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mu = tbpm.hall_mu(H, psi, N, 1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kt = {}
+    if args.profile_steps > 0:
+        h.set_kernel_timing(True)
+        try:
+            mu2 = tbpm.hall_mu(H, psi, N, 1)
+            kt = h.kernel_times()
+        finally:
+            h.set_kernel_timing(False)
+        assert (mu2 == mu).all(), "Hall moments differ between two runs"
+    H.close()
+    rl = None
+    if "gram" in kt:
+        ms, cnt = kt["gram"]
+        avg_s = ms / cnt / 1e3
+        flops = 8.0 * 256 * N * n          # one block of 256 rows of mu
+        ach = flops / avg_s / 1e12
+        # only the Gram blocks are timed per launch here: their share of
+        # the untimed run's wall time
+        rl = {"bound": "mfma", "kernel": "k_gram", "launches": cnt,
+              "launch_avg_us": round(avg_s * 1e6, 2),
+              "share_of_run_time": round(ms / 1e3 / dt, 4),
+              "achieved": round(ach, 2), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+              "frac": round(ach / FP64_PEAK_TFS, 4), "flops_per_launch": flops,
+              "traffic": None,
+              "launches_by_kind": {k: {"launches": v[1], "avg_us": round(v[0] / v[1] * 1e3, 2)}
+                                   for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])}}
+        tr = counter_traffic("hall", "k_gram")
+        if tr is not None:
+            rl["traffic"] = tr["bytes_per_launch"]
+            rl["traffic_source"] = tr.get("summary")
+    return {"metric": "KPM Hall moments, seconds for 2048 kernels (200,000-orbital graphene, "
+                      "B = 100 T)", "value": round(dt, 4), "unit": "s", "higher_is_better": False,
+            "dtype": "fp64",
+            "config": {"workload": f"graphene 500x200, t = 3 eV, B = 100 T, n_kernel {N}, "
+                                   "direction x, 1 state", "n_orbitals": n,
+                       "nnz": int(ix.numel())},
+            "roofline": rl, "mu_00": [float(mu[0, 0, 0].real), float(mu[0, 0, 0].imag)]}
+
+
 def main():
     args = parse()
     if args.cpu_child:
@@ -951,7 +1023,7 @@ def main():
         copy_probe()
         return 0
     subs = {"ac": ac_workload, "dc": dc_workload, "batch": batch_workload,
-            "latency": latency_workload, "assembly": assembly_workload}
+            "latency": latency_workload, "assembly": assembly_workload, "hall": hall_workload}
     if args.workload in subs:
         rec = subs[args.workload](args)
         rec["n_gpus"] = 1

@@ -194,7 +194,8 @@ int tbg_ham_blocking_bytes(tbg_handle *h, const tbg_ham *H, int states,
  * (graph replay is skipped meanwhile).  tbg_kernel_times synchronises the
  * stream, sums the durations per launch kind and clears them: kinds[i]
  * (1 = one-term kernel, 2 = AC correlation, 3 = DC quasi-eigenstate update,
- * 4 = DC current correlation, 100 + 10 * mode + terms = tile pass with mode
+ * 4 = DC current correlation, 5 = Hall moments' Gram block (k_gram),
+ * 100 + 10 * mode + terms = tile pass with mode
  * 0 continue, 1 first terms from v_0, 2 last terms, 3 last terms with the
  * norm, 4 last terms with dot and norm), total_ms[i], counts[i], for at
  * most max_kinds kinds; *n_kinds = kinds seen. */

@@ -193,7 +193,8 @@ class Handle:
     def kernel_times(self):
         """{kind: (total ms, launches)} of the launches timed since the last
         call (synchronises the handle's stream).  Kinds: "term" (one-term
-        kernel), "ac_corr", "dc_qe", "dc_dot", and "pass<mode>x<terms>" for
+        kernel), "ac_corr", "dc_qe", "dc_dot", "gram" (the Hall moments'
+        Gram block), and "pass<mode>x<terms>" for
         the tile pass (mode 0 continue, 1 first terms from v_0, 2 last
         terms, 3 with the norm, 4 with dot and norm)."""
         cap = 64
@@ -203,7 +204,7 @@ class Handle:
         nk = ctypes.c_int()
         check(lib().tbg_kernel_times(self._h, cap, kinds.ctypes.data, ms.ctypes.data,
                                      cnt.ctypes.data, ctypes.byref(nk)))
-        names = {1: "term", 2: "ac_corr", 3: "dc_qe", 4: "dc_dot"}
+        names = {1: "term", 2: "ac_corr", 3: "dc_qe", 4: "dc_dot", 5: "gram"}
         out = {}
         for i in range(min(nk.value, cap)):
             k = int(kinds[i])

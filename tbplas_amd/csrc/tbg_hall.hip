@@ -68,21 +68,33 @@ k_gram(const double2* __restrict__ P, const double2* __restrict__ W, int64_t np,
             cre[a][b] = gram_d4{0.0, 0.0, 0.0, 0.0};
             cim[a][b] = gram_d4{0.0, 0.0, 0.0, 0.0};
         }
+    // Stage k0's operands are loaded into registers while the MFMAs of the
+    // previous stage run, then stored to LDS.
     const int sk = t & 15;
-    for (int64_t k0 = kb; k0 < ke; k0 += kGramK) {
+    double2 pq[4], xq[4];
+    auto fetch = [&](int64_t k0) {
         const int64_t k = k0 + sk;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int sm = (t >> 4) + 16 * q;
-            double2 p = make_double2(0.0, 0.0), x = make_double2(0.0, 0.0);
-            if (k < ke && m0 + sm < rows) p = P[(int64_t)(m0 + sm) * np + k];
-            if (k < ke && j0 + sm < N) x = W[(int64_t)(j0 + sm) * np + k];
-            lds[0][sk][sm] = p.x;
-            lds[1][sk][sm] = p.y;
-            lds[2][sk][sm] = x.x;
-            lds[3][sk][sm] = x.y;
+            pq[q] = make_double2(0.0, 0.0);
+            xq[q] = make_double2(0.0, 0.0);
+            if (k < ke && m0 + sm < rows) pq[q] = P[(int64_t)(m0 + sm) * np + k];
+            if (k < ke && j0 + sm < N) xq[q] = W[(int64_t)(j0 + sm) * np + k];
+        }
+    };
+    if (kb < ke) fetch(kb);
+    for (int64_t k0 = kb; k0 < ke; k0 += kGramK) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int sm = (t >> 4) + 16 * q;
+            lds[0][sk][sm] = pq[q].x;
+            lds[1][sk][sm] = pq[q].y;
+            lds[2][sk][sm] = xq[q].x;
+            lds[3][sk][sm] = xq[q].y;
         }
         __syncthreads();
+        if (k0 + kGramK < ke) fetch(k0 + kGramK);
 #pragma unroll
         for (int kk = 0; kk < kGramK; kk += 4) {
             double ar[2], ai[2], br[2], bi[2];
@@ -93,15 +105,23 @@ k_gram(const double2* __restrict__ P, const double2* __restrict__ W, int64_t np,
                 br[a] = lds[2][kk + lk][wn + 16 * a + li];
                 bi[a] = lds[3][kk + lk][wn + 16 * a + li];
             }
+            // eight independent accumulators between two products into
+            // the same one; each still gets its products in the order
+            // re: Pr Wr, Pi Wi and im: Pr Wi, -Pi Wr
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
                     cre[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cre[a][b], 0,
                                                                      0, 0);
-                    cre[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bi[b], cre[a][b], 0,
-                                                                     0, 0);
                     cim[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], cim[a][b], 0,
+                                                                     0, 0);
+                }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    cre[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bi[b], cre[a][b], 0,
                                                                      0, 0);
                     cim[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], br[b], cim[a][b],
                                                                      0, 0, 0);
@@ -259,9 +279,10 @@ void hall_mu_impl(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, in
         a.beta = -1.0;
         launch_term<double>(first ? F_STORE : (F_PREV | F_STORE), a, e.plan.nchunks, st);
     };
-    // k split of the Gram blocks: enough workgroups for two per CU
+    // k split of the Gram blocks: enough workgroups for three per CU (the
+    // registers of k_gram allow three 4-wave workgroups per CU)
     const int tiles = ((B + kGramTile - 1) / kGramTile) * ((N + kGramTile - 1) / kGramTile);
-    int64_t nsplit = std::max<int64_t>(1, (2 * (int64_t)h->num_cus + tiles - 1) / tiles);
+    int64_t nsplit = std::max<int64_t>(1, (3 * (int64_t)h->num_cus + tiles - 1) / tiles);
     nsplit = std::min<int64_t>(nsplit, std::max<int64_t>(1, np / (4 * kGramK)));
     const int64_t kchunk = ((np + nsplit - 1) / nsplit + kGramK - 1) / kGramK * kGramK;
     nsplit = (np + kchunk - 1) / kchunk;
@@ -307,11 +328,13 @@ void hall_mu_impl(tbg_handle* h, const tbg_ham* H, const double* psi0, int K, in
                 const int i0 = i - i % B, rows = i % B + 1;
                 const dim3 grid((unsigned)((N + kGramTile - 1) / kGramTile),
                                 (unsigned)((rows + kGramTile - 1) / kGramTile), (unsigned)nsplit);
-                hipLaunchKernelGGL(k_gram, grid, dim3(256), 0, st,
-                                   reinterpret_cast<const double2*>(Pb.p),
-                                   reinterpret_cast<const double2*>(W.p), np, rows, N, kchunk,
-                                   gpart.as<double2>());
-                TBG_HIP(hipGetLastError());
+                timed_launch(h, kTimeGram, [&] {
+                    hipLaunchKernelGGL(k_gram, grid, dim3(256), 0, st,
+                                       reinterpret_cast<const double2*>(Pb.p),
+                                       reinterpret_cast<const double2*>(W.p), np, rows, N,
+                                       kchunk, gpart.as<double2>());
+                    TBG_HIP(hipGetLastError());
+                });
                 const int64_t nq = (int64_t)rows * N;
                 hipLaunchKernelGGL(k_gram_sum, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
                                    st, gpart.as<double2>(), (int)nsplit, rows, N,

@@ -142,9 +142,10 @@ struct tbg_handle {
 namespace tbg {
 
 // Kinds of timed launches: the one-term kernel, the AC / DC correlation
-// kernels, and the tile pass as kTimePass + 10 * mode + terms.
+// kernels, the Hall moments' Gram block, and the tile pass as kTimePass +
+// 10 * mode + terms.
 enum TimedKind : int { kTimeTerm = 1, kTimeAcCorr = 2, kTimeQe = 3, kTimeDcDot = 4,
-                       kTimePass = 100 };
+                       kTimeGram = 5, kTimePass = 100 };
 
 // Runs launch() on h->stream, bracketed by two events when timing is on.
 template <typename F>

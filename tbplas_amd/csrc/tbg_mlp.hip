@@ -1,5 +1,5 @@
 // Small MLP surrogate (dense layers, GELU) trained and evaluated on the GPU:
-// BF16 operands on the matrix cores (v_mfma_f32_32x32x8_bf16), FP32
+// BF16 operands on the matrix cores (v_mfma_f32_32x32x16_bf16), FP32
 // accumulation, FP32 master weights, Adam.  No reference counterpart: it
 // maps disorder/strain parameters to DOS spectra produced by the TBPM path.
 //
@@ -14,7 +14,7 @@
 
 namespace tbg {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // float -> bf16, round to nearest even (NaN kept quiet)
@@ -46,28 +46,30 @@ struct GemmArgs {
     int epi;
 };
 
-// One wavefront computes a 32x32 tile of C = A B with 32x32x8 BF16 MFMAs;
-// a block of 4 waves covers 64x64.  Operand lane layout of the instruction:
-// A: lane l holds A[l%32][4*(l/32) + j], B: B[4*(l/32) + j][l%32], j<4;
-// result register r of lane l is C[8*(r/4) + 4*(l/32) + r%4][l%32].
+// One wavefront computes a 32x32 tile of C = A B with gfx950's 32x32x16
+// BF16 MFMAs; a block of 4 waves covers 64x64.  Operand lane layout of the
+// instruction: A: lane l holds A[l%32][8*(l/32) + j], B: B[8*(l/32) + j][l%32],
+// j<8; result register r of lane l is C[8*(r/4) + 4*(l/32) + r%4][l%32].
 __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int tm = blockIdx.y * 64 + (wv >> 1) * 32;
     const int tn = blockIdx.x * 64 + (wv & 1) * 32;
-    const int li = lane & 31, lk = (lane >> 5) * 4;
+    const int li = lane & 31, lk = (lane >> 5) * 8;
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
     const int m = tm + li, n = tn + li;
-    for (int k0 = 0; k0 < g.K; k0 += 8) {
-        bf16x4 a, b;
+    for (int k0 = 0; k0 < g.K; k0 += 16) {
+        bf16x8 a, b;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 8; ++j) {
             const int k = k0 + lk + j;
-            a[j] = (m < g.M && k < g.K) ? (short)g.A[m * g.sam + k * g.sak] : (short)0;
-            b[j] = (k < g.K && n < g.N) ? (short)g.B[k * g.sbk + n * g.sbn] : (short)0;
+            const uint16_t ua = (m < g.M && k < g.K) ? g.A[m * g.sam + k * g.sak] : (uint16_t)0;
+            const uint16_t ub = (k < g.K && n < g.N) ? g.B[k * g.sbk + n * g.sbn] : (uint16_t)0;
+            a[j] = __builtin_bit_cast(__bf16, ua);
+            b[j] = __builtin_bit_cast(__bf16, ub);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {

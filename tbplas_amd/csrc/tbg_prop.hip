@@ -195,9 +195,11 @@ template <typename R>
 __global__ void k_pack(const double2* __restrict__ src, int K, int64_t n,
                        typename CplxT<R>::T* __restrict__ dst, int Kp, int slot0,
                        const int* __restrict__ perm) {
+    // consecutive threads take consecutive states of one row: the stores of
+    // a row's slots are one contiguous run
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)K * n) return;
-    const int64_t k = i / n, r = i - k * n;
+    const int64_t r = i / K, k = i - r * K;
     const int64_t c = perm ? (int64_t)perm[r] : r;
     const double2 v = src[k * n + c];
     typename CplxT<R>::T o;
@@ -211,13 +213,16 @@ __global__ void k_unpack(const typename CplxT<R>::T* __restrict__ src, int Kp, i
                          int64_t n, double2* __restrict__ dst, int K, const int* __restrict__ perm) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)K * n) return;
-    const int64_t k = i / n, r = i - k * n;
+    const int64_t r = i / K, k = i - r * K;
     const int64_t c = perm ? (int64_t)perm[r] : r;
     const typename CplxT<R>::T v = src[r * Kp + slot0 + k];
     dst[k * n + c] = make_double2((double)v.x, (double)v.y);
 }
 
-// conj(a) . b and |b|^2 per vector over the first K slots, per-chunk partials.
+// conj(a) . b and |b|^2 per vector over the first K slots, per-chunk
+// partials.  Thread t takes slot t % Kp of every (256 / Kp)-th row from row
+// t / Kp, so a wave reads whole rows (coalesced); the row groups' sums are
+// then added in group order.  Kp <= 256.
 template <typename R>
 __global__ void __launch_bounds__(256)
 k_dotnorm(const int64_t* chunk, const typename CplxT<R>::T* a, const typename CplxT<R>::T* b,
@@ -225,9 +230,11 @@ k_dotnorm(const int64_t* chunk, const typename CplxT<R>::T* a, const typename Cp
     const int c = remap_chunk(blockIdx.x, gridDim.x);
     const int64_t r0 = chunk[c] * 64, r1 = chunk[c + 1] * 64;
     __shared__ double sh[256 * 3];
-    for (int k = 0; k < K; ++k) {
-        double dr = 0, di = 0, nn = 0;
-        for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) {
+    const int RG = 256 / Kp;
+    const int k = threadIdx.x % Kp, rg = threadIdx.x / Kp;
+    double dr = 0, di = 0, nn = 0;
+    if (rg < RG && k < K) {
+        for (int64_t r = r0 + rg; r < r1; r += RG) {
             const auto x = a[r * Kp + k];
             const auto y = b[r * Kp + k];
             const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
@@ -235,17 +242,16 @@ k_dotnorm(const int64_t* chunk, const typename CplxT<R>::T* a, const typename Cp
             di = fma(xr, yi, fma(-xi, yr, di));
             nn = fma(yr, yr, fma(yi, yi, nn));
         }
-        sh[threadIdx.x * 3 + 0] = dr;
-        sh[threadIdx.x * 3 + 1] = di;
-        sh[threadIdx.x * 3 + 2] = nn;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s)
-                for (int q = 0; q < 3; ++q) sh[threadIdx.x * 3 + q] += sh[(threadIdx.x + s) * 3 + q];
-            __syncthreads();
-        }
-        if (threadIdx.x < 3) part[(int64_t)c * 3 * K + k * 3 + threadIdx.x] = sh[threadIdx.x];
-        __syncthreads();
+    }
+    sh[threadIdx.x * 3 + 0] = dr;
+    sh[threadIdx.x * 3 + 1] = di;
+    sh[threadIdx.x * 3 + 2] = nn;
+    __syncthreads();
+    for (int q = threadIdx.x; q < 3 * K; q += 256) {
+        const int kk = q / 3, e = q - kk * 3;
+        double t = 0.0;
+        for (int g = 0; g < RG; ++g) t += sh[(g * Kp + kk) * 3 + e];
+        part[(int64_t)c * 3 * K + q] = t;
     }
 }
 
@@ -391,6 +397,7 @@ template void launch_unpack<double>(const void*, int, int, int64_t, double*, int
 template <typename R>
 void launch_dotnorm(const Plan& p, const void* a, const void* b, int Kp, int K, double* part,
                     hipStream_t s) {
+    if (Kp < 1 || Kp > 256 || K > Kp) throw Error{TBG_EINVAL, "internal: bad slot count"};
     hipLaunchKernelGGL(k_dotnorm<R>, dim3(p.nchunks), dim3(256), 0, s, p.chunk.as<int64_t>(),
                        reinterpret_cast<const typename CplxT<R>::T*>(a),
                        reinterpret_cast<const typename CplxT<R>::T*>(b), Kp, K, part);
