@@ -93,9 +93,11 @@ def parse():
                    help="row order of the device layout: the sample's own, or square "
                         "tiles (Sample.tile_order), which lets the tuner choose three or "
                         "four Chebyshev terms per pass")
-    p.add_argument("--tile", type=float, default=5.0,
-                   help="dos and batch: tile edge in nm (5.0 nm fits 8 complex64 states a "
-                        "row with three halo rings in the LDS of one workgroup)")
+    p.add_argument("--tile", type=float, default=None,
+                   help="dos and batch: tile edge in nm (default: the library's "
+                        "auto_tile_nm for each leg's states and precision, as the Solver's "
+                        "DOS driver picks it: 5.0 nm for 8 complex64 states a row, 3.4 nm "
+                        "for 8 complex128 states, 5.0 nm for the batch's 4)")
     p.add_argument("--cond-tile", type=float, default=None,
                    help="dc and ac: tile edge in nm (0: the sample's own order; default "
                         "2.0 for dc, 1.5 for ac)")
@@ -384,7 +386,7 @@ def roofline(kt, bytes_of, label_of, workload, extra=None):
            "launch_avg_us": round(avg_s * 1e6, 2),
            "share_of_device_time": round(ms / tot, 4) if tot > 0 else None,
            "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-           "traffic": None, "bytes_per_launch": None}
+           "traffic": None, "bytes_per_launch": None, "traffic_key": workload}
     if b is not None:
         ach = b / avg_s / 1e9
         rec.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
@@ -444,9 +446,21 @@ def pass_bytes_fn(sched, n, nnz, K, s, real):
 # ---------------------------------------------------------------------------
 # Headline: DOS on 8.4M-orbital graphene
 # ---------------------------------------------------------------------------
-def dos_leg(args, s, csr, ot, bes, psi, prec, K, h, total):
+def leg_tile(args, K, prec):
+    """Tile edge (nm) of a DOS leg: --tile, else the library's rule."""
+    from tbplas_amd.sample import auto_tile_nm
+    return args.tile if args.tile is not None else round(auto_tile_nm(K, prec), 2)
+
+
+def traffic_key(workload, tile):
+    """Key of a workload's counter profile: the layout changes the bytes."""
+    return f"{workload}@{tile:g}nm" if tile else f"{workload}@caller"
+
+
+def dos_leg(args, s, csr, tile, bes, psi, prec, K, h, total):
     from tbplas_amd import tbpm, tune
     ip, ix, dt = csr
+    ot = s.tile_order(tile, return_tiles=True) if tile else (None, None)
     n, nnz = s.num_orb, int(ix.numel())
     H = tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h, order=ot[0], tiles=ot[1])
     real = H.info()["real_values"]
@@ -463,7 +477,7 @@ def dos_leg(args, s, csr, ot, bes, psi, prec, K, h, total):
     nterm = len(bes) - 1
     label = lambda k: kernel_label(k, prec, K, real, threads)  # noqa: E731
     rl = roofline(kt, pass_bytes_fn(sched, n, nnz, K, sz, real), label,
-                  f"dos_{prec}_k{K}") if kt else None
+                  traffic_key(f"dos_{prec}_k{K}", tile)) if kt else None
     one = term_bytes(n, nnz, K, sz, real) / (t_step / nterm) / 1e9
     if rl is not None:
         rl["one_term_equivalent"] = {
@@ -476,7 +490,8 @@ def dos_leg(args, s, csr, ot, bes, psi, prec, K, h, total):
         if sched is not None:
             rl["schedule"] = sched
     return {"value": K / t_step, "ms_per_step": t_step * 1e3, "shape": shape, "corr": corr,
-            "roofline": rl, "real": real}
+            "roofline": rl, "real": real,
+            "layout": (f"tiles of {tile:g} nm" if tile else "caller order") + passes_name(shape)}
 
 
 def headline(args, cpu, exact_proc):
@@ -492,23 +507,24 @@ def headline(args, cpu, exact_proc):
     bes = tbpm.bessel_series(2 * math.pi / s.energy_range * s.rescale, 1e-15)
     K = args.vectors
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
-    ot = (s.tile_order(args.tile, return_tiles=True) if args.order == "tiles" else (None, None))
     log(f"setup {time.perf_counter() - t0:.1f}s: n={n} nnz={nnz} n_bes={len(bes)}")
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
     total = max(args.accuracy_steps, args.warmup + args.steps + args.profile_steps)
 
-    r32 = dos_leg(args, s, csr, ot, bes, psi, "fp32", K, h, total)
+    tiles = args.order == "tiles"
+    r32 = dos_leg(args, s, csr, leg_tile(args, K, "fp32") if tiles else 0, bes, psi, "fp32", K,
+                  h, total)
     log(f"fp32 K={K}: {r32['value']:.1f} step·vectors/s ({r32['ms_per_step']:.3f} ms/step)")
-    r64 = dos_leg(args, s, csr, ot, bes, psi, "fp64", K, h, total)
+    r64 = dos_leg(args, s, csr, leg_tile(args, K, "fp64") if tiles else 0, bes, psi, "fp64", K,
+                  h, total)
     log(f"fp64 K={K}: {r64['value']:.1f} step·vectors/s ({r64['ms_per_step']:.3f} ms/step)")
     a1 = argparse.Namespace(**vars(args))
     a1.profile_steps = 0
-    r1 = dos_leg(a1, s, csr, ot, bes, psi, "fp64", 1, h,
-                 args.warmup + args.steps)
+    r1 = dos_leg(a1, s, csr, leg_tile(args, 1, "fp64") if tiles else 0, bes, psi, "fp64", 1,
+                 h, args.warmup + args.steps)
     log(f"fp64 K=1: {r1['value']:.1f} steps/s")
 
-    layout = ("caller order" if args.order == "caller" else f"tiles of {args.tile:g} nm")
     out = {
         "metric": METRIC, "value": round(r32["value"], 3), "unit": "step·vectors/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -519,17 +535,17 @@ def headline(args, cpu, exact_proc):
                                 f"together, complex64 storage/FP64 correlation"),
                    "n_orbitals": n, "nnz": nnz, "states": K, "bessel_terms": len(bes),
                    "terms_per_step": len(bes) - 1, "hipgraph": not args.no_graph,
-                   "layout": layout + passes_name(r32["shape"]),
+                   "layout": r32["layout"],
                    "launch_shape": shape_json(r32["shape"]),
                    "run_steps": total},
         "roofline": r32["roofline"],
         "fp64_k8": {"value": round(r64["value"], 3), "unit": "step·vectors/s",
                     "ms_per_step": round(r64["ms_per_step"], 4),
-                    "layout": layout + passes_name(r64["shape"]),
+                    "layout": r64["layout"],
                     "launch_shape": shape_json(r64["shape"]), "roofline": r64["roofline"]},
         "fp64_k1": {"value": round(r1["value"], 3), "unit": "step·vectors/s",
                     "ms_per_step": round(r1["ms_per_step"], 4),
-                    "layout": layout + passes_name(r1["shape"]),
+                    "layout": r1["layout"],
                     "launch_shape": shape_json(r1["shape"])},
         "speedup_vs_fp64_k1": round(r32["value"] / r1["value"], 3),
     }
@@ -633,7 +649,8 @@ def ac_workload(args, exact=None):
                  kernel_label(k, prec, 3 * K, real, threads))
         res[prec] = {"value": K * (T - 1) / t_prop, "ms_per_step": t_prop / (T - 1) * 1e3,
                      "fermi_setup_s": t1 - t0, "shape": shape, "corr": c,
-                     "roofline": roofline(kt, bytes_of, label, f"ac_{prec}") if kt else None,
+                     "roofline": (roofline(kt, bytes_of, label, traffic_key(f"ac_{prec}", tile))
+                                  if kt else None),
                      "err_vs_exact": acc}
         log(f"ac {prec}: {res[prec]['value']:.1f} state·steps/s, max |C - C_exact| {acc}")
     err = float(np.max(np.abs(res["fp32"]["corr"] - res["fp64"]["corr"])))
@@ -731,7 +748,8 @@ def dc_workload(args):
                        + passes_name(shape),
                        "launch_shape": shape_json(shape)},
             "roofline": (roofline(kt, bytes_of,
-                                  lambda k: kernel_label(k, "fp32", 2 * K, real, threads), "dc")
+                                  lambda k: kernel_label(k, "fp32", 2 * K, real, threads),
+                                  traffic_key("dc", tile))
                          if kt else None),
             "dos_corr_t1_mean": float(np.mean(d[:, 0]).real),
             "dc_corr_t1_mean": [float(np.mean(c[:, a, :, 0]).real) for a in range(2)]}
@@ -753,6 +771,7 @@ def batch_workload(args):
     vac = rng.uniform(0.0, 0.03, 32)
     Ws = rng.uniform(0.0, 2.0, 32)
     K = 4
+    tile = leg_tile(args, K, "fp32") if args.order == "tiles" else 0
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
     insts = []
@@ -764,8 +783,7 @@ def batch_workload(args):
         ip, ix, dt = s.build_ham_csr_device()
         insts.append((s.num_orb, int(ix.numel()), ip, ix, dt,
                       tb.states.random_states(s.num_orb, tb.states.sample_seeds(300 + i, K)),
-                      s.tile_order(args.tile, return_tiles=True) if args.order == "tiles"
-                      else (None, None)))
+                      s.tile_order(tile, return_tiles=True) if tile else (None, None)))
     bes = tbpm.bessel_series(math.pi, 1e-14)
     total = args.warmup + args.steps + args.profile_steps
     res = {}
@@ -794,11 +812,12 @@ def batch_workload(args):
                                    f"{sorted(set(edges.tolist()))}, {K} states each, "
                                    f"{args.steps} timed steps", "n_orbitals_total": n_tot,
                        "nnz_total": nnz_tot, "launch_shape": shape_json(shape),
-                       "layout": (f"tiles of {args.tile:g} nm" if args.order == "tiles"
+                       "layout": (f"tiles of {tile:g} nm" if tile
                                   else "caller order") + passes_name(shape),
                        "hipgraph": not args.no_graph},
             "roofline": (roofline(kt, pass_bytes_fn(sched, n_tot, nnz_tot, K, 8, real),
-                                  lambda k: kernel_label(k, "fp32", K, real, 1024), "batch")
+                                  lambda k: kernel_label(k, "fp32", K, real, 1024),
+                                  traffic_key("batch", tile))
                          if kt else None),
             "fp64_ms_per_step": round(res["fp64"][0] / args.steps, 4),
             "ct_err_fp32_vs_fp64_per_instance_max": max(errs),
@@ -840,7 +859,8 @@ def latency_workload(args):
         sz = 8 if prec == "fp32" else 16
         rl = (roofline(kt, pass_bytes_fn(sched, n, nnz, 1, sz, real),
                        lambda k, prec=prec, real=real, threads=threads:
-                       kernel_label(k, prec, 1, real, threads), f"latency_{prec}")
+                       kernel_label(k, prec, 1, real, threads),
+                       traffic_key(f"latency_{prec}", 2.0 if order is not None else 0))
               if kt else None)
         if rl is not None:
             rl["note"] = ("the vectors fit in the 256 MB last-level cache: launch and "
