@@ -104,7 +104,8 @@ struct PassArgs {
     double c[6][2];          // c_{m-2} .. c_{m+3} per coefficient class
     uint64_t bwd_mask, cls_mask;
     int K, lgG;
-    int nA_max;
+    int nA_max, nB_max;      // widest A and B: B starts at row nA_max of the LDS,
+                             // its spare row (phase 1) is row nB_max of B
     int* counter;            // PM_M1
     const void* psi0;        // PM_DOTNORM
     double* part;            // PM_NORM, PM_DOTNORM

@@ -148,7 +148,8 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
                 }
                 const int nA = (int)(ends[depth] - base), nB = (int)(ends[depth - 1] - base);
                 // (LDS positions are 16 bits: nA < 65536 whenever A fits the LDS)
-                if ((int64_t)(nA + nB) * row_bytes > budget || nA > 65535) {
+                // (A, B and B's spare row)
+                if ((int64_t)(nA + nB + 1) * row_bytes > budget || nA > 65535) {
                     for (size_t q = base; q < rows.size(); ++q) mark[rows[q]] = -1;
                     rows.resize(base);
                     if (r1 - r0 < 32) return nullptr;   // a few rows with too wide a halo
@@ -224,7 +225,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
             }
         }
         seg_block.push_back((int)meta.size());
-        if ((int64_t)(nA_max + nB_max) * row_bytes <= h->max_lds) break;
+        if ((int64_t)(nA_max + nB_max + 1) * row_bytes <= h->max_lds) break;
         if (attempt == 3) return nullptr;
         budget = budget * 7 / 8;
     }

@@ -17,9 +17,11 @@ namespace tbg {
 // One lane's rows l = l0, l0 + step, ... < n of a phase when every row has
 // NJ <= 4 entries: the row number and the NJ table entries (index idx(j, l),
 // value at vals[j * nA + l]) of the next row are loaded before the current
-// row's work, so their latency hides behind it.  body(l, t) does the row's
-// work from its prefetched entries t (sums via tab_sum, in j order as
-// row_sum).  With GR the row numbers come from rows[l], else they are the
+// row's work, so their latency hides behind it.  The last row loads its own
+// entries again instead of a next row's: the loads are never conditional,
+// so the compiler can count them and does not wait for them at the top of
+// the next row's work.  body(l, t) does the row's work from its prefetched
+// entries t (sums via tab_sum, in j order as row_sum).  With GR the row numbers come from rows[l], else they are the
 // owned run r0 + l (the phases after the first only write owned rows).
 template <typename R, bool RV, int NJ>
 struct RowTab {
@@ -49,8 +51,7 @@ __device__ __forceinline__ void rows_prefetched(int l0, int n, int step, const i
     while (true) {
         const int ln = l + step;
         const bool more = ln < n;
-        RowTab<R, RV, NJ> nxt;
-        if (more) nxt = load(ln);
+        const RowTab<R, RV, NJ> nxt = load(more ? ln : l);
         body(l, cur);
         if (!more) break;
         cur = nxt;
@@ -230,13 +231,15 @@ __global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
         }
     };
 
-    // phase 1: v_m
+    // phase 1: v_m.  Each row's own v_{m-1} is loaded with its gathers and
+    // parked in B after them (rows beyond B's n[T-2] into the spare row at
+    // nB_max): no branch separates load and store, so each row waits one
+    // memory round trip, not two.
+    const int spare = a.nB_max;
     phase(table_rows, m.n[T - 1], idxG, gatherG,
-          [&](int l, int gr) {
-              if (l < m.n[T - 2]) lds_st(ldsB, l, vld<R, V>(a.p1, (int64_t)gr * K + k0));
-              return vzero<R, V>();
-          },
-          [&](int l, int gr, const VT& s, const VT&) {
+          [&](int, int gr) { return vld<R, V>(a.p1, (int64_t)gr * K + k0); },
+          [&](int l, int gr, const VT& s, const VT& own) {
+              lds_st(ldsB, l < m.n[T - 2] ? l : spare, own);
               VT v;
               if constexpr (M1) {
                   v = recur<R, V, false>(ar1, ai1, beta, s, s);
@@ -393,7 +396,8 @@ static void launch_f(const PassArgs& a, const Blocking& b, hipStream_t s) {
         V == 1 || (V == 2 && (sizeof(R) == 4 || (RV && !(MODE == PM_DOTNORM && T >= 3))));
     int nt = term3_launch_threads((int)sizeof(R), RV, a.K, b.nA_max);
     if (!wide && V >= 2) nt = 512;
-    size_t lds = (size_t)(b.nA_max + b.nB_max) * a.K * sizeof(typename CplxT<R>::T);
+    // areas A and B and B's spare row (phase 1 parks the rows beyond B there)
+    size_t lds = (size_t)(b.nA_max + b.nB_max + 1) * a.K * sizeof(typename CplxT<R>::T);
     if constexpr (MODE == PM_NORM || MODE == PM_DOTNORM)   // the waves' partial sums
         lds = std::max(lds, (size_t)(nt / 64) * 3 * a.K * sizeof(double));
     if constexpr (wide) {

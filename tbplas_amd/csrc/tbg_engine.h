@@ -203,6 +203,7 @@ struct Engine {
         base.K = Kp;
         base.lgG = lgG3;
         base.nA_max = blk->nA_max;
+        base.nB_max = blk->nB_max;
         base.nv = 3 * Kp;
         auto pass = [&](int mode, int terms, int m, void* p0, void* p1, void* q1, void* q2) {
             PassArgs t = base;
