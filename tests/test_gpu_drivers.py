@@ -290,6 +290,32 @@ def test_hall_mu_gram_split_over_workgroups(gpu):
     H.close()
 
 
+def test_hall_mu_2048_kernels_vs_oracle(gpu):
+    """The reference Hall example's kernel count (2048: eight Gram blocks of
+    256 rows, 32 full 64-wide tiles per row, the orbital dimension split in
+    six) on its lattice scaled down (graphene 50x20 with t = 3 eV, B = 100 T):
+    against orc_tbpm_kbdc, and the per-launch timing sees eight Gram blocks."""
+    s = tb.make_graphene_sample(50, 20, t=3.0)
+    s.set_magnetic_field(100.0, 0)
+    s.rescale_ham(9.0)
+    H, (ip, ix, dt, dx, dy) = _dxy_ham(s)
+    psi = states.random_states(s.num_orb, states.sample_seeds(31, 1))
+    h = H.handle
+    h.set_kernel_timing(True)
+    try:
+        g = tbpm.hall_mu(H, psi, 2048, 1)
+        kt = h.kernel_times()
+    finally:
+        h.set_kernel_timing(False)
+    r = oracle.orc_tbpm_kbdc(ip, ix, dt, dx, dy, s.rescale, psi, 2048, 1)
+    err = float(np.max(np.abs(g - r)))
+    print(f"hall 2048 kernels: max |mu_gpu - mu_oracle| = {err:.3e} (max |mu| "
+          f"{np.max(np.abs(r)):.3e})", flush=True)
+    assert err < 1e-12 * max(1.0, np.max(np.abs(r)))
+    assert kt["gram"][1] == 8
+    H.close()
+
+
 @needs_ref
 def test_hall_mu_vs_reference_live(gpu):
     s = _sample(L=5, field=15.0)
