@@ -100,7 +100,10 @@ def parse():
                         "for 8 complex128 states, 5.0 nm for the batch's 4)")
     p.add_argument("--cond-tile", type=float, default=None,
                    help="dc and ac: tile edge in nm (0: the sample's own order; default "
-                        "2.0 for dc, 1.5 for ac)")
+                        "2.0 for dc, 2.5 for ac)")
+    p.add_argument("--ac-states", type=int, default=4,
+                   help="ac: states per run (three propagated vectors each), as the "
+                        "Solver's gpu['ac_states']")
     p.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     p.add_argument("--exact-child", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
@@ -577,19 +580,22 @@ def cond_order(s, tile):
 
 def ac_workload(args, exact=None):
     """Pristine graphene 2048x2048 (rescale 9), AC conductivity correlation
-    of 8 states at 300 K, mu = 0: the Fermi and one-minus-Fermi polynomials
-    (FFT coefficients at 2^15 points, precision 1e-12) applied once, then
-    J_x psi1, J_y psi1 and psi2 of every state (24 vectors) propagated
-    together with tau = pi/2 (16 Bessel terms at 1e-14) and the four current
-    correlations formed every step; complex64 (and complex128 for the error)
-    with FP64 sums; 1.5 nm tiles."""
+    of 8 states at 300 K, mu = 0, run as the Solver runs it: --ac-states
+    states at a time (default 4, on 2.5 nm tiles).  Per run the Fermi and
+    one-minus-Fermi polynomials (FFT coefficients at 2^15 points, precision
+    1e-12) are applied once, then J_x psi1, J_y psi1 and psi2 of every state
+    propagated together with tau = pi/2 (16 Bessel terms at 1e-14) and the
+    four current correlations formed every step; complex64 (and complex128
+    for the error) with FP64 sums.  state-steps/s = states x steps over the
+    propagation time of all runs (the Fermi operators excluded)."""
     import numpy as np
     import torch
     import tbplas_amd as tb
     from tbplas_amd import tbpm, _lib, tune
     L, K = args.L, args.vectors
+    per = max(1, min(args.ac_states, K))
     T = max(args.steps, 2)
-    tile = 1.5 if args.cond_tile is None else args.cond_tile
+    tile = 2.5 if args.cond_tile is None else args.cond_tile
     s = tb.make_graphene_sample(L)
     s.rescale_ham(9.0)
     ip, ix, hop, dx, dy = s.build_ham_dxy_device()
@@ -603,52 +609,59 @@ def ac_workload(args, exact=None):
     h.set_graph(not args.no_graph)
     order, tiles = cond_order(s, tile)
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
+    chunks = [(a, min(K, a + per)) for a in range(0, K, per)]
     res = {}
     for prec in ("fp32", "fp64"):
         H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision=prec,
                              handle=h, order=order, tiles=tiles)
         real = H.info()["real_values"]
-        shape = tune.apply(H, 3 * K, bes, not args.no_tune)
-        threads = H.blocking_launch(3 * K)[1]
-        sched = H.blocking_bytes(3 * K)
-        # Fermi operators alone (T = 1 also forms one correlation step)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        tbpm.corr_ac(H, bes, cF, cO, psi, 1, chk_thr=1e-4)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        c = tbpm.corr_ac(H, bes, cF, cO, psi, T, chk_thr=1e-4)
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
+        shape = tune.apply(H, 3 * per, bes, not args.no_tune)
+        threads = H.blocking_launch(3 * per)[1]
+        sched = H.blocking_bytes(3 * per)
+        t_prop, t_fermi, cs = 0.0, 0.0, []
+        for a, b in chunks:
+            # Fermi operators alone (T = 1 also forms one correlation step)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tbpm.corr_ac(H, bes, cF, cO, psi[a:b], 1, chk_thr=1e-4)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            cs.append(tbpm.corr_ac(H, bes, cF, cO, psi[a:b], T, chk_thr=1e-4))
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            t_prop += (t2 - t1) - (t1 - t0)
+            t_fermi += t1 - t0
+        c = np.concatenate(cs)
         kt = {}
         if args.profile_steps > 0:
-            # the library leaves the Fermi operators' launches untimed
+            # one run of the first chunk; the library leaves the Fermi
+            # operators' launches untimed
             h.set_kernel_timing(True)
             try:
-                tbpm.corr_ac(H, bes, cF, cO, psi, 1 + args.profile_steps, chk_thr=1e-4)
+                a, b = chunks[0]
+                tbpm.corr_ac(H, bes, cF, cO, psi[a:b], 1 + args.profile_steps, chk_thr=1e-4)
                 kt = h.kernel_times()
             finally:
                 h.set_kernel_timing(False)
         acc = None
         if exact is not None and L == args.L and K == exact[1].shape[0]:
-            # a 256-step run against the exact k-space correlations
+            # 256-step runs against the exact k-space correlations
             pts, cex = exact
-            ca = tbpm.corr_ac(H, bes, cF, cO, psi, AC_ACC_STEPS, chk_thr=1e-4)
+            ca = np.concatenate([tbpm.corr_ac(H, bes, cF, cO, psi[a:b], AC_ACC_STEPS,
+                                              chk_thr=1e-4) for a, b in chunks])
             acc = float(np.max(np.abs(ca[:, :, [t - 1 for t in pts]] - cex)))
         H.close()
-        t_prop = (t2 - t1) - (t1 - t0)
         sz = 8 if prec == "fp32" else 16
-        vsz = sz // 2 if real else sz
 
-        def bytes_of(kind, sched=sched, sz=sz, vsz=vsz):
+        def bytes_of(kind, sched=sched, sz=sz):
             if kind == "ac_corr":
                 # cols, J_x, J_y values (complex), psi1x, psi1y gathered, psi2 own row
-                return nnz * (4 + 2 * sz) + 3 * K * n * sz
-            return pass_bytes_fn(sched, n, nnz, 3 * K, sz, real)(kind)
+                return nnz * (4 + 2 * sz) + 3 * per * n * sz
+            return pass_bytes_fn(sched, n, nnz, 3 * per, sz, real)(kind)
         label = (lambda k, prec=prec, real=real, threads=threads:
-                 kernel_label(k, prec, 3 * K, real, threads))
+                 kernel_label(k, prec, 3 * per, real, threads))
         res[prec] = {"value": K * (T - 1) / t_prop, "ms_per_step": t_prop / (T - 1) * 1e3,
-                     "fermi_setup_s": t1 - t0, "shape": shape, "corr": c,
+                     "fermi_setup_s": t_fermi, "shape": shape, "corr": c,
                      "roofline": (roofline(kt, bytes_of, label, traffic_key(f"ac_{prec}", tile))
                                   if kt else None),
                      "err_vs_exact": acc}
@@ -661,8 +674,9 @@ def ac_workload(args, exact=None):
             "value": round(r["value"], 2), "unit": "state·steps/s", "higher_is_better": True,
             "dtype": "fp32", "ms_per_step": round(r["ms_per_step"], 4),
             "fermi_setup_s": round(r["fermi_setup_s"], 3),
-            "config": {"workload": f"graphene {L}x{L} AC, {K} states ({3 * K} vectors), "
-                                   f"{T} steps, {len(bes)} Bessel terms, Fermi terms {len(cF)}",
+            "config": {"workload": f"graphene {L}x{L} AC, {K} states in runs of {per} "
+                                   f"({3 * per} vectors each), {T} steps, {len(bes)} Bessel "
+                                   f"terms, Fermi terms {len(cF)}",
                        "n_orbitals": n, "nnz": nnz,
                        "layout": (f"tiles of {tile:g} nm" if tile > 0 else "caller order")
                        + passes_name(r["shape"]),

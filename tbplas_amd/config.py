@@ -22,8 +22,14 @@ look at:
                        Chebyshev terms per pass; same results, 0 keeps the
                        sample's order, 'auto' sizes the tiles for the
                        states per row (sample.auto_tile_nm)   default 'auto'
-    gpu['cond_tile_nm'] the same for AC and dynamic-polarisation runs (more
+    gpu['cond_tile_nm'] the same for dynamic-polarisation runs (more
                        vectors per row, so smaller tiles)     default 1.5
+    gpu['ac_states']   states per AC run (three propagated vectors each):
+                       fewer vectors per row allow larger tiles
+                       (8.4M-orbital graphene, complex64: 4 states on
+                       2.5 nm tiles 638 state-steps/s, 8 on 1.5 nm 519)
+                                                              default 4
+    gpu['ac_tile_nm']  the tile edge of AC runs               default 2.5
     gpu['dc_tile_nm']  the same for DC runs                   default 2.0
 """
 import pickle
@@ -71,6 +77,8 @@ class Config:
                     'fp32_norm_thr': 1.0e-4,
                     'tile_nm': 'auto',
                     'cond_tile_nm': 1.5,
+                    'ac_states': 4,
+                    'ac_tile_nm': 2.5,
                     'dc_tile_nm': 2.0}
         self._legal_params = dict()
         self.set_legal_params()

@@ -349,14 +349,14 @@ class Solver:
                                  g["Fermi_cheb_precision"])
         coef_omF = fermi_cheb_coef(g["nr_Fermi_fft_steps"], beta_re, mu_re, True,
                                    g["Fermi_cheb_precision"])
-        order, tiles = self._row_order(gpu, "cond_tile_nm")
+        order, tiles = self._row_order(gpu, "ac_tile_nm")
         H = Hamiltonian(self._sample.num_orb, indptr, indices, hop, dx, dy,
                         cur_scale=self._sample.rescale, precision=gpu["precision"],
                         handle=h, order=order, tiles=tiles)
         psi0 = self._states(num_sample)
         chk, thr = self._chk()
         T = g["nr_time_steps"]
-        per = max(1, min(21, int(gpu["max_vectors"])))
+        per = max(1, min(21, int(gpu["ac_states"]), int(gpu["max_vectors"])))
         tune.apply(H, 3 * min(per, num_sample), bes, gpu["autotune"])
         corr = np.zeros((4, T), dtype=np.complex128)
         for a, b in self._batches(num_sample, per):

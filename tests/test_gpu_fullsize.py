@@ -7,9 +7,10 @@ oracle.
   against exact k-space propagation.  Tolerances: complex128 1e-10;
   complex64 (FP32 storage and arithmetic, FP64 sums) 1e-6, about ten times
   the largest error measured over 2048 steps.
-* AC on the same sample (8 states, 1.5 nm tiles, three terms per pass)
-  against exact k-space propagation with the product's Fermi polynomial
-  over 256 steps: complex128 1e-11, complex64 1e-6 (correlations ~4e-2).
+* AC on the same sample (8 states in runs of 4 on 2.5 nm tiles with four
+  terms per pass, and in one run on 1.5 nm tiles with three) against exact
+  k-space propagation with the product's Fermi polynomial over 256 steps:
+  complex128 1e-11, complex64 1e-6 (correlations ~4e-2).
 * The disordered, field-threaded DC sample (1% vacancies, Anderson disorder
   U[-0.5, 0.5] eV, B = 20 T in the Landau gauge, 16 states, 5 energies,
   2.0 nm tiles with three and four terms per pass) on 24x24 cells against
@@ -109,30 +110,36 @@ def ac2048(graphene2048):
     cO = tbpm.fermi_cheb_coef(2 ** 15, beta, 0.0, True, 1e-12)
     exact = oracle.kspace_graphene_ac(L2048, L2048, s.hop_v[:3], s.dr[:3], s.rescale, cF, cO,
                                       psi, tau, AC_PTS)
-    order, tiles = s.tile_order(1.5, return_tiles=True)
-    return s, (ip, ix, hop, dx, dy), (order, tiles), bes, (cF, cO), psi, exact
+    return s, (ip, ix, hop, dx, dy), bes, (cF, cO), psi, exact
 
 
-@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("fp64", 1e-11)])
-def test_ac_benchmark_layout_vs_kspace(gpu, ac2048, prec, tol):
-    """AC on 8.4M-orbital graphene as the benchmark runs it (1.5 nm tiles,
-    three terms per pass, 24 vectors: J_x psi1, J_y psi1 and psi2 of 8
-    states) against exact k-space propagation with the same Fermi
-    polynomial, at sampled steps of 256 (the correlations are ~4e-2)."""
-    s, (ip, ix, hop, dx, dy), (order, tiles), bes, (cF, cO), psi, exact = ac2048
+@pytest.mark.parametrize("prec,tol,tile,terms,per", [
+    ("fp32", 1e-6, 2.5, 4, 4), ("fp64", 1e-11, 2.5, 4, 4),
+    ("fp32", 1e-6, 1.5, 3, 8), ("fp64", 1e-11, 1.5, 3, 8)])
+def test_ac_benchmark_layout_vs_kspace(gpu, ac2048, prec, tol, tile, terms, per):
+    """AC on 8.4M-orbital graphene as the benchmark and the Solver run it
+    (runs of 4 states, 12 vectors: J_x psi1, J_y psi1 and psi2 of each, on
+    2.5 nm tiles with four terms per pass) and as one run of 8 states (24
+    vectors, 1.5 nm tiles, three terms per pass), against exact k-space
+    propagation with the same Fermi polynomial, at sampled steps of 256
+    (the correlations are ~4e-2)."""
+    s, (ip, ix, hop, dx, dy), bes, (cF, cO), psi, exact = ac2048
+    order, tiles = s.tile_order(tile, return_tiles=True)
     h = tb._lib.Handle(0)
     try:
-        h.set_blocking(True, terms=3)
+        h.set_blocking(True, terms=terms)
         H = tbpm.Hamiltonian(s.num_orb, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision=prec,
                              handle=h, order=order, tiles=tiles)
-        assert H.blocking_info(24)[0] > 10000
-        c = tbpm.corr_ac(H, bes, cF, cO, psi, AC_STEPS, chk_thr=1e-4)
+        assert H.blocking_info(3 * per)[0] > 1000
+        c = np.concatenate([tbpm.corr_ac(H, bes, cF, cO, psi[a:a + per], AC_STEPS, chk_thr=1e-4)
+                            for a in range(0, len(psi), per)])
         H.close()
     finally:
         h.close()
     got = c[:, :, [t - 1 for t in AC_PTS]]
     err = float(np.max(np.abs(got - exact)))
-    print(f"AC {prec} 2048x2048, 8 states, {AC_STEPS} steps: max |C - C_exact| = {err:.3e} "
+    print(f"AC {prec} 2048x2048, 8 states in runs of {per}, {tile} nm tiles, {AC_STEPS} "
+          f"steps: max |C - C_exact| = {err:.3e} "
           f"(max |C| {np.max(np.abs(exact)):.3e})", flush=True)
     assert np.max(np.abs(c[:, :, 0])) == 0.0
     assert err < tol
