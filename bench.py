@@ -101,6 +101,9 @@ def parse():
     p.add_argument("--cond-tile", type=float, default=None,
                    help="dc and ac: tile edge in nm (0: the sample's own order; default "
                         "2.0 for dc, 2.5 for ac)")
+    p.add_argument("--dc-states", type=int, default=16,
+                   help="dc: states per run (two propagated vectors each), as the "
+                        "Solver's gpu['max_vectors']")
     p.add_argument("--ac-states", type=int, default=4,
                    help="ac: states per run (three propagated vectors each), as the "
                         "Solver's gpu['ac_states']")
@@ -703,6 +706,7 @@ def dc_workload(args):
     import tbplas_amd as tb
     from tbplas_amd import tbpm, _lib, tune
     L, K = 1024, 16
+    per = max(1, min(args.dc_states, K))
     T = max(2, min(args.steps, 1024))
     tile = 2.0 if args.cond_tile is None else args.cond_tile
     s = tb.make_graphene_sample(L, vacancies=tb.random_vacancies(2 * L * L, 0.01, 2024))
@@ -721,15 +725,18 @@ def dc_workload(args):
     H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision="fp32",
                          handle=h, order=order, tiles=tiles)
     real = H.info()["real_values"]
-    shape = tune.apply(H, 2 * K, bes, not args.no_tune)
-    threads = H.blocking_launch(2 * K)[1]
-    sched = H.blocking_bytes(2 * K)
+    shape = tune.apply(H, 2 * per, bes, not args.no_tune)
+    threads = H.blocking_launch(2 * per)[1]
+    sched = H.blocking_bytes(2 * per)
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
+    chunks = [(a, min(K, a + per)) for a in range(0, K, per)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    d, c = tbpm.corr_dc(H, bes, psi, T, ts, en, inds, chk_thr=1e-4)
+    parts = [tbpm.corr_dc(H, bes, psi[a:b], T, ts, en, inds, chk_thr=1e-4) for a, b in chunks]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    d = np.concatenate([p[0] for p in parts])
+    c = np.concatenate([p[1] for p in parts])
     kt = {}
     if args.profile_steps > 0:
         h.set_kernel_timing(True)
@@ -738,7 +745,8 @@ def dc_workload(args):
             enp = np.array([0.5 * i * s.energy_range / Tp - s.energy_range / 2.
                             for i in range(2 * Tp)])
             indp = np.array([int(np.argmin(np.abs(enp - e))) for e in (-1.0, -0.5, 0.0, 0.5, 1.0)])
-            tbpm.corr_dc(H, bes, psi, Tp, ts, enp, indp, chk_thr=1e-4)
+            tbpm.corr_dc(H, bes, psi[chunks[0][0]:chunks[0][1]], Tp, ts, enp, indp,
+                         chk_thr=1e-4)
             kt = h.kernel_times()
         finally:
             h.set_kernel_timing(False)
@@ -747,22 +755,22 @@ def dc_workload(args):
 
     def bytes_of(kind):
         if kind == "dc_qe":    # psi(+t), psi(-t), psi0 read; QE read and written (FP64)
-            return 3 * K * n * 8 + 2 * ne * K * n * 16
+            return 3 * per * n * 8 + 2 * ne * per * n * 16
         if kind == "dc_dot":   # J_a psi(t) read, J_a QE read (FP64)
-            return 2 * K * n * 8 + 2 * ne * K * n * 16
-        return pass_bytes_fn(sched, n, nnz, 2 * K, 8, real)(kind)
+            return 2 * per * n * 8 + 2 * ne * per * n * 16
+        return pass_bytes_fn(sched, n, nnz, 2 * per, 8, real)(kind)
     value = 2 * K * (2 * T - 1) / dt
     return {"metric": "DC correlation step·vectors/s (disordered field-threaded graphene)",
             "value": round(value, 2), "unit": "step·vectors/s", "higher_is_better": True,
             "dtype": "fp32", "seconds": round(dt, 3),
             "config": {"workload": f"graphene {L}x{L}, 1% vacancies, Anderson W=1 eV, B=20 T, "
-                                   f"DC at 5 energies, {K} states, {T} steps",
+                                   f"DC at 5 energies, {K} states in runs of {per}, {T} steps",
                        "n_orbitals": n, "nnz": nnz, "energies": en[inds].tolist(),
                        "layout": (f"tiles of {tile:g} nm" if tile > 0 else "caller order")
                        + passes_name(shape),
                        "launch_shape": shape_json(shape)},
             "roofline": (roofline(kt, bytes_of,
-                                  lambda k: kernel_label(k, "fp32", 2 * K, real, threads),
+                                  lambda k: kernel_label(k, "fp32", 2 * per, real, threads),
                                   traffic_key("dc", tile))
                          if kt else None),
             "dos_corr_t1_mean": float(np.mean(d[:, 0]).real),
