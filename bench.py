@@ -96,8 +96,9 @@ def parse():
     p.add_argument("--tile", type=float, default=None,
                    help="dos and batch: tile edge in nm (default: the library's "
                         "auto_tile_nm for each leg's states and precision, as the Solver's "
-                        "DOS driver picks it: 5.0 nm for 8 complex64 states a row, 3.4 nm "
-                        "for 8 complex128 states, 5.0 nm for the batch's 4)")
+                        "DOS driver picks it, to 0.1 nm: 5.0 nm for 8 complex64 states a "
+                        "row, 3.4 nm for 8 complex128 states, 7.5 nm for one, 7.4 nm for "
+                        "the batch's 4 complex64 states)")
     p.add_argument("--cond-tile", type=float, default=None,
                    help="dc and ac: tile edge in nm (0: the sample's own order; default "
                         "2.0 for dc, 2.5 for ac)")
@@ -453,9 +454,10 @@ def pass_bytes_fn(sched, n, nnz, K, s, real):
 # Headline: DOS on 8.4M-orbital graphene
 # ---------------------------------------------------------------------------
 def leg_tile(args, K, prec):
-    """Tile edge (nm) of a DOS leg: --tile, else the library's rule."""
+    """Tile edge (nm) of a DOS leg: --tile, else the library's rule (to
+    0.1 nm)."""
     from tbplas_amd.sample import auto_tile_nm
-    return args.tile if args.tile is not None else round(auto_tile_nm(K, prec), 2)
+    return args.tile if args.tile is not None else round(auto_tile_nm(K, prec), 1)
 
 
 def traffic_key(workload, tile):
@@ -793,7 +795,9 @@ def batch_workload(args):
     vac = rng.uniform(0.0, 0.03, 32)
     Ws = rng.uniform(0.0, 2.0, 32)
     K = 4
-    tile = leg_tile(args, K, "fp32") if args.order == "tiles" else 0
+    tiles_on = args.order == "tiles"
+    tile = leg_tile(args, K, "fp32") if tiles_on else 0
+    tile64 = leg_tile(args, K, "fp64") if tiles_on else 0
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
     insts = []
@@ -805,13 +809,15 @@ def batch_workload(args):
         ip, ix, dt = s.build_ham_csr_device()
         insts.append((s.num_orb, int(ix.numel()), ip, ix, dt,
                       tb.states.random_states(s.num_orb, tb.states.sample_seeds(300 + i, K)),
-                      s.tile_order(tile, return_tiles=True) if tile else (None, None)))
+                      {"fp32": s.tile_order(tile, return_tiles=True) if tile else (None, None),
+                       "fp64": (s.tile_order(tile64, return_tiles=True) if tile64
+                                else (None, None))}))
     bes = tbpm.bessel_series(math.pi, 1e-14)
     total = args.warmup + args.steps + args.profile_steps
     res = {}
     for prec in ("fp32", "fp64"):
-        hams = [tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h, order=ot[0],
-                                 tiles=ot[1])
+        hams = [tbpm.Hamiltonian(n, ip, ix, dt, precision=prec, handle=h, order=ot[prec][0],
+                                 tiles=ot[prec][1])
                 for (n, nnz, ip, ix, dt, _, ot) in insts]
         shape = tune.apply(hams, K, bes, not args.no_tune)
         real = all(H.info()["real_values"] for H in hams)
@@ -836,6 +842,8 @@ def batch_workload(args):
                        "nnz_total": nnz_tot, "launch_shape": shape_json(shape),
                        "layout": (f"tiles of {tile:g} nm" if tile
                                   else "caller order") + passes_name(shape),
+                       "layout_fp64": (f"tiles of {tile64:g} nm" if tile64 else "caller order")
+                       + passes_name(res["fp64"][2]),
                        "hipgraph": not args.no_graph},
             "roofline": (roofline(kt, pass_bytes_fn(sched, n_tot, nnz_tot, K, 8, real),
                                   lambda k: kernel_label(k, "fp32", K, real, 1024),

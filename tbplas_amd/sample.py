@@ -76,10 +76,13 @@ def auto_tile_nm(vectors, precision="fp32", density=GRAPHENE_ORB_DENSITY):
     B: twice the tile, plus its first ring twice, its second twice and its
     third once) fit 150 KB of LDS, with 2 rho a^2 tile rows and a ring of
     2.96 sqrt(rho) a rows (nearest-neighbour lattices; fitted on graphene,
-    17.6 a + 2 per ring), within [1.5, 5.0] nm.  A tile any larger is split
+    17.6 a + 2 per ring), within [1.5, 7.5] nm.  A tile any larger is split
     by the library into strips, which costs more than it saves.  Measured on
     8.4M-orbital graphene (step·vectors/s): complex64 K=8 1607 at 5.0 nm,
-    1458 at 3.5 nm; K=16 1643 at 3.0 nm, 1436 at 3.54 nm (split)."""
+    1458 at 3.5 nm; K=16 1643 at 3.0 nm, 1436 at 3.54 nm (split);
+    complex128 K=1 (capped) 591 steps/s at 7.0 nm, 587 at 5.0, 570 at 9.0;
+    and a batch of 32 disordered samples with complex64 K=4 (7.35 nm)
+    13426 instance-step·vectors/s at 7.0 nm against 11720 at 5.0."""
     p2, p3 = 1, 3            # vector slots of the device layout
     while p2 < vectors:
         p2 <<= 1
@@ -90,7 +93,7 @@ def auto_tile_nm(vectors, precision="fp32", density=GRAPHENE_ORB_DENSITY):
     a2 = 2.0 * density                            # tile rows per nm^2, A and B
     a1 = 5.0 * 2.96 * math.sqrt(density)          # ring rows per nm of edge
     edge = (-a1 + math.sqrt(a1 * a1 + 4.0 * a2 * rows)) / (2.0 * a2)
-    return float(min(5.0, max(1.5, edge)))
+    return float(min(7.5, max(1.5, edge)))
 
 
 def random_vacancies(n_orb, fraction, seed):
