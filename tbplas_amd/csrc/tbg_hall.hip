@@ -39,21 +39,25 @@ __global__ void k_jackson_scale(double2* __restrict__ mu, int N, const double* _
 // j < N, with P and W column-major (column m of P at P + m * np).  A
 // workgroup (4 waves) owns a 64 x 64 tile of the output and one split of
 // k; it stages 16 rows of k of both operands in LDS (real and imaginary
-// parts apart) and each wave accumulates a 32 x 32 sub-tile as 2 x 2 blocks
-// of v_mfma_f64_16x16x4f64: re += Pr Wr + Pi Wi, im += Pr Wi - Pi Wr.
+// parts apart, each column's 16 values together) and each wave accumulates
+// a 32 x 32 sub-tile as 2 x 2 blocks of v_mfma_f64_16x16x4f64:
+// re += Pr Wr + Pi Wi, im += Pr Wi - Pi Wr.
 // Operand layout of the instruction: lane i + 16 k holds A[i][k] and
 // B[k][i]; the result D[i][j] sits in lane j + 16 (i % 4), element i / 4
 // (checked on the device with known operands).
 constexpr int kGramTile = 64;   // output rows / columns per workgroup
 constexpr int kGramK = 16;      // rows of k per LDS stage
-constexpr int kGramLd = 80;     // LDS row stride in doubles: the MFMA operand
-                                // reads of rows k and k + 1 land 32 banks apart
+constexpr int kGramLd = kGramK + 1;   // LDS stride in doubles from one operand
+                                       // column to the next (its 16 values of k,
+                                       // one pad): 34 banks, so the 16 lanes of
+                                       // an MFMA operand read land on distinct
+                                       // banks
 typedef double gram_d4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(256)
 k_gram(const double2* __restrict__ P, const double2* __restrict__ W, int64_t np, int rows,
        int N, int64_t kchunk, double2* __restrict__ part) {
-    __shared__ double lds[4][kGramK][kGramLd];   // Pr, Pi, Wr, Wi
+    __shared__ double lds[4][kGramTile][kGramLd];   // Pr, Pi, Wr, Wi: [column][k]
     const int m0 = blockIdx.y * kGramTile, j0 = blockIdx.x * kGramTile;
     const int64_t kb = (int64_t)blockIdx.z * kchunk;
     const int64_t ke = kb + kchunk < np ? kb + kchunk : np;
@@ -88,10 +92,10 @@ k_gram(const double2* __restrict__ P, const double2* __restrict__ W, int64_t np,
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int sm = (t >> 4) + 16 * q;
-            lds[0][sk][sm] = pq[q].x;
-            lds[1][sk][sm] = pq[q].y;
-            lds[2][sk][sm] = xq[q].x;
-            lds[3][sk][sm] = xq[q].y;
+            lds[0][sm][sk] = pq[q].x;
+            lds[1][sm][sk] = pq[q].y;
+            lds[2][sm][sk] = xq[q].x;
+            lds[3][sm][sk] = xq[q].y;
         }
         __syncthreads();
         if (k0 + kGramK < ke) fetch(k0 + kGramK);
@@ -100,10 +104,10 @@ k_gram(const double2* __restrict__ P, const double2* __restrict__ W, int64_t np,
             double ar[2], ai[2], br[2], bi[2];
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
-                ar[a] = lds[0][kk + lk][wm + 16 * a + li];
-                ai[a] = lds[1][kk + lk][wm + 16 * a + li];
-                br[a] = lds[2][kk + lk][wn + 16 * a + li];
-                bi[a] = lds[3][kk + lk][wn + 16 * a + li];
+                ar[a] = lds[0][wm + 16 * a + li][kk + lk];
+                ai[a] = lds[1][wm + 16 * a + li][kk + lk];
+                br[a] = lds[2][wn + 16 * a + li][kk + lk];
+                bi[a] = lds[3][wn + 16 * a + li][kk + lk];
             }
             // eight independent accumulators between two products into
             // the same one; each still gets its products in the order
