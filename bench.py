@@ -293,12 +293,28 @@ def finish_exact(proc):
 # ---------------------------------------------------------------------------
 # GPU helpers
 # ---------------------------------------------------------------------------
-def src_hash():
-    """Hash of the kernel sources and the header: a counter profile applies
-    to the build it was measured on."""
+# Sources a workload's kernels are not built from: a change there leaves
+# its counter profile valid (every other library source and the header
+# count).  The Hall moments run the one-term kernel and k_gram, no tile
+# pass; the propagation workloads run no Gram block.
+_NOT_BUILT_FROM = {
+    "hall": ("tbg_mlp.hip", "tbg_super.hip", "tbg_static.hip", "tbg_block_pass.h",
+             "tbg_block.hip", "tbg_pass_"),
+    "prop": ("tbg_mlp.hip", "tbg_super.hip", "tbg_static.hip", "tbg_hall.hip"),
+}
+
+
+def src_hash(workload=None):
+    """Hash of the library sources a workload's kernels are built from
+    (every source when workload is None) and the header: a counter profile
+    applies to the build it was measured on."""
+    skip = () if workload is None else _NOT_BUILT_FROM[
+        "hall" if workload.startswith("hall") else "prop"]
     h = hashlib.sha256()
     csrc = os.path.join(HERE, "tbplas_amd", "csrc")
     for name in sorted(os.listdir(csrc)):
+        if any(name.startswith(x) for x in skip):
+            continue
         with open(os.path.join(csrc, name), "rb") as f:
             h.update(name.encode() + b"\0" + f.read())
     with open(os.path.join(HERE, "include", "tbplas_amd.h"), "rb") as f:
@@ -316,7 +332,7 @@ def counter_traffic(workload, kernel):
     except (OSError, ValueError):
         return None
     e = d.get(workload, {}).get(kernel)
-    if not e or e.get("src_hash") != src_hash():
+    if not e or e.get("src_hash") != src_hash(workload):
         return None
     return e
 
