@@ -5,9 +5,10 @@ Headline workload (the metric's configuration): pristine graphene 2048 x
 2048 with periodic boundaries (8,388,608 orbitals, 25,165,824 non-zeros),
 rescaled by 9 eV, 8 random-phase states propagated together (21 Bessel
 terms at precision 1e-15, i.e. 20 SpMM terms per time step) with the DOS
-correlation C(t) formed every step; rows in 5.0 nm spatial tiles so the
-tuner can run several Chebyshev terms per pass.  complex64 storage and
-arithmetic, FP64 correlation sums.
+correlation C(t) formed every step; rows in square spatial tiles (the
+library's tile rule for the leg's row width: 5.0 nm for 8 complex64
+states) so the tuner can run several Chebyshev terms per pass.  complex64
+storage and arithmetic, FP64 correlation sums.
 
 A "step" is one time step of all 8 states; ``value`` is step*vectors/s over
 the timed steps (--steps, after --warmup untimed ones) with the states
@@ -23,12 +24,13 @@ resident in HBM, replayed from a hipGraph.  The one JSON line also holds
     from the tables by the library), ``frac`` = those bytes per second over
     8 TB/s.  ``traffic`` is the counter-measured L2-to-HBM bytes per launch
     (FETCH_SIZE and WRITE_SIZE, copy-calibrated) from profiles/traffic.json
-    when that file was measured on this build of the kernel sources, else
-    null.  ``one_term_equivalent`` restates the speed as one-term-per-pass
+    when that file was measured on the sources the workload's kernels are
+    built from and on the same tiles, else null.  ``one_term_equivalent`` restates the speed as one-term-per-pass
     algorithmic bytes (five vector streams per state and H once per term)
     per measured time per term;
-  * ``fp64_k8`` (the same 8 states in complex128) and ``fp64_k1`` (one
-    complex128 state: the speedup denominator);
+  * ``fp64_k8`` (the same 8 states in complex128, 3.4 nm tiles) and
+    ``fp64_k1`` (one complex128 state, 7.5 nm tiles: the speedup
+    denominator);
   * ``ct_err_fp32_vs_exact`` / ``ct_err_fp64_vs_exact``: max |C(t) - C_exact|
     over the 8 states at sampled steps of a 2048-step run (the timed steps
     are part of it), C_exact from k-space propagation of the same states
@@ -36,10 +38,11 @@ resident in HBM, replayed from a hipGraph.  The one JSON line also holds
   * ``cpu_baseline``: the reference's own Fortran tbpm_dos (compiled from its
     sources into oracle/_ref) on the host's physical cores;
   * sub-records for the other workloads: ``ac`` (AC conductivity on the same
-    sample), ``dc`` (disordered field-threaded DC), ``batch`` (32 uneven
-    instances), ``latency`` (131,072 orbitals) and ``assembly`` (GPU CSR
-    assembly at 8.4M orbitals and 10^7 atoms), each with its dominant
-    kernel's roofline.
+    sample, 4 states per run), ``dc`` (disordered field-threaded DC),
+    ``batch`` (32 uneven instances), ``latency`` (131,072 orbitals),
+    ``assembly`` (GPU CSR assembly at 8.4M orbitals and 10^7 atoms) and
+    ``hall`` (the Kubo-Bastin moments of the reference's Hall example, on
+    the FP64 matrix cores), each with its dominant kernel's roofline.
 
 ``--workload X`` runs one of them alone (for profiling); ``--gpus N`` with
 N > 1 exits: this build drives one GPU.
