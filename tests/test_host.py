@@ -136,6 +136,44 @@ def test_bench_refuses_multi_gpu():
     assert r.returncode == 2 and "one GPU" in r.stderr
 
 
+def test_bench_traffic_keyed_by_sources_and_layout(tmp_path, monkeypatch):
+    """A counter profile applies only to the workload and tile layout it was
+    measured on and to the sources that workload's kernels are built from:
+    a change in the tile pass leaves the Hall profile valid and a change in
+    the Hall code leaves the propagation profiles valid."""
+    import json
+    sys.path.insert(0, ROOT)
+    import bench
+    csrc = tmp_path / "tbplas_amd" / "csrc"
+    csrc.mkdir(parents=True)
+    (tmp_path / "include").mkdir()
+    (tmp_path / "include" / "tbplas_amd.h").write_text("header")
+    for name in ("tbg_block_pass.h", "tbg_pass_f32_t4_real.hip", "tbg_hall.hip", "tbg_prop.hip",
+                 "tbg_mlp.hip"):
+        (csrc / name).write_text(name)
+    monkeypatch.setattr(bench, "HERE", str(tmp_path))
+    dos, hall = "dos_fp32_k8@5nm", "hall"
+    h_dos, h_hall = bench.src_hash(dos), bench.src_hash(hall)
+    (tmp_path / "profiles").mkdir()
+    (tmp_path / "profiles" / "traffic.json").write_text(json.dumps({
+        dos: {"k_pass": {"src_hash": h_dos, "bytes_per_launch": 5}},
+        hall: {"k_gram": {"src_hash": h_hall, "bytes_per_launch": 7}}}))
+    assert bench.counter_traffic(dos, "k_pass")["bytes_per_launch"] == 5
+    assert bench.counter_traffic("dos_fp32_k8@3.4nm", "k_pass") is None
+    assert bench.counter_traffic(dos, "k_other") is None
+    (csrc / "tbg_hall.hip").write_text("changed")
+    assert bench.src_hash(dos) == h_dos and bench.src_hash(hall) != h_hall
+    assert bench.counter_traffic(dos, "k_pass") is not None
+    assert bench.counter_traffic(hall, "k_gram") is None
+    (csrc / "tbg_hall.hip").write_text("tbg_hall.hip")
+    (csrc / "tbg_mlp.hip").write_text("changed")        # built into neither
+    assert bench.src_hash(dos) == h_dos and bench.src_hash(hall) == h_hall
+    (csrc / "tbg_pass_f32_t4_real.hip").write_text("changed")
+    assert bench.src_hash(dos) != h_dos and bench.src_hash(hall) == h_hall
+    (csrc / "tbg_prop.hip").write_text("changed")       # built into both
+    assert bench.src_hash(hall) != h_hall
+
+
 def test_product_has_no_oracle_import():
     """The product path never imports the oracle (no CPU fallback)."""
     pkg = os.path.join(ROOT, "tbplas_amd")
