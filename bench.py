@@ -6,8 +6,9 @@ Headline workload (the metric's configuration): pristine graphene 2048 x
 rescaled by 9 eV, 8 random-phase states propagated together (21 Bessel
 terms at precision 1e-15, i.e. 20 SpMM terms per time step) with the DOS
 correlation C(t) formed every step; rows in square spatial tiles (the
-library's tile rule for the leg's row width: 5.0 nm for 8 complex64
-states) so the tuner can run several Chebyshev terms per pass.  complex64
+library's tile rule for the leg's row width and terms per pass: 4.8 nm
+for 8 complex64 states with five terms) so the tuner can run several
+Chebyshev terms per pass.  complex64
 storage and arithmetic, FP64 correlation sums.
 
 A "step" is one time step of all 8 states; ``value`` is step*vectors/s over
@@ -94,13 +95,13 @@ def parse():
     p.add_argument("--workload", choices=WORKLOADS, default="all")
     p.add_argument("--order", choices=("caller", "tiles"), default="tiles",
                    help="row order of the device layout: the sample's own, or square "
-                        "tiles (Sample.tile_order), which lets the tuner choose three or "
-                        "four Chebyshev terms per pass")
+                        "tiles (Sample.tile_order), which lets the tuner choose three, "
+                        "four or five Chebyshev terms per pass")
     p.add_argument("--tile", type=float, default=None,
                    help="dos and batch: tile edge in nm (default: the library's "
                         "auto_tile_nm for each leg's states and precision, as the Solver's "
-                        "DOS driver picks it, to 0.1 nm: 5.0 nm for 8 complex64 states a "
-                        "row, 3.4 nm for 8 complex128 states, 7.5 nm for one, 7.4 nm for "
+                        "DOS driver picks it, to 0.1 nm: 4.8 nm for 8 complex64 states a "
+                        "row, 3.4 nm for 8 complex128 states, 7.5 nm for one, 7.1 nm for "
                         "the batch's 4 complex64 states)")
     p.add_argument("--cond-tile", type=float, default=None,
                    help="dc and ac: tile edge in nm (0: the sample's own order; default "
@@ -346,8 +347,8 @@ def passes_terms(shape):
 
 
 def passes_name(shape):
-    return {1: ", one term per pass", 3: ", three terms per pass",
-            4: ", four terms per pass"}.get(passes_terms(shape), ", several terms per pass")
+    return {1: ", one term per pass", 3: ", three terms per pass", 4: ", four terms per pass",
+            5: ", five terms per pass"}.get(passes_terms(shape), ", several terms per pass")
 
 
 def shape_json(shape):
@@ -877,8 +878,8 @@ def latency_workload(args):
     """Pristine graphene 256x256 (131,072 orbitals, rescale 9), one state,
     1024 DOS steps (21 Bessel terms at 1e-15): the case whose vectors fit in
     the last-level cache, reported as microseconds per step, complex128 and
-    complex64, hipGraph replay, 2 nm tiles (the tuner picks one, three or
-    four terms per pass)."""
+    complex64, hipGraph replay, 2 nm tiles (the tuner picks one, three,
+    four or five terms per pass)."""
     import tbplas_amd as tb
     from tbplas_amd import tbpm, _lib, tune
     L, T = 256, 1024

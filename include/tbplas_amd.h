@@ -57,9 +57,9 @@ int tbg_set_tuning(tbg_handle *h, int blocks_per_cu, int stage);
  * tbplas/fortran/csr.f90:9-14). */
 int tbg_set_real_layout(tbg_handle *h, int enable);
 /* Several Chebyshev terms per pass for Hamiltonians created with spatial
- * tiles (tbg_ham_create_ordered): enable = 1 or 3 three terms, 4 four terms
- * (tiles with one more halo ring, so smaller tiles fit the LDS), 0 one term
- * per pass (same states, for comparison).  max_rows > 0 caps the rows one
+ * tiles (tbg_ham_create_ordered): enable = 1 or 3 three terms, 4 four terms,
+ * 5 five terms (each term more needs one more halo ring, so smaller tiles
+ * fit the LDS), 0 one term per pass (same states, for comparison).  max_rows > 0 caps the rows one
  * workgroup owns (tiles are merged up to it; 0 = sized from the LDS).
  * Default (3, 0).  No reference counterpart. */
 int tbg_set_blocking(tbg_handle *h, int enable, int max_rows);

@@ -177,12 +177,12 @@ class Handle:
         self.tuning = (int(blocks_per_cu), bool(stage))
 
     def set_blocking(self, enable=True, max_rows=0, terms=3):
-        """Several Chebyshev terms per pass (terms = 3 or 4) for
+        """Several Chebyshev terms per pass (terms = 3, 4 or 5) for
         Hamiltonians with spatial tiles (same states; off runs one term per
         pass); max_rows caps the rows one workgroup owns (0: sized from the
         LDS)."""
-        if terms not in (3, 4):
-            raise ValueError("terms per pass must be 3 or 4")
+        if terms not in (3, 4, 5):
+            raise ValueError("terms per pass must be 3, 4 or 5")
         check(lib().tbg_set_blocking(self._h, int(terms) if enable else 0, int(max_rows)))
 
     def set_kernel_timing(self, enable=True):

@@ -15,8 +15,10 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libtbplas_amd.so")
 # the longest compiles first: the one-term kernels, then the tile-pass instances
-SOURCES = ["tbg_prop.hip", "tbg_pass_f32_t4_real.hip", "tbg_pass_f32_t4_cplx.hip",
-           "tbg_pass_f64_t4_real.hip", "tbg_pass_f64_t4_cplx.hip", "tbg_pass_t2.hip",
+SOURCES = ["tbg_pass_f32_t5_real.hip", "tbg_pass_f32_t5_cplx.hip", "tbg_pass_f64_t5_real.hip",
+           "tbg_pass_f64_t5_cplx.hip", "tbg_prop.hip", "tbg_pass_f32_t4_real.hip",
+           "tbg_pass_f32_t4_cplx.hip", "tbg_pass_f64_t4_real.hip", "tbg_pass_f64_t4_cplx.hip",
+           "tbg_pass_t2.hip",
            "tbg_pass_f32_t3_real.hip", "tbg_pass_f32_t3_cplx.hip", "tbg_pass_f64_t3_real.hip",
            "tbg_pass_f64_t3_cplx.hip", "tbg_super.hip", "tbg_api.hip", "tbg_cond.hip",
            "tbg_block.hip", "tbg_mlp.hip", "tbg_csr.hip", "tbg_hall.hip", "tbg_static.hip"]

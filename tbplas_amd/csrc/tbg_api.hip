@@ -380,9 +380,9 @@ int tbg_set_blocking(tbg_handle* h, int enable, int max_rows) {
     return guard([&] {
         check_handle(h);
         TBG_REQUIRE(max_rows >= 0, "max_rows must be >= 0");
-        TBG_REQUIRE(enable == 0 || enable == 1 || enable == 3 || enable == 4,
-                    "enable must be 0 (off), 1 or 3 (three terms per pass) or 4 (four)");
-        h->blocking = enable == 0 ? 0 : enable == 4 ? 4 : 3;
+        TBG_REQUIRE(enable == 0 || enable == 1 || enable == 3 || enable == 4 || enable == 5,
+                    "enable must be 0 (off), 1 or 3 (three terms per pass), 4 or 5");
+        h->blocking = enable == 0 ? 0 : enable == 1 ? 3 : enable;
         h->block_rows = max_rows;
     });
 }

@@ -24,13 +24,13 @@ namespace tbg {
 // owned rows l < n[0].
 struct BlockMeta {
     int64_t offA, offL, offG;
-    int n[4];
+    int n[5];
     int nA, nB, w, r0;
 };
 
 struct Blocking {
     int row_bytes = 0;       // bytes of one row of states it was sized for
-    int depth = 2;           // halo rings: passes of up to depth + 1 terms
+    int depth = 2;           // halo rings (2 to 4): passes of up to depth + 1 terms
     int nblocks = 0;
     std::vector<int> seg_block;   // blocks of segment i: [seg_block[i], seg_block[i+1])
     int nA_max = 0, nB_max = 0;
@@ -63,7 +63,7 @@ struct BlockSrc {
     size_t vsz = 0;            // bytes per stored value
 };
 
-// Tables with depth (2 or 3) halo rings for tiles given per segment
+// Tables with depth (2 to 4) halo rings for tiles given per segment
 // (absolute row offsets, first = the segment's start, last = its end);
 // blocks never cross a segment end; null when a block-tile would not fit
 // the LDS of one workgroup.
@@ -101,7 +101,7 @@ struct PassArgs {
     double ar[2], ai[2];
     double ar1[2], ai1[2];   // alpha of the first term (PM_M1)
     double beta;
-    double c[6][2];          // c_{m-2} .. c_{m+3} per coefficient class
+    double c[7][2];          // c_{m-2} .. c_{m+4} per coefficient class
     uint64_t bwd_mask, cls_mask;
     int K, lgG;
     int nA_max, nB_max;      // widest A and B: B starts at row nA_max of the LDS,

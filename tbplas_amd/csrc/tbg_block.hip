@@ -49,7 +49,7 @@ constexpr int kBlockLdsTarget = 72 * 1024;
 std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
                                          const std::vector<std::vector<int64_t>>& segs,
                                          int row_bytes, int depth) {
-    TBG_REQUIRE(depth == 2 || depth == 3, "internal: halo depth must be 2 or 3");
+    TBG_REQUIRE(depth >= 2 && depth <= 4, "internal: halo depth must be 2 to 4");
     const int64_t n = src.nrows, ns = src.nslices;
     const size_t vsz = src.vsz;
     std::vector<int> cols((size_t)src.padded), sw((size_t)ns);
@@ -71,7 +71,8 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
     // owned rows per block-tile: whole tiles, up to the row target
     const int lds = std::min(h->max_lds, kBlockLdsTarget);
     // LDS rows (A + B) per owned row: about 2.6 with two rings, 3 with three
-    int64_t target = std::max<int64_t>(64, (int64_t)(lds / ((depth == 2 ? 2.6 : 3.0) * row_bytes)));
+    int64_t target = std::max<int64_t>(
+        64, (int64_t)(lds / ((depth == 2 ? 2.6 : depth == 3 ? 3.0 : 3.4) * row_bytes)));
     // small samples: at least two workgroups per CU
     int64_t covered = 0;
     for (const auto& t : segs) covered += t.back() - t.front();
@@ -140,7 +141,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
                         }
                     }
                 };
-                size_t ends[4];
+                size_t ends[5];
                 ends[0] = rows.size();
                 for (int k = 1; k <= depth; ++k) {   // E1 .. E_depth
                     ring(k == 1 ? base : ends[k - 2], ends[k - 1]);
@@ -173,7 +174,7 @@ std::shared_ptr<Blocking> build_blocking(tbg_handle* h, const BlockSrc& src,
                 m.offA = (int64_t)base;
                 m.offL = (int64_t)colL.size();
                 m.offG = (int64_t)colG.size();
-                for (int k = 0; k < 4; ++k) m.n[k] = (int)(ends[std::min(k, depth)] - base);
+                for (int k = 0; k < 5; ++k) m.n[k] = (int)(ends[std::min(k, depth)] - base);
                 m.nB = nB;
                 m.nA = nA;
                 m.w = wA;
@@ -339,6 +340,8 @@ void prepare_term3(int max_lds) {
     allow_lds_t<double, 2>(max_lds);
     allow_lds_t<double, 3>(max_lds);
     allow_lds_t<double, 4>(max_lds);
+    allow_lds_t<float, 5>(max_lds);
+    allow_lds_t<double, 5>(max_lds);
 }
 
 template <typename R>
@@ -366,7 +369,8 @@ void launch_pass(int mode, int T, bool real_vals, const PassArgs& a, const Block
         case 2: TBG_PASS_T(2); break;
         case 3: TBG_PASS_T(3); break;
         case 4: TBG_PASS_T(4); break;
-        default: throw Error{TBG_EINVAL, "internal: two to four terms per pass"};
+        case 5: TBG_PASS_T(5); break;
+        default: throw Error{TBG_EINVAL, "internal: two to five terms per pass"};
     }
 #undef TBG_PASS_T
 }

@@ -84,17 +84,19 @@ __device__ __forceinline__ Vec<R, V> tab_sum(const RowTab<R, RV, NJ>& t, GATHER 
 //      in (the still unused) area B: read next to this tile's gathers of the
 //      same rows, it mostly hits L2;
 //   2: v_{m+1} on n[T-2] rows from A and the parked v_{m-1}, into B;
-//   3: v_{m+2} on n[T-3] rows from B and v_m.  For T = 4 it goes over v_m in
-//      A (each row reads its own v_m first, and the phase gathers only from
-//      B) and the owned rows' output so far is stored in acc;
-//   4: v_{m+3} on the owned rows from A (v_{m+2}) and B (v_{m+1}).
+//   3: v_{m+2} on n[T-3] rows from B and v_m.  For T >= 4 it goes over v_m
+//      in A (each row reads its own v_m first, and the phase gathers only
+//      from B) and the owned rows' output so far waits in a register queue;
+//   4: v_{m+3} from A (v_{m+2}) and B (v_{m+1}): for T = 4 on the owned
+//      rows (the last phase); for T = 5 on n[1] rows over v_{m+1} in B;
+//   5: (T = 5) v_{m+4} on the owned rows from B (v_{m+3}) and A (v_{m+2}).
 // The last phase works on the owned rows only: acc gets the pass's terms
 // (read first, formed from v_0 for PM_M1, or the partial of phase 3), the
 // last two terms go to q1, q2 unless the pass ends the step, and the
 // ending modes sum the FP64 norm / dot per tile into part.
 template <typename R, int V, bool RV, int MODE, int NT, int T>
 __global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
-    static_assert(T >= 2 && T <= 4, "two to four terms per pass");
+    static_assert(T >= 2 && T <= 5, "two to five terms per pass");
     static_assert(T > 2 || MODE >= PM_LAST, "a pair only ends a step");
     using C = typename CplxT<R>::T;
     using VT = Vec<R, V>;
@@ -121,7 +123,7 @@ __global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
     const int rpp = NT >> lgG;
     const int k0 = g * V;
 
-    R ar[V], ai[V], cf[6][V];
+    R ar[V], ai[V], cf[T == 5 ? 7 : 6][V];   // c_{m-2} .. c_{m+3} (c_{m+4})
     R ar1[M1 ? V : 1], ai1[M1 ? V : 1];
 #pragma unroll
     for (int v = 0; v < V; ++v) {
@@ -135,7 +137,7 @@ __global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
             ai1[v] = bwd ? (R)(-a.ai1[cl]) : (R)a.ai1[cl];
         }
 #pragma unroll
-        for (int q = 0; q < 6; ++q) cf[q][v] = (R)a.c[q][cl];
+        for (int q = 0; q < (T == 5 ? 7 : 6); ++q) cf[q][v] = (R)a.c[q][cl];
     }
     const R beta = (R)a.beta;
     constexpr size_t vsz = RV ? sizeof(R) : sizeof(C);
@@ -284,6 +286,94 @@ __global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
                       axpy<R, V>(o, cf[2], vm);
                       axpy<R, V>(o, cf[3], lds(ldsB, l));
                       axpy<R, V>(o, cf[4], v2);
+                      finish(vidx, o);
+                  });
+        } else if constexpr (T == 5) {
+            // As with four terms, the owned rows' output waits in a register
+            // queue (part[HELD - 1] leaves first; rows beyond HELD per lane
+            // go through acc), here across phases 3 to 5: phase 4 pops each
+            // queued partial, adds its term and pushes it back, so the
+            // queue is in row order again for phase 5.
+            constexpr int HELD = V == 1 ? 1 : (V * sizeof(R) <= 8 ? 4 : 2);
+            VT part[HELD];
+#pragma unroll
+            for (int q = 0; q < HELD; ++q) part[q] = vzero<R, V>();
+            auto shift = [&]() {
+#pragma unroll
+                for (int q = HELD - 1; q > 0; --q) part[q] = part[q - 1];
+            };
+            int held = 0;
+            // phase 3: v_{m+2} on R + E1 + E2, over v_m in A
+            phase(owned_rows, m.n[2], idxL, gatherB,
+                  [&](int l, int gr) { return l < nR ? acc0(gr) : vzero<R, V>(); },
+                  [&](int l, int gr, const VT& s, const VT& o0) {
+                      const VT vm = lds(ldsA, l);
+                      const VT v2 = recur<R, V, true>(ar, ai, beta, s, vm);
+                      lds_st(ldsA, l, v2);
+                      if (l < nR) {
+                          VT o = o0;
+                          axpy<R, V>(o, cf[2], vm);
+                          axpy<R, V>(o, cf[3], lds(ldsB, l));
+                          axpy<R, V>(o, cf[4], v2);
+                          if (held < HELD) {
+                              shift();
+                              part[0] = o;
+                              ++held;
+                          } else {
+                              vst<R, V>(a.acc, (int64_t)gr * K + k0, o);
+                          }
+                      }
+                  });
+            for (int q = held; q < HELD; ++q) shift();   // first partial to part[HELD - 1]
+            __syncthreads();
+            // phase 4: v_{m+3} on R + E1 over v_{m+1} in B (each row reads
+            // its own v_{m+1} first; the phase gathers only from A); the
+            // owned rows' v_{m+3} to q1
+            int seen = 0;
+            phase(owned_rows, m.n[1], idxL, gatherA,
+                  [&](int l, int gr) {
+                      if (l < nR && ++seen > held) return vld<R, V>(a.acc, (int64_t)gr * K + k0);
+                      return vzero<R, V>();
+                  },
+                  [&](int l, int gr, const VT& s, const VT& o0) {
+                      const VT v3 = recur<R, V, true>(ar, ai, beta, s, lds(ldsB, l));
+                      lds_st(ldsB, l, v3);
+                      if (l < nR) {
+                          const int64_t vidx = (int64_t)gr * K + k0;
+                          if constexpr (!END) vst<R, V>(a.q1, vidx, v3);
+                          if (seen <= held) {
+                              VT o = part[HELD - 1];
+                              axpy<R, V>(o, cf[5], v3);
+                              shift();
+                              part[0] = o;
+                          } else {
+                              VT o = o0;
+                              axpy<R, V>(o, cf[5], v3);
+                              vst<R, V>(a.acc, vidx, o);
+                          }
+                      }
+                  });
+            for (int q = held; q < HELD; ++q) shift();
+            __syncthreads();
+            // phase 5 (last): v_{m+4} on the owned rows from B (v_{m+3})
+            // and A (v_{m+2}), the output from the queue (or acc)
+            int used = 0;
+            phase(owned_rows, nR, idxL, gatherB,
+                  [&](int, int gr) {
+                      if (used < held) {
+                          const VT o = part[HELD - 1];
+                          shift();
+                          ++used;
+                          return o;
+                      }
+                      return vld<R, V>(a.acc, (int64_t)gr * K + k0);
+                  },
+                  [&](int l, int gr, const VT& s, const VT& o0) {
+                      const int64_t vidx = (int64_t)gr * K + k0;
+                      const VT v4 = recur<R, V, true>(ar, ai, beta, s, lds(ldsA, l));
+                      if constexpr (!END) vst<R, V>(a.q2, vidx, v4);
+                      VT o = o0;
+                      axpy<R, V>(o, cf[T + 1], v4);
                       finish(vidx, o);
                   });
         } else {

@@ -174,10 +174,10 @@ struct Engine {
         return a;
     }
 
-    // T = depth + 1 (3 or 4) terms per pass: terms 1 .. T (PM_M1), then
+    // T = depth + 1 (3 to 5) terms per pass: terms 1 .. T (PM_M1), then
     // passes of T; the rest r of the N - 1 - T terms after them end the
     // step with the epilogue's sums (per tile): r = 0 as the last full pass,
-    // r = 2 or 3 as one pass of r terms, r = 1 with the one-term kernel.
+    // r = 2 .. T - 1 as one pass of r terms, r = 1 with the one-term kernel.
     // The step output receives the terms in the order m = 0, 1, 2, ... as in
     // the one-term schedule, so the states are bit-identical to it; the
     // epilogue's sums of a pass are grouped by tile instead of by chunk.
@@ -211,8 +211,8 @@ struct Engine {
             t.p1 = p1;
             t.q1 = q1;
             t.q2 = q2;
-            for (int q = 0; q < 6; ++q) {
-                const int j = m - 2 + q;   // c_{m-2} .. c_{m+3}, those in the series
+            for (int q = 0; q < 7; ++q) {
+                const int j = m - 2 + q;   // c_{m-2} .. c_{m+4}, those in the series
                 t.c[q][0] = (j >= 0 && j < N) ? sp.c[0][j] : 0.0;
                 t.c[q][1] = (j >= 0 && j < N) ? sp.c[1][j] : 0.0;
             }

@@ -69,29 +69,49 @@ def graphene_hops(L1, L2, t=GRAPHENE_T, pbc=(True, True), vacancies=None,
 GRAPHENE_ORB_DENSITY = 38.2   # orbitals per nm^2 (2 per 0.0524 nm^2 cell)
 
 
-def auto_tile_nm(vectors, precision="fp32", density=GRAPHENE_ORB_DENSITY):
-    """Edge (nm) of the square tiles of ``Sample.tile_order`` for runs of
-    ``vectors`` states per row on a sample of ``density`` orbitals per nm^2:
-    the largest tile whose rows with four terms per pass (LDS areas A and
-    B: twice the tile, plus its first ring twice, its second twice and its
-    third once) fit 150 KB of LDS, with 2 rho a^2 tile rows and a ring of
-    2.96 sqrt(rho) a rows (nearest-neighbour lattices; fitted on graphene,
-    17.6 a + 2 per ring), within [1.5, 7.5] nm.  A tile any larger is split
-    by the library into strips, which costs more than it saves.  Measured on
-    8.4M-orbital graphene (step·vectors/s): complex64 K=8 1607 at 5.0 nm,
-    1458 at 3.5 nm; K=16 1643 at 3.0 nm, 1436 at 3.54 nm (split);
-    complex128 K=1 (capped) 591 steps/s at 7.0 nm, 587 at 5.0, 570 at 9.0;
-    and a batch of 32 disordered samples with complex64 K=4 (7.35 nm)
-    13426 instance-step·vectors/s at 7.0 nm against 11720 at 5.0."""
+def pass_terms(vectors, precision="fp32"):
+    """Terms per pass the tiles of ``auto_tile_nm`` are sized for: five
+    where a row of the device layout takes at most 64 B (complex64 up to 8
+    states, complex128 1 state), else four.  Five terms hold two more rings
+    in LDS, which shrinks the tile; with rows this short the tile stays
+    large enough that the halo costs less than the pass saves.  Measured
+    on 8.4M-orbital graphene (step·vectors/s): complex64 K=8 1904 with
+    five terms at 4.8 nm against 1724 with four at 5.0 nm; K=4 1803 at
+    7.1 nm against 1620 at 7.4 nm; complex128 K=1 658 against 588 at
+    7.5 nm."""
+    return 5 if _row_bytes(vectors, precision) <= 64 else 4
+
+
+def _row_bytes(vectors, precision):
     p2, p3 = 1, 3            # vector slots of the device layout
     while p2 < vectors:
         p2 <<= 1
     while p3 < vectors:
         p3 <<= 1
-    row = min(p2, p3) * (8 if precision == "fp32" else 16)
-    rows = 150.0 * 1024 / row                     # LDS rows of one workgroup
+    return min(p2, p3) * (8 if precision == "fp32" else 16)
+
+
+def auto_tile_nm(vectors, precision="fp32", density=GRAPHENE_ORB_DENSITY, terms=None):
+    """Edge (nm) of the square tiles of ``Sample.tile_order`` for runs of
+    ``vectors`` states per row on a sample of ``density`` orbitals per nm^2
+    with ``terms`` Chebyshev terms per pass (default ``pass_terms``): the
+    largest tile whose rows fit 150 KB of LDS, within [1.5, 7.5] nm.  The
+    LDS areas A and B hold the tile twice and its rings 2 T - 3 times (four
+    terms: the first ring twice, the second twice, the third once; five
+    terms two rings more), with 2 rho a^2 tile rows and a ring of
+    2.96 sqrt(rho) a rows (nearest-neighbour lattices; fitted on graphene,
+    17.6 a + 2 per ring).  A tile any larger is split by the library into
+    strips, which costs more than it saves.  Measured with four terms on
+    8.4M-orbital graphene (step·vectors/s): complex64 K=8 1607 at 5.0 nm,
+    1458 at 3.5 nm; K=16 1643 at 3.0 nm, 1436 at 3.54 nm (split);
+    complex128 K=1 (capped) 591 steps/s at 7.0 nm, 587 at 5.0, 570 at 9.0;
+    and a batch of 32 disordered samples with complex64 K=4 (7.35 nm)
+    13426 instance-step·vectors/s at 7.0 nm against 11720 at 5.0."""
+    if terms is None:
+        terms = pass_terms(vectors, precision)
+    rows = 150.0 * 1024 / _row_bytes(vectors, precision)   # LDS rows of one workgroup
     a2 = 2.0 * density                            # tile rows per nm^2, A and B
-    a1 = 5.0 * 2.96 * math.sqrt(density)          # ring rows per nm of edge
+    a1 = (2 * terms - 3) * 2.96 * math.sqrt(density)   # ring rows per nm of edge
     edge = (-a1 + math.sqrt(a1 * a1 + 4.0 * a2 * rows)) / (2.0 * a2)
     return float(min(7.5, max(1.5, edge)))
 

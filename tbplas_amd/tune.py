@@ -4,8 +4,9 @@ For a prepared Hamiltonian, a state count and a precision it times a few
 Chebyshev steps (HIP events on the library stream, after warm-up steps) for
 every candidate (persistent blocks per CU, LDS staging of the slice rows)
 and keeps the fastest; for a Hamiltonian with spatial tiles it then times
-the best shape with three and with four terms per pass and keeps the
-fastest.  A shape is (blocks per CU, staging, terms per pass: 1, 3 or 4).
+the best shape with three, four and five terms per pass and keeps the
+fastest.  A shape is (blocks per CU, staging, terms per pass: 1, 3, 4 or
+5).
 Results are cached per (device name, library build, instances, rows, stored
 elements, widest slice, value layout, row order and tile count, states,
 precision) in memory and in a JSON file
@@ -110,7 +111,7 @@ def autotune(H, K, bes, steps=6, warmup=2, candidates=None, use_cache=True):
             timings[cand + (1,)] = timed(cand + (1,))
         best = min(timings, key=timings.get)
         if all(getattr(Hi, "tiled", False) for Hi in hams):
-            for terms in (3, 4):
+            for terms in (3, 4, 5):
                 blocked = best[:2] + (terms,)
                 timings[blocked] = timed(blocked)
             best = min(timings, key=timings.get)

@@ -53,27 +53,29 @@ def graphene2048():
     return s, (ip, ix, dt), bes, psi, exact
 
 
-@pytest.mark.parametrize("prec,tol,tile", [("fp32", 1e-6, 5.0), ("fp64", 1e-10, 3.4),
-                                           ("fp64", 1e-10, 5.0)])
-def test_benchmark_layout_2048_steps_vs_kspace(gpu, graphene2048, prec, tol, tile):
-    """The benchmark's kernel configuration over the whole 2048-step run:
-    every pass is a four-term tile pass (terms 1-4 from v_0, three full
-    passes, terms 17-20 with the correlation sums per tile), 16-wave
-    workgroups, on the tile edge the benchmark takes for the leg (the
-    library's rule to 0.1 nm: 5.0 nm for 8 complex64 states a row, 3.4 nm
-    for 8 complex128 states); and complex128 on 5.0 nm tiles, which the
-    library splits to fit the LDS, so that the output partials of owned rows
-    beyond the register queue go through memory (two held per lane, up to
-    eight owned rows per lane)."""
+@pytest.mark.parametrize("prec,tol,tile,terms", [("fp32", 1e-6, 4.8, 5), ("fp64", 1e-10, 3.4, 4),
+                                                 ("fp32", 1e-6, 5.0, 4), ("fp64", 1e-10, 5.0, 4)])
+def test_benchmark_layout_2048_steps_vs_kspace(gpu, graphene2048, prec, tol, tile, terms):
+    """The benchmark's kernel configurations over the whole 2048-step run,
+    16-wave workgroups on the tile edge the benchmark takes for the leg
+    (the library's rule to 0.1 nm): 8 complex64 states a row on 4.8 nm
+    tiles with five terms per pass (terms 1-5 from v_0, three full passes,
+    terms 16-20 with the correlation sums per tile), 8 complex128 states on
+    3.4 nm tiles with four (terms 1-4, three full passes, terms 17-20);
+    complex64 with four terms on 5.0 nm tiles, the edge sized for four; and
+    complex128 on 5.0 nm tiles, which the library splits to fit the LDS, so
+    that the output partials of owned rows beyond the register queue go
+    through memory (two held per lane, up to eight owned rows per lane)."""
     s, (ip, ix, dt), bes, psi, exact = graphene2048
     assert len(bes) == 21
-    assert round(tb.auto_tile_nm(8, "fp32"), 1) == 5.0
+    assert round(tb.auto_tile_nm(8, "fp32"), 1) == 4.8
+    assert round(tb.auto_tile_nm(8, "fp32", terms=4), 1) == 5.0
     assert round(tb.auto_tile_nm(8, "fp64"), 1) == 3.4
     order, tiles = s.tile_order(tile, return_tiles=True)
     h = tb._lib.Handle(0)
     try:
         h.set_tuning(8, True)
-        h.set_blocking(True, terms=4)
+        h.set_blocking(True, terms=terms)
         H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec, handle=h, order=order,
                              tiles=tiles)
         assert H.info()["real_values"]
@@ -92,7 +94,7 @@ def test_benchmark_layout_2048_steps_vs_kspace(gpu, graphene2048, prec, tol, til
     finally:
         h.close()
     err = float(np.max(np.abs(c[:, PTS] - exact)))
-    print(f"{prec} K=8 2048x2048, {tile} nm tiles, four terms per pass, {nb} tiles, halo "
+    print(f"{prec} K=8 2048x2048, {tile} nm tiles, {terms} terms per pass, {nb} tiles, halo "
           f"{halo:.3f} "
           f"(widest {rows} rows, {nt} threads): max |C - C_exact| over {STEPS} steps = "
           f"{err:.3e}", flush=True)

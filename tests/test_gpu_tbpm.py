@@ -198,9 +198,9 @@ def _blocked_sample(L=48):
 @pytest.mark.parametrize("prec,K", [("fp32", 8), ("fp32", 3), ("fp32", 1), ("fp64", 1),
                                     ("fp64", 4)])
 @pytest.mark.parametrize("nb_prec", [1e-15, 1e-14, 1e-6])
-@pytest.mark.parametrize("terms", [3, 4])
+@pytest.mark.parametrize("terms", [3, 4, 5])
 def test_three_term_pass_bitwise(gpu, prec, K, nb_prec, terms):
-    """Three or four terms per pass over spatial tiles give the same states,
+    """Three, four or five terms per pass over spatial tiles give the same states,
     bit for bit, as one term per pass, and the same correlations up to the
     grouping of their final FP64 sums; series lengths 21, 20 and 12 end the
     step with a pair pass, one single term and a pair pass (three terms per
@@ -232,7 +232,7 @@ def test_three_term_pass_bitwise(gpu, prec, K, nb_prec, terms):
 
 
 @pytest.mark.parametrize("prec,K", [("fp64", 4), ("fp32", 6)])
-@pytest.mark.parametrize("terms", [3, 4])
+@pytest.mark.parametrize("terms", [3, 4, 5])
 def test_three_term_pass_scattered_tiles(gpu, prec, K, terms):
     """Tiles that are not spatial (a random row order cut every 16 rows):
     the halo rings then reach far across the sample, which the tables must
@@ -266,7 +266,7 @@ def test_three_term_pass_scattered_tiles(gpu, prec, K, terms):
         h.close()
 
 
-@pytest.mark.parametrize("terms", [3, 4])
+@pytest.mark.parametrize("terms", [3, 4, 5])
 def test_three_term_pass_wide_rows(gpu, terms):
     """Rows of ~9 entries (a random sparse Hermitian matrix, wider than the
     lattice rows the pipelined phase is specialised for)."""
@@ -304,7 +304,7 @@ def test_three_term_pass_wide_rows(gpu, terms):
 
 @pytest.mark.parametrize("prec,K,field", [("fp32", 8, 0.0), ("fp32", 16, 0.0), ("fp64", 8, 0.0),
                                           ("fp64", 2, 0.0), ("fp64", 8, 30.0)])
-@pytest.mark.parametrize("terms", [3, 4])
+@pytest.mark.parametrize("terms", [3, 4, 5])
 def test_three_term_pass_workgroup_sizes(gpu, prec, K, field, terms):
     """Tiles of the benchmark's size (3.5 nm on 131,072-orbital graphene;
     3.0 nm with four terms per pass, whose three halo rings would not fit
@@ -349,7 +349,7 @@ def test_three_term_pass_workgroup_sizes(gpu, prec, K, field, terms):
         h.close()
 
 
-@pytest.mark.parametrize("terms", [3, 4])
+@pytest.mark.parametrize("terms", [3, 4, 5])
 def test_three_term_pass_graph_and_step(gpu, terms):
     """hipGraph replay of blocked steps equals launched steps; one blocked
     step (cheb_step) equals the oracle's step."""
@@ -382,7 +382,7 @@ def test_three_term_pass_graph_and_step(gpu, terms):
         h.close()
 
 
-@pytest.mark.parametrize("terms", [3, 4])
+@pytest.mark.parametrize("terms", [3, 4, 5])
 def test_block_tiles_split_to_fit(gpu, terms):
     """Tiles too large for the LDS at this row size (8 nm tiles, 16
     complex128 states: 256 bytes a row) are split until each block-tile's
@@ -415,7 +415,8 @@ def test_block_tiles_split_to_fit(gpu, terms):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
-def test_four_term_pass_long_tiles(gpu, prec):
+@pytest.mark.parametrize("terms", [4, 5])
+def test_four_term_pass_long_tiles(gpu, prec, terms):
     """12 nm tiles with one state (8 or 16 bytes a row): a lane owns more
     rows than the four output partials it holds in registers between the
     third and fourth phase of a pass (complex64: ~5,500 owned rows for
@@ -428,7 +429,7 @@ def test_four_term_pass_long_tiles(gpu, prec):
     psi = _states(s.num_orb, 1, seed=43)
     h = tb._lib.Handle(0)
     try:
-        h.set_blocking(True, terms=4)
+        h.set_blocking(True, terms=terms)
         H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec, handle=h, order=order,
                              tiles=tiles)
         rows, nt = H.blocking_launch(1)
