@@ -101,7 +101,7 @@ def parse():
                    help="dos and batch: tile edge in nm (default: the library's "
                         "auto_tile_nm for each leg's states and precision, as the Solver's "
                         "DOS driver picks it, to 0.1 nm: 4.8 nm for 8 complex64 states a "
-                        "row, 3.4 nm for 8 complex128 states, 7.5 nm for one, 7.1 nm for "
+                        "row, 3.2 nm for 8 complex128 states, 7.5 nm for one, 7.1 nm for "
                         "the batch's 4 complex64 states)")
     p.add_argument("--cond-tile", type=float, default=None,
                    help="dc and ac: tile edge in nm (0: the sample's own order; default "

@@ -293,14 +293,27 @@ __global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
             // queue (part[HELD - 1] leaves first; rows beyond HELD per lane
             // go through acc), here across phases 3 to 5: phase 4 pops each
             // queued partial, adds its term and pushes it back, so the
-            // queue is in row order again for phase 5.
+            // queue is in row order again for phase 5.  Every queue update
+            // is a select over all HELD entries: with pops and pushes in
+            // one phase, conditional moves would leave the queue in scratch.
             constexpr int HELD = V == 1 ? 1 : (V * sizeof(R) <= 8 ? 4 : 2);
             VT part[HELD];
 #pragma unroll
             for (int q = 0; q < HELD; ++q) part[q] = vzero<R, V>();
-            auto shift = [&]() {
+            // when c: part[q] = part[q - 1] for q > 0
+            auto shift_if = [&](bool c) {
 #pragma unroll
-                for (int q = HELD - 1; q > 0; --q) part[q] = part[q - 1];
+                for (int q = HELD - 1; q > 0; --q) part[q] = vsel<R, V>(c, part[q - 1], part[q]);
+            };
+            // when c: shift, then part[0] = x
+            auto shift_in = [&](bool c, VT x) {
+                shift_if(c);
+                part[0] = vsel<R, V>(c, x, part[0]);
+            };
+            // the first of held partials to part[HELD - 1]
+            auto align = [&](int held) {
+#pragma unroll
+                for (int i = 0; i < HELD - 1; ++i) shift_if(i < HELD - held);
             };
             int held = 0;
             // phase 3: v_{m+2} on R + E1 + E2, over v_m in A
@@ -315,16 +328,13 @@ __global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
                           axpy<R, V>(o, cf[2], vm);
                           axpy<R, V>(o, cf[3], lds(ldsB, l));
                           axpy<R, V>(o, cf[4], v2);
-                          if (held < HELD) {
-                              shift();
-                              part[0] = o;
-                              ++held;
-                          } else {
-                              vst<R, V>(a.acc, (int64_t)gr * K + k0, o);
-                          }
+                          const bool push = held < HELD;
+                          if (!push) vst<R, V>(a.acc, (int64_t)gr * K + k0, o);
+                          shift_in(push, o);
+                          held += push;
                       }
                   });
-            for (int q = held; q < HELD; ++q) shift();   // first partial to part[HELD - 1]
+            align(held);
             __syncthreads();
             // phase 4: v_{m+3} on R + E1 over v_{m+1} in B (each row reads
             // its own v_{m+1} first; the phase gathers only from A); the
@@ -341,31 +351,25 @@ __global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
                       if (l < nR) {
                           const int64_t vidx = (int64_t)gr * K + k0;
                           if constexpr (!END) vst<R, V>(a.q1, vidx, v3);
-                          if (seen <= held) {
-                              VT o = part[HELD - 1];
-                              axpy<R, V>(o, cf[5], v3);
-                              shift();
-                              part[0] = o;
-                          } else {
-                              VT o = o0;
-                              axpy<R, V>(o, cf[5], v3);
-                              vst<R, V>(a.acc, vidx, o);
-                          }
+                          const bool mine = seen <= held;
+                          VT o = vsel<R, V>(mine, part[HELD - 1], o0);
+                          axpy<R, V>(o, cf[5], v3);
+                          if (!mine) vst<R, V>(a.acc, vidx, o);
+                          shift_in(mine, o);
                       }
                   });
-            for (int q = held; q < HELD; ++q) shift();
+            align(held);
             __syncthreads();
             // phase 5 (last): v_{m+4} on the owned rows from B (v_{m+3})
             // and A (v_{m+2}), the output from the queue (or acc)
             int used = 0;
             phase(owned_rows, nR, idxL, gatherB,
                   [&](int, int gr) {
-                      if (used < held) {
-                          const VT o = part[HELD - 1];
-                          shift();
-                          ++used;
-                          return o;
-                      }
+                      const bool mine = used < held;
+                      const VT o = part[HELD - 1];
+                      shift_if(mine);
+                      used += mine;
+                      if (mine) return o;
                       return vld<R, V>(a.acc, (int64_t)gr * K + k0);
                   },
                   [&](int l, int gr, const VT& s, const VT& o0) {

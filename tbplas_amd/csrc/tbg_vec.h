@@ -191,6 +191,19 @@ __device__ __forceinline__ Vec<R, V> recur(const R* ar, const R* ai, R beta, con
     return cur;
 }
 
+// c ? x : y per register (no branch; x and y by value, so the select stays
+// one of values and a queue of these stays in registers).
+template <typename R, int V>
+__device__ __forceinline__ Vec<R, V> vsel(bool c, Vec<R, V> x, Vec<R, V> y) {
+    Vec<R, V> o;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        o.re[v] = c ? x.re[v] : y.re[v];
+        o.im[v] = c ? x.im[v] : y.im[v];
+    }
+    return o;
+}
+
 // o += c x per state.
 template <typename R, int V>
 __device__ __forceinline__ void axpy(Vec<R, V>& o, const R* c, const Vec<R, V>& x) {

@@ -71,15 +71,16 @@ GRAPHENE_ORB_DENSITY = 38.2   # orbitals per nm^2 (2 per 0.0524 nm^2 cell)
 
 def pass_terms(vectors, precision="fp32"):
     """Terms per pass the tiles of ``auto_tile_nm`` are sized for: five
-    where a row of the device layout takes at most 64 B (complex64 up to 8
-    states, complex128 1 state), else four.  Five terms hold two more rings
-    in LDS, which shrinks the tile; with rows this short the tile stays
-    large enough that the halo costs less than the pass saves.  Measured
-    on 8.4M-orbital graphene (step·vectors/s): complex64 K=8 1904 with
-    five terms at 4.8 nm against 1724 with four at 5.0 nm; K=4 1803 at
-    7.1 nm against 1620 at 7.4 nm; complex128 K=1 658 against 588 at
-    7.5 nm."""
-    return 5 if _row_bytes(vectors, precision) <= 64 else 4
+    where a row of the device layout takes at most 128 B (complex64 up to
+    16 states, complex128 up to 8), else four.  Five terms hold two more
+    rings in LDS, which shrinks the tile; with rows this short the tile
+    stays large enough that the halo costs less than the pass saves.
+    Measured on 8.4M-orbital graphene (step·vectors/s, five terms against
+    four): complex64 K=8 1904 at 4.8 nm against 1724 at 5.0 nm; K=4 1803
+    at 7.1 nm against 1620 at 7.4 nm; K=16 1822 at 3.1 nm against 1777 at
+    3.4 nm; complex128 K=1 658 against 588 at 7.5 nm; K=4 915 at 4.8 nm
+    against 805 at 5.0 nm; K=8 896 at 3.1 nm against 848 at 3.4 nm."""
+    return 5 if _row_bytes(vectors, precision) <= 128 else 4
 
 
 def _row_bytes(vectors, precision):

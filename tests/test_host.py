@@ -60,16 +60,17 @@ class _S:
 def test_auto_tile_size():
     """Tiles sized for the bytes of states per row (vector slots: the
     smaller of the next power of two and the next 3 * 2^j) and the terms
-    per pass T (five up to 64 bytes a row, else four): the edge a whose LDS
-    rows 2 rho a^2 + (2 T - 3) * 2.96 sqrt(rho) a fill 150 KB, within
+    per pass T (five up to 128 bytes a row, else four): the edge a whose
+    LDS rows 2 rho a^2 + (2 T - 3) * 2.96 sqrt(rho) a fill 150 KB, within
     [1.5, 7.5] nm; on graphene with four terms 5.04 nm at 64 bytes a row,
-    3.41 at 128, 7.35 at 32; with five 4.83 at 64 and 7.13 at 32."""
+    3.41 at 128, 7.35 at 32; with five 4.83 at 64, 3.21 at 128 and 7.13 at
+    32."""
     rho = 38.2
 
     def rows(a, terms):
         return 2 * rho * a * a + (2 * terms - 3) * 2.96 * math.sqrt(rho) * a
 
-    for vec, prec, row, terms in ((16, "fp32", 128, 4), (10, "fp32", 96, 4),
+    for vec, prec, row, terms in ((16, "fp32", 128, 5), (10, "fp32", 96, 5),
                                   (24, "fp32", 192, 4), (16, "fp64", 256, 4),
                                   (8, "fp32", 64, 5), (4, "fp32", 32, 5), (2, "fp64", 32, 5)):
         assert tb.sample.pass_terms(vec, prec) == terms
@@ -80,12 +81,14 @@ def test_auto_tile_size():
         for t in (3, 4, 5):
             a = tb.auto_tile_nm(vec, prec, terms=t)
             assert a == 7.5 or math.isclose(rows(a, t) * row, 150 * 1024, rel_tol=1e-9)
-    assert abs(tb.auto_tile_nm(16, "fp32") - 3.41) < 0.01
+    assert abs(tb.auto_tile_nm(16, "fp32") - 3.21) < 0.01
+    assert abs(tb.auto_tile_nm(16, "fp32", terms=4) - 3.41) < 0.01
     assert abs(tb.auto_tile_nm(8, "fp32") - 4.83) < 0.01
     assert abs(tb.auto_tile_nm(8, "fp32", terms=4) - 5.04) < 0.01
     assert abs(tb.auto_tile_nm(4, "fp32") - 7.13) < 0.01
     assert abs(tb.auto_tile_nm(4, "fp32", terms=4) - 7.35) < 0.01
-    assert tb.auto_tile_nm(8, "fp64") == tb.auto_tile_nm(8, "fp64", terms=4)
+    assert tb.auto_tile_nm(8, "fp64") == tb.auto_tile_nm(16, "fp32")
+    assert tb.auto_tile_nm(24, "fp32") == tb.auto_tile_nm(24, "fp32", terms=4)
     assert tb.auto_tile_nm(1, "fp64") == 7.5            # 16 B: capped
     assert tb.auto_tile_nm(64, "fp64") == 1.5           # 1024 B, floor
     # a denser lattice takes smaller tiles
