@@ -105,10 +105,11 @@ def parse():
                         "the batch's 4 complex64 states)")
     p.add_argument("--cond-tile", type=float, default=None,
                    help="dc and ac: tile edge in nm (0: the sample's own order; default "
-                        "2.0 for dc, 2.5 for ac)")
-    p.add_argument("--dc-states", type=int, default=16,
+                        "for dc the library's auto_tile_nm for the run's vectors, as the "
+                        "Solver's gpu['dc_tile_nm'] 'auto': 3.2 nm for 8 states; 2.5 for ac)")
+    p.add_argument("--dc-states", type=int, default=8,
                    help="dc: states per run (two propagated vectors each), as the "
-                        "Solver's gpu['max_vectors']")
+                        "Solver's gpu['dc_states']")
     p.add_argument("--ac-states", type=int, default=4,
                    help="ac: states per run (three propagated vectors each), as the "
                         "Solver's gpu['ac_states']")
@@ -389,7 +390,7 @@ def kernel_label(kind, prec, K, real, threads):
         R = "float" if prec == "fp32" else "double"
         # the launch narrows 16-wave workgroups where the instance needs more
         # registers than a lane has at 16 waves (complex128 states with the
-        # dot epilogue after three or four terms, three states per lane)
+        # dot epilogue after three or more terms, three states per lane)
         wide = V == 1 or (V == 2 and (prec == "fp32" or
                                       (real and not (mode == "4" and int(terms) >= 3))))
         nt = threads if (wide or threads != 1024) else 512
@@ -721,8 +722,9 @@ def dc_workload(args):
     """Graphene 1024x1024, 1% vacancies, Anderson disorder U[-0.5, 0.5] eV,
     Peierls phases for B = 20 T (gauge 0), rescale 9; DC correlation at the 5
     grid energies nearest -1, -0.5, 0, 0.5, 1 eV; 16 states, --steps steps
-    (at most 1024), complex64 (FP64 quasi-eigenstates and correlations),
-    2.0 nm tiles."""
+    (at most 1024) in runs of --dc-states, complex64 (FP64 quasi-eigenstates
+    and correlations), tiles sized by the library's rule for a run's
+    vectors (3.2 nm for 8 states) unless --cond-tile."""
     import numpy as np
     import torch
     import tbplas_amd as tb
@@ -730,7 +732,8 @@ def dc_workload(args):
     L, K = 1024, 16
     per = max(1, min(args.dc_states, K))
     T = max(2, min(args.steps, 1024))
-    tile = 2.0 if args.cond_tile is None else args.cond_tile
+    tile = (round(tb.auto_tile_nm(2 * per, "fp32"), 1) if args.cond_tile is None
+            else args.cond_tile)
     s = tb.make_graphene_sample(L, vacancies=tb.random_vacancies(2 * L * L, 0.01, 2024))
     tb.add_anderson_disorder(s, 1.0, 2025)
     s.set_magnetic_field(20.0, 0)

@@ -18,7 +18,7 @@ look at:
                        drifts ~1e-6 per hundred steps         default 1e-4
     gpu['tile_nm']     DOS, LDOS and quasi-eigenstate runs: store the rows
                        in square tiles of this size (nm, from the orbital
-                       positions) so the tuner may run three or four
+                       positions) so the tuner may run three to five
                        Chebyshev terms per pass; same results, 0 keeps the
                        sample's order, 'auto' sizes the tiles for the
                        states per row (sample.auto_tile_nm)   default 'auto'
@@ -30,7 +30,13 @@ look at:
                        2.5 nm tiles 638 state-steps/s, 8 on 1.5 nm 519)
                                                               default 4
     gpu['ac_tile_nm']  the tile edge of AC runs               default 2.5
-    gpu['dc_tile_nm']  the same for DC runs                   default 2.0
+    gpu['dc_states']   states per DC run (two propagated vectors each)
+                       (2M-orbital disordered graphene, complex64,
+                       step-vectors/s: 8 states on 3.2 nm tiles 4696,
+                       16 on 2.0 nm 4312, 4 on 4.8 nm 4632)   default 8
+    gpu['dc_tile_nm']  the tile edge of DC runs ('auto': auto_tile_nm for
+                       the run's vectors, 3.2 nm for 8 complex64 states
+                       on graphene)                           default 'auto'
 """
 import pickle
 
@@ -79,7 +85,8 @@ class Config:
                     'cond_tile_nm': 1.5,
                     'ac_states': 4,
                     'ac_tile_nm': 2.5,
-                    'dc_tile_nm': 2.0}
+                    'dc_states': 8,
+                    'dc_tile_nm': 'auto'}
         self._legal_params = dict()
         self.set_legal_params()
 

@@ -2,7 +2,7 @@
 // tbg_block.hip).  Its instances are many (real type x states per lane x
 // value layout x pass mode x workgroup size x terms per pass), so each
 // (real type, terms per pass) pair is instantiated in a translation unit of
-// its own (and value layout, real or complex, for three and four terms),
+// its own (and value layout, real or complex, for three to five terms),
 // tbg_pass_*.hip, and the build compiles them in parallel.
 #pragma once
 
@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(NT) k_block_pass(PassArgs a) {
 template <typename R, int V, bool RV, int MODE, int T>
 static void launch_f(const PassArgs& a, const Blocking& b, hipStream_t s) {
     // the instances term3_launch_threads can ask for; complex128 states with
-    // the dot epilogue after three or four terms need more than the 128
+    // the dot epilogue after three or more terms need more than the 128
     // registers a lane has at 16 waves, so they take 8
     constexpr bool wide =
         V == 1 || (V == 2 && (sizeof(R) == 4 || (RV && !(MODE == PM_DOTNORM && T >= 3))));

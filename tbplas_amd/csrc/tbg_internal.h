@@ -118,7 +118,7 @@ struct tbg_handle {
     int blocks_per_cu = 16;
     int stage = 1;
     int real_layout = 1;      // store real values when H has no imaginary part
-    int blocking = 3;         // terms per pass where H has spatial tiles (3 or 4; 0: one)
+    int blocking = 3;         // terms per pass where H has spatial tiles (3 to 5; 0: one)
     int block_rows = 0;       // cap on rows owned per workgroup (0: from LDS)
     int max_lds = 65536;      // LDS bytes one workgroup may use
     tbg::DevBuf scratch;      // reduction partials and small tables

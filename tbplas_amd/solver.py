@@ -421,13 +421,13 @@ class Solver:
         indptr, indices, hop, dx, dy = self._sample.build_ham_dxy_device()
         num_sample = self._dist_sample()
         self._echo_time_step_fs(time_step)
-        order, tiles = self._row_order(gpu, "dc_tile_nm")
+        per = max(1, min(32, int(gpu["dc_states"]), int(gpu["max_vectors"])))
+        order, tiles = self._row_order(gpu, "dc_tile_nm", 2 * min(per, num_sample))
         H = Hamiltonian(self._sample.num_orb, indptr, indices, hop, dx, dy,
                         cur_scale=self._sample.rescale, precision=gpu["precision"],
                         handle=h, order=order, tiles=tiles)
         psi0 = self._states(num_sample)
         chk, thr = self._chk()
-        per = max(1, min(32, int(gpu["max_vectors"])))
         tune.apply(H, 2 * min(per, num_sample), bes, gpu["autotune"])
         corr_dos_ = np.zeros(tnr, dtype=np.complex128)
         corr_dc_ = np.zeros((2, len(energies_dos), tnr), dtype=np.complex128)

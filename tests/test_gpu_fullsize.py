@@ -1,20 +1,22 @@
 """The benchmark configurations at their own sizes and layouts, against the
 oracle.
 
-* 2048x2048 graphene (8,388,608 orbitals), 8 states, 2048 steps, four
+* 2048x2048 graphene (8,388,608 orbitals), 8 states, 2048 steps, five
   Chebyshev terms per pass on the tiles the benchmark times (complex64
-  5.0 nm, complex128 3.4 nm; complex128 also on 5.0 nm, split to fit):
-  C(t) at sampled steps over the whole run against exact k-space
+  4.8 nm, complex128 3.2 nm) and four on the edges sized for four
+  (complex64 5.0 nm, complex128 3.4 nm; complex128 also on 5.0 nm, split
+  to fit): C(t) at sampled steps over the whole run against exact k-space
   propagation.  Tolerances: complex128 1e-10;
   complex64 (FP32 storage and arithmetic, FP64 sums) 1e-6, about ten times
   the largest error measured over 2048 steps.
 * AC on the same sample (8 states in runs of 4 on 2.5 nm tiles with four
-  terms per pass, and in one run on 1.5 nm tiles with three) against exact
+  terms per pass, complex128 also with five, and in one run on 1.5 nm
+  tiles with three) against exact
   k-space propagation with the product's Fermi polynomial over 256 steps:
   complex128 1e-11, complex64 1e-6 (correlations ~4e-2).
 * The disordered, field-threaded DC sample (1% vacancies, Anderson disorder
   U[-0.5, 0.5] eV, B = 20 T in the Landau gauge, 16 states, 5 energies,
-  2.0 nm tiles with three and four terms per pass) on 24x24 cells against
+  2.0 nm tiles with three, four and five terms per pass) on 24x24 cells against
   the C restatement of tbpm_dccond, and at 1024x1024 (2,076,180 orbitals)
   through properties that hold at any size: the DC run's DOS correlation
   equals a DOS run's C(t) on the same states, and the quasi-eigenstates are
@@ -60,14 +62,14 @@ def test_benchmark_layout_2048_steps_vs_kspace(gpu, graphene2048, prec, tol, til
     """The benchmark's kernel configurations over the whole 2048-step run,
     16-wave workgroups on the tile edge the benchmark takes for the leg
     (the library's rule to 0.1 nm) with five terms per pass (terms 1-5
-    from v_0, three full passes, terms 16-20 with the correlation sums per
+    from v_0, two full passes, terms 16-20 with the correlation sums per
     tile): 8 complex64 states a row on 4.8 nm tiles, 8 complex128 states on
     3.2 nm tiles; with four terms (terms 1-4, three full passes, terms
     17-20) on the edges sized for four: complex64 on 5.0 nm, complex128 on
     3.4 nm; and complex128 on 5.0 nm tiles, which the library splits to fit
-    the LDS, so
-    that the output partials of owned rows beyond the register queue go
-    through memory (two held per lane, up to eight owned rows per lane)."""
+    the LDS, so that the output partials of owned rows beyond the register
+    queue go through memory (two held per lane, up to eight owned rows per
+    lane)."""
     s, (ip, ix, dt), bes, psi, exact = graphene2048
     assert len(bes) == 21
     assert round(tb.auto_tile_nm(8, "fp32"), 1) == 4.8
@@ -128,12 +130,13 @@ def ac2048(graphene2048):
 
 
 @pytest.mark.parametrize("prec,tol,tile,terms,per", [
-    ("fp32", 1e-6, 2.5, 4, 4), ("fp64", 1e-11, 2.5, 4, 4),
+    ("fp32", 1e-6, 2.5, 4, 4), ("fp64", 1e-11, 2.5, 4, 4), ("fp64", 1e-11, 2.5, 5, 4),
     ("fp32", 1e-6, 1.5, 3, 8), ("fp64", 1e-11, 1.5, 3, 8)])
 def test_ac_benchmark_layout_vs_kspace(gpu, ac2048, prec, tol, tile, terms, per):
     """AC on 8.4M-orbital graphene as the benchmark and the Solver run it
     (runs of 4 states, 12 vectors: J_x psi1, J_y psi1 and psi2 of each, on
-    2.5 nm tiles with four terms per pass) and as one run of 8 states (24
+    2.5 nm tiles with four terms per pass, complex128 also with five, which
+    the tuner picks for it) and as one run of 8 states (24
     vectors, 1.5 nm tiles, three terms per pass), against exact k-space
     propagation with the same Fermi polynomial, at sampled steps of 256
     (the correlations are ~4e-2)."""
@@ -173,7 +176,7 @@ def _en_grid(s, T):
     return en, inds
 
 
-@pytest.mark.parametrize("terms", [3, 4])
+@pytest.mark.parametrize("terms", [3, 4, 5])
 def test_cfg3_dc_vs_oracle(gpu, terms):
     """1% vacancies, Anderson W = 1 eV, B = 20 T, 16 states (32 propagated
     vectors), 5 energies in [-1, 1] eV, 2.0 nm tiles: complex128 to 1e-10
