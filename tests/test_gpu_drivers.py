@@ -32,9 +32,9 @@ def _ham(s, prec, layout, handle=None):
     kw = {}
     if layout.startswith("tiles"):
         kw["order"], kw["tiles"] = s.tile_order(0.8, return_tiles=True)
-        if layout == "tiles4" and handle is None:    # four terms per pass
+        if layout in ("tiles4", "tiles5") and handle is None:   # four or five terms per pass
             handle = tb._lib.Handle(0)
-            handle.set_blocking(True, terms=4)
+            handle.set_blocking(True, terms=int(layout[-1]))
     H = tbpm.Hamiltonian(s.num_orb, ip, ix, dt, precision=prec, handle=handle, **kw)
     if layout.startswith("tiles"):
         assert H.blocking_info(4)[0] > 1
@@ -46,7 +46,7 @@ def _tol(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("layout", ["caller", "tiles", "tiles4"])
+@pytest.mark.parametrize("layout", ["caller", "tiles", "tiles4", "tiles5"])
 def test_ldos_vs_oracle(gpu, prec, layout):
     s = _sample(field=20.0 if layout == "caller" else 0.0)
     H, (ip, ix, dt) = _ham(s, prec, layout)
@@ -81,7 +81,7 @@ def test_ldos_vs_reference_live(gpu):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("layout", ["caller", "tiles", "tiles4"])
+@pytest.mark.parametrize("layout", ["caller", "tiles", "tiles4", "tiles5"])
 @pytest.mark.parametrize("tl", [[0, 1, 2, 25, 40], [40], [0], [3, 17, 39]])
 def test_psi_t_vs_oracle(gpu, prec, layout, tl):
     s = _sample()
@@ -111,7 +111,7 @@ def test_psi_t_rejects_bad_time_log(gpu):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("layout", ["caller", "tiles", "tiles4"])
+@pytest.mark.parametrize("layout", ["caller", "tiles", "tiles4", "tiles5"])
 def test_quasi_eigenstates_vs_oracle(gpu, prec, layout):
     s = _sample(L=10)
     H, (ip, ix, dt) = _ham(s, prec, layout)
@@ -192,7 +192,7 @@ def test_solver_ldos_qe_psi_t(gpu, tmp_path):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("layout", ["caller", "tiles", "tiles4"])
+@pytest.mark.parametrize("layout", ["caller", "tiles", "tiles4", "tiles5"])
 def test_dyn_pol_vs_oracle(gpu, prec, layout):
     s = _sample(L=10, field=15.0 if layout == "caller" else 0.0)
     H, (ip, ix, dt) = _ham(s, prec, layout)

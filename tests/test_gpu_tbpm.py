@@ -676,17 +676,18 @@ def _row_tiles(n, rows):
     return np.r_[np.arange(0, n, rows), n].astype(np.int64)
 
 
-@pytest.mark.parametrize("layout", ["plain", "ordered", "tiles", "tiles4"])
+@pytest.mark.parametrize("layout", ["plain", "ordered", "tiles", "tiles4", "tiles5"])
 def test_ac_reference_fixture(gpu, layout):
     """ordered: the device layout stores the rows in a random order (every
-    gather leaves its slice); tiles: three terms per pass (tiles4: four),
-    workgroups of 32 rows; results stay in the caller's order."""
+    gather leaves its slice); tiles: three terms per pass (tiles4: four,
+    tiles5: five), workgroups of 32 rows; results stay in the caller's
+    order."""
     g = golden("ref_ac.npz")
     n = len(g["indptr"]) - 1
     tiled = layout.startswith("tiles")
     hd = tb._lib.Handle(0) if tiled else None
     if hd is not None:
-        hd.set_blocking(True, 32, terms=4 if layout == "tiles4" else 3)
+        hd.set_blocking(True, 32, terms={"tiles4": 4, "tiles5": 5}.get(layout, 3))
     H = tbpm.Hamiltonian(n, g["indptr"], g["indices"], g["hop"], g["dx"], g["dy"],
                          cur_scale=float(g["rescale"]), precision="fp64", handle=hd,
                          order=_random_order(n, 7) if layout == "ordered" else None,
@@ -740,14 +741,14 @@ def test_fermi_coef_product_vs_reference_fixture(gpu):
     assert np.max(np.abs(c - g["coef_F_cold"])) < 1e-14
 
 
-@pytest.mark.parametrize("layout", ["plain", "ordered", "tiles", "tiles4"])
+@pytest.mark.parametrize("layout", ["plain", "ordered", "tiles", "tiles4", "tiles5"])
 def test_dc_reference_fixture(gpu, layout):
     g = golden("ref_dc.npz")
     n = len(g["indptr"]) - 1
     tiled = layout.startswith("tiles")
     hd = tb._lib.Handle(0) if tiled else None
     if hd is not None:
-        hd.set_blocking(True, 32, terms=4 if layout == "tiles4" else 3)
+        hd.set_blocking(True, 32, terms={"tiles4": 4, "tiles5": 5}.get(layout, 3))
     H = tbpm.Hamiltonian(n, g["indptr"], g["indices"], g["hop"], g["dx"], g["dy"],
                          cur_scale=float(g["rescale"]), precision="fp64", handle=hd,
                          order=_random_order(n, 8) if layout == "ordered" else None,
