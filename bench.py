@@ -881,8 +881,10 @@ def latency_workload(args):
     """Pristine graphene 256x256 (131,072 orbitals, rescale 9), one state,
     1024 DOS steps (21 Bessel terms at 1e-15): the case whose vectors fit in
     the last-level cache, reported as microseconds per step, complex128 and
-    complex64, hipGraph replay, 2 nm tiles (the tuner picks one, three,
-    four or five terms per pass)."""
+    complex64, hipGraph replay, tiles by the library's rule for a small
+    sample (two per compute unit: 2.6 nm on MI355X; the tuner picks one,
+    three, four or five terms per pass)."""
+    import torch
     import tbplas_amd as tb
     from tbplas_amd import tbpm, _lib, tune
     L, T = 256, 1024
@@ -894,7 +896,9 @@ def latency_workload(args):
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, 1))
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
-    order, tiles = (s.tile_order(2.0, return_tiles=True) if args.order == "tiles"
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    tile = round(tb.auto_tile_nm(1, "fp64", n_orb=n, cus=cus), 1)
+    order, tiles = (s.tile_order(tile, return_tiles=True) if args.order == "tiles"
                     else (None, None))
     a = argparse.Namespace(**vars(args))
     a.warmup, a.steps = 32, T
@@ -913,14 +917,14 @@ def latency_workload(args):
         rl = (roofline(kt, pass_bytes_fn(sched, n, nnz, 1, sz, real),
                        lambda k, prec=prec, real=real, threads=threads:
                        kernel_label(k, prec, 1, real, threads),
-                       traffic_key(f"latency_{prec}", 2.0 if order is not None else 0))
+                       traffic_key(f"latency_{prec}", tile if order is not None else 0))
               if kt else None)
         if rl is not None:
             rl["note"] = ("the vectors fit in the 256 MB last-level cache: launch and "
                           "latency bound, HBM bandwidth does not bound it")
         res[prec] = {"us_per_step": round(ms / T * 1e3, 2),
                      "us_per_term": round(ms / T / (len(bes) - 1) * 1e3, 3),
-                     "layout": ("caller order" if order is None else "tiles of 2.0 nm")
+                     "layout": ("caller order" if order is None else f"tiles of {tile:g} nm")
                      + passes_name(shape),
                      "launch_shape": shape_json(shape), "roofline": rl}
     return {"metric": "TBPM microseconds per step, 131,072-orbital graphene, 1 state",

@@ -322,7 +322,9 @@ std::shared_ptr<Blocking> get_blocking(tbg_handle* h, const tbg_ham* H, int row_
 // the 128 registers a lane has at 16 waves, so that pair stays at 8.
 int term3_launch_threads(int elem_bytes, bool real_vals, int K, int nA) {
     const int V = term3_vectors_per_lane(K);
-    if (V == 1) return 1024;
+    // one state per lane on small tiles: 4-wave workgroups, several per CU,
+    // so that a launch over a small sample runs its tiles at once
+    if (V == 1) return nA <= 768 ? 256 : 1024;
     if (V != 2 || (elem_bytes == 8 && !real_vals)) return 512;
     return (int64_t)nA * (K / V) >= 2048 ? 1024 : 512;
 }

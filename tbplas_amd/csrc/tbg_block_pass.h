@@ -74,8 +74,9 @@ __device__ __forceinline__ Vec<R, V> tab_sum(const RowTab<R, RV, NJ>& t, GATHER 
 
 // NT threads per workgroup: the tile's LDS limits a CU to about two
 // workgroups, so they are large (16 waves, or 8 where term3_launch_threads
-// says so) to keep enough gathers in flight.  G = K/V lanes share a row,
-// NT/G rows per pass of a phase.
+// says so) to keep enough gathers in flight; tiles of up to 768 rows with
+// one state per lane take 4 waves, several workgroups per CU.  G = K/V
+// lanes share a row, NT/G rows per pass of a phase.
 //
 // T terms m .. m+T-1 per pass (2 <= T <= the tables' depth + 1), phase by
 // phase, n[k] = rows of R and the first k halo rings:
@@ -494,6 +495,14 @@ static void launch_f(const PassArgs& a, const Blocking& b, hipStream_t s) {
     size_t lds = (size_t)(b.nA_max + b.nB_max + 1) * a.K * sizeof(typename CplxT<R>::T);
     if constexpr (MODE == PM_NORM || MODE == PM_DOTNORM)   // the waves' partial sums
         lds = std::max(lds, (size_t)(nt / 64) * 3 * a.K * sizeof(double));
+    if constexpr (V == 1) {
+        if (nt == 256) {
+            hipLaunchKernelGGL((k_block_pass<R, V, RV, MODE, 256, T>), dim3(b.nblocks),
+                               dim3(256), lds, s, a);
+            TBG_HIP(hipGetLastError());
+            return;
+        }
+    }
     if constexpr (wide) {
         if (nt == 1024) {
             hipLaunchKernelGGL((k_block_pass<R, V, RV, MODE, 1024, T>), dim3(b.nblocks),
@@ -547,6 +556,7 @@ static void allow_lds_v(int bytes) {
 
 template <typename R, int T, bool RV>
 void allow_lds_rt(int bytes) {
+    allow_lds_v<R, 1, 256, T, RV>(bytes);
     allow_lds_v<R, 1, 1024, T, RV>(bytes);
     allow_lds_v<R, 2, 1024, T, RV>(bytes);
     allow_lds_v<R, 2, 512, T, RV>(bytes);

@@ -92,7 +92,8 @@ def _row_bytes(vectors, precision):
     return min(p2, p3) * (8 if precision == "fp32" else 16)
 
 
-def auto_tile_nm(vectors, precision="fp32", density=GRAPHENE_ORB_DENSITY, terms=None):
+def auto_tile_nm(vectors, precision="fp32", density=GRAPHENE_ORB_DENSITY, terms=None,
+                 n_orb=None, cus=None):
     """Edge (nm) of the square tiles of ``Sample.tile_order`` for runs of
     ``vectors`` states per row on a sample of ``density`` orbitals per nm^2
     with ``terms`` Chebyshev terms per pass (default ``pass_terms``): the
@@ -107,13 +108,22 @@ def auto_tile_nm(vectors, precision="fp32", density=GRAPHENE_ORB_DENSITY, terms=
     1458 at 3.5 nm; K=16 1643 at 3.0 nm, 1436 at 3.54 nm (split);
     complex128 K=1 (capped) 591 steps/s at 7.0 nm, 587 at 5.0, 570 at 9.0;
     and a batch of 32 disordered samples with complex64 K=4 (7.35 nm)
-    13426 instance-step·vectors/s at 7.0 nm against 11720 at 5.0."""
+    13426 instance-step·vectors/s at 7.0 nm against 11720 at 5.0.
+
+    Given the sample's ``n_orb`` orbitals and the GPU's ``cus`` compute
+    units, a small sample is cut into at least two tiles per CU (never
+    below 1.5 nm), so that a pass has enough workgroups: on 131,072-orbital
+    graphene with one state (2.59 nm) complex128 runs 41.6 us per step on
+    2.5 nm tiles, 49.8 on 2.0 and 45.8 on 3.0, against 68 with one term per
+    pass, which the tuner takes on 7.5 nm tiles (61 of them)."""
     if terms is None:
         terms = pass_terms(vectors, precision)
     rows = 150.0 * 1024 / _row_bytes(vectors, precision)   # LDS rows of one workgroup
     a2 = 2.0 * density                            # tile rows per nm^2, A and B
     a1 = (2 * terms - 3) * 2.96 * math.sqrt(density)   # ring rows per nm of edge
     edge = (-a1 + math.sqrt(a1 * a1 + 4.0 * a2 * rows)) / (2.0 * a2)
+    if n_orb is not None and cus:
+        edge = min(edge, math.sqrt(n_orb / density / (2.0 * cus)))
     return float(min(7.5, max(1.5, edge)))
 
 

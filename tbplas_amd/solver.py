@@ -138,8 +138,9 @@ class Solver:
 
     def _row_order(self, gpu, key, vectors=1):
         """(row order, tile offsets) of square tiles of gpu[key] nm ('auto':
-        sized for `vectors` states per row and the sample's orbitals per nm^2
-        in the x-y plane, ``auto_tile_nm``) when the sample has orbital
+        sized for `vectors` states per row, the sample's orbitals per nm^2
+        in the x-y plane and, for a small sample, at least two tiles per
+        compute unit, ``auto_tile_nm``) when the sample has orbital
         positions, else (None, None)."""
         pos = self._sample.orb_pos
         if pos is None:
@@ -152,7 +153,10 @@ class Solver:
             cell = np.floor((xy - xy.min(axis=0)) / 2.0).astype(np.int64)
             key = cell[:, 1] * (int(cell[:, 0].max()) + 1) + cell[:, 0]
             density = float(np.median(np.unique(key, return_counts=True)[1])) / 4.0
-            tile = auto_tile_nm(vectors, gpu["precision"], max(density, 1.0))
+            import torch
+            cus = torch.cuda.get_device_properties(gpu["device"]).multi_processor_count
+            tile = auto_tile_nm(vectors, gpu["precision"], max(density, 1.0),
+                                n_orb=len(xy), cus=cus)
         tile = float(tile or 0)
         if tile > 0:
             return self._sample.tile_order(tile, return_tiles=True)

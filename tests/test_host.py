@@ -93,6 +93,11 @@ def test_auto_tile_size():
     assert tb.auto_tile_nm(64, "fp64") == 1.5           # 1024 B, floor
     # a denser lattice takes smaller tiles
     assert tb.auto_tile_nm(16, "fp32", density=4 * rho) < tb.auto_tile_nm(16, "fp32")
+    # a small sample keeps two tiles per compute unit: 131,072 orbitals on
+    # 256 CUs, sqrt(131072 / rho / 512) = 2.59 nm; a large one is unchanged
+    assert abs(tb.auto_tile_nm(1, "fp64", n_orb=131072, cus=256) - 2.589) < 0.001
+    assert tb.auto_tile_nm(8, "fp32", n_orb=8388608, cus=256) == tb.auto_tile_nm(8, "fp32")
+    assert tb.auto_tile_nm(1, "fp64", n_orb=1000, cus=256) == 1.5
 
 
 def test_analyzer_dos_matches_loop_restatement():
