@@ -104,9 +104,10 @@ def parse():
                         "row, 3.2 nm for 8 complex128 states, 7.5 nm for one, 7.1 nm for "
                         "the batch's 4 complex64 states)")
     p.add_argument("--cond-tile", type=float, default=None,
-                   help="dc and ac: tile edge in nm (0: the sample's own order; default "
-                        "for dc the library's auto_tile_nm for the run's vectors, as the "
-                        "Solver's gpu['dc_tile_nm'] 'auto': 3.2 nm for 8 states; 2.5 for ac)")
+                   help="dc and ac: tile edge in nm (0: the sample's own order; default: "
+                        "the library's rule for the run's vectors, as the Solver's "
+                        "gpu['dc_tile_nm'] and gpu['ac_tile_nm'] 'auto': dc 3.2 nm for 8 "
+                        "states, ac 3.8 nm for 4 complex64 states, 2.5 for complex128)")
     p.add_argument("--dc-states", type=int, default=8,
                    help="dc: states per run (two propagated vectors each), as the "
                         "Solver's gpu['dc_states']")
@@ -390,9 +391,10 @@ def kernel_label(kind, prec, K, real, threads):
         R = "float" if prec == "fp32" else "double"
         # the launch narrows 16-wave workgroups where the instance needs more
         # registers than a lane has at 16 waves (complex128 states with the
-        # dot epilogue after three or more terms, three states per lane)
-        wide = V == 1 or (V == 2 and (prec == "fp32" or
-                                      (real and not (mode == "4" and int(terms) >= 3))))
+        # dot epilogue after three or more terms, three complex128 states per
+        # lane)
+        wide = V == 1 or (V == 3 and prec == "fp32") or (
+            V == 2 and (prec == "fp32" or (real and not (mode == "4" and int(terms) >= 3))))
         nt = threads if (wide or threads != 1024) else 512
         return (f"k_block_pass<{R}, {V}, {str(bool(real)).lower()}, {mode}, {nt}, "
                 f"{terms}>")
@@ -607,7 +609,9 @@ def cond_order(s, tile):
 def ac_workload(args, exact=None):
     """Pristine graphene 2048x2048 (rescale 9), AC conductivity correlation
     of 8 states at 300 K, mu = 0, run as the Solver runs it: --ac-states
-    states at a time (default 4, on 2.5 nm tiles).  Per run the Fermi and
+    states at a time (default 4) on the tiles of the library's AC rule for
+    each precision (3.8 nm for complex64, 2.5 nm for complex128) unless
+    --cond-tile.  Per run the Fermi and
     one-minus-Fermi polynomials (FFT coefficients at 2^15 points, precision
     1e-12) are applied once, then J_x psi1, J_y psi1 and psi2 of every state
     propagated together with tau = pi/2 (16 Bessel terms at 1e-14) and the
@@ -621,7 +625,6 @@ def ac_workload(args, exact=None):
     L, K = args.L, args.vectors
     per = max(1, min(args.ac_states, K))
     T = max(args.steps, 2)
-    tile = 2.5 if args.cond_tile is None else args.cond_tile
     s = tb.make_graphene_sample(L)
     s.rescale_ham(9.0)
     ip, ix, hop, dx, dy = s.build_ham_dxy_device()
@@ -633,11 +636,14 @@ def ac_workload(args, exact=None):
     cO = tbpm.fermi_cheb_coef(2 ** 15, beta, 0.0, True, 1e-12)
     h = _lib.default_handle(0)
     h.set_graph(not args.no_graph)
-    order, tiles = cond_order(s, tile)
+    tile_of = {prec: (round(tb.ac_tile_nm(3 * per, prec), 1) if args.cond_tile is None
+                      else args.cond_tile) for prec in ("fp32", "fp64")}
     psi = tb.states.random_states(n, tb.states.sample_seeds(1337, K))
     chunks = [(a, min(K, a + per)) for a in range(0, K, per)]
     res = {}
     for prec in ("fp32", "fp64"):
+        tile = tile_of[prec]
+        order, tiles = cond_order(s, tile)
         H = tbpm.Hamiltonian(n, ip, ix, hop, dx, dy, cur_scale=s.rescale, precision=prec,
                              handle=h, order=order, tiles=tiles)
         real = H.info()["real_values"]
@@ -688,6 +694,8 @@ def ac_workload(args, exact=None):
                  kernel_label(k, prec, 3 * per, real, threads))
         res[prec] = {"value": K * (T - 1) / t_prop, "ms_per_step": t_prop / (T - 1) * 1e3,
                      "fermi_setup_s": t_fermi, "shape": shape, "corr": c,
+                     "layout": (f"tiles of {tile:g} nm" if tile > 0 else "caller order")
+                     + passes_name(shape),
                      "roofline": (roofline(kt, bytes_of, label, traffic_key(f"ac_{prec}", tile))
                                   if kt else None),
                      "err_vs_exact": acc}
@@ -704,12 +712,12 @@ def ac_workload(args, exact=None):
                                    f"({3 * per} vectors each), {T} steps, {len(bes)} Bessel "
                                    f"terms, Fermi terms {len(cF)}",
                        "n_orbitals": n, "nnz": nnz,
-                       "layout": (f"tiles of {tile:g} nm" if tile > 0 else "caller order")
-                       + passes_name(r["shape"]),
+                       "layout": r["layout"],
                        "launch_shape": shape_json(r["shape"])},
             "roofline": r["roofline"],
             "fp64": {"value": round(res["fp64"]["value"], 2),
                      "ms_per_step": round(res["fp64"]["ms_per_step"], 4),
+                     "layout": res["fp64"]["layout"],
                      "roofline": res["fp64"]["roofline"]},
             "corr_err_fp32_vs_fp64": err, "corr_max_abs": scale, "corr_err_steps": T,
             "ct_err_fp32_vs_exact": res["fp32"]["err_vs_exact"],

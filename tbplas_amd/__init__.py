@@ -14,7 +14,7 @@ used only to hold device buffers.
 from .config import Config, read_config
 from .sample import (Sample, make_graphene_sample, graphene_hops, supercell_hops,
                      add_anderson_disorder, apply_uniaxial_strain,
-                     random_vacancies, auto_tile_nm)
+                     random_vacancies, auto_tile_nm, ac_tile_nm)
 from .solver import Solver
 from .analysis import Analyzer, window_hanning, window_exp, window_exp_ten
 from . import states, tbpm
@@ -23,5 +23,5 @@ __version__ = "0.1.0"
 
 __all__ = ["Config", "read_config", "Sample", "make_graphene_sample",
            "graphene_hops", "add_anderson_disorder", "apply_uniaxial_strain",
-           "random_vacancies", "auto_tile_nm", "Solver", "Analyzer", "window_hanning",
+           "random_vacancies", "auto_tile_nm", "ac_tile_nm", "Solver", "Analyzer", "window_hanning",
            "window_exp", "window_exp_ten", "states", "tbpm"]

@@ -29,7 +29,10 @@ look at:
                        (8.4M-orbital graphene, complex64: 4 states on
                        2.5 nm tiles 638 state-steps/s, 8 on 1.5 nm 519)
                                                               default 4
-    gpu['ac_tile_nm']  the tile edge of AC runs               default 2.5
+    gpu['ac_tile_nm']  the tile edge of AC runs ('auto': sample.ac_tile_nm
+                       for the run's vectors, 3.8 nm for 4 complex64
+                       states on graphene, 2.5 nm for complex128)
+                                                              default 'auto'
     gpu['dc_states']   states per DC run (two propagated vectors each)
                        (2M-orbital disordered graphene, complex64,
                        step-vectors/s: 8 states on 3.2 nm tiles 4696,
@@ -84,7 +87,7 @@ class Config:
                     'tile_nm': 'auto',
                     'cond_tile_nm': 1.5,
                     'ac_states': 4,
-                    'ac_tile_nm': 2.5,
+                    'ac_tile_nm': 'auto',
                     'dc_states': 8,
                     'dc_tile_nm': 'auto'}
         self._legal_params = dict()

@@ -325,6 +325,7 @@ int term3_launch_threads(int elem_bytes, bool real_vals, int K, int nA) {
     // one state per lane on small tiles: 4-wave workgroups, several per CU,
     // so that a launch over a small sample runs its tiles at once
     if (V == 1) return nA <= 768 ? 256 : 1024;
+    if (V == 3 && elem_bytes == 4) return (int64_t)nA * (K / V) >= 2048 ? 1024 : 512;
     if (V != 2 || (elem_bytes == 8 && !real_vals)) return 512;
     return (int64_t)nA * (K / V) >= 2048 ? 1024 : 512;
 }

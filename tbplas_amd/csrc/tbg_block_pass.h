@@ -486,9 +486,11 @@ template <typename R, int V, bool RV, int MODE, int T>
 static void launch_f(const PassArgs& a, const Blocking& b, hipStream_t s) {
     // the instances term3_launch_threads can ask for; complex128 states with
     // the dot epilogue after three or more terms need more than the 128
-    // registers a lane has at 16 waves, so they take 8
+    // registers a lane has at 16 waves, so they take 8, and so do three
+    // complex128 states per lane
     constexpr bool wide =
-        V == 1 || (V == 2 && (sizeof(R) == 4 || (RV && !(MODE == PM_DOTNORM && T >= 3))));
+        V == 1 || (V == 3 && sizeof(R) == 4) ||
+        (V == 2 && (sizeof(R) == 4 || (RV && !(MODE == PM_DOTNORM && T >= 3))));
     int nt = term3_launch_threads((int)sizeof(R), RV, a.K, b.nA_max);
     if (!wide && V >= 2) nt = 512;
     // areas A and B and B's spare row (phase 1 parks the rows beyond B there)
@@ -560,6 +562,7 @@ void allow_lds_rt(int bytes) {
     allow_lds_v<R, 1, 1024, T, RV>(bytes);
     allow_lds_v<R, 2, 1024, T, RV>(bytes);
     allow_lds_v<R, 2, 512, T, RV>(bytes);
+    if constexpr (sizeof(R) == 4) allow_lds_v<R, 3, 1024, T, RV>(bytes);
     allow_lds_v<R, 3, 512, T, RV>(bytes);
 }
 

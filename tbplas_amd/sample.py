@@ -25,7 +25,7 @@ from . import _lib
 
 __all__ = ["Sample", "make_graphene_sample", "graphene_hops", "supercell_hops",
            "add_anderson_disorder", "apply_uniaxial_strain", "random_vacancies",
-           "auto_tile_nm"]
+           "auto_tile_nm", "ac_tile_nm"]
 
 GRAPHENE_A = 0.246      # nm, make_graphene_diamond (materials/graphene.py:16-31)
 GRAPHENE_T = -2.7       # eV
@@ -125,6 +125,21 @@ def auto_tile_nm(vectors, precision="fp32", density=GRAPHENE_ORB_DENSITY, terms=
     if n_orb is not None and cus:
         edge = min(edge, math.sqrt(n_orb / density / (2.0 * cus)))
     return float(min(7.5, max(1.5, edge)))
+
+
+def ac_tile_nm(vectors, precision="fp32", density=GRAPHENE_ORB_DENSITY, n_orb=None, cus=None):
+    """Tile edge (nm) of AC runs with ``vectors`` propagated vectors per row
+    (three per state): complex64 takes ``auto_tile_nm``; complex128, whose
+    three states per lane run 8-wave workgroups that gain from two per CU,
+    takes the smaller of that and 2.5 nm at graphene's density (scaled to
+    the same rows per tile at others).  Measured on 8.4M-orbital graphene,
+    4 states a run (state·steps/s): complex64 753 on 3.8 nm (the rule),
+    725 on 3.6, 645 on 3.0, 648 on 2.5; complex128 376 on 2.5, 336 on 2.7,
+    335 on 3.0, 332 on 3.8."""
+    edge = auto_tile_nm(vectors, precision, density, n_orb=n_orb, cus=cus)
+    if precision == "fp64":
+        edge = min(edge, max(1.5, 2.5 * math.sqrt(GRAPHENE_ORB_DENSITY / density)))
+    return float(edge)
 
 
 def random_vacancies(n_orb, fraction, seed):

@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _lib, states, tune
 from .config import Config
-from .sample import Sample, auto_tile_nm
+from .sample import Sample, ac_tile_nm, auto_tile_nm
 from .tbpm import (Hamiltonian, bessel_series, corr_ac, corr_dc, corr_dos, corr_dyn_pol,
                    corr_ldos, fermi_cheb_coef, hall_mu, haydock_coef, haydock_green,
                    ldos_states, psi_t, quasi_eigenstates)
@@ -136,12 +136,12 @@ class Solver:
             yield i, min(total, i + per)
             i += per
 
-    def _row_order(self, gpu, key, vectors=1):
+    def _row_order(self, gpu, key, vectors=1, rule=auto_tile_nm):
         """(row order, tile offsets) of square tiles of gpu[key] nm ('auto':
-        sized for `vectors` states per row, the sample's orbitals per nm^2
-        in the x-y plane and, for a small sample, at least two tiles per
-        compute unit, ``auto_tile_nm``) when the sample has orbital
-        positions, else (None, None)."""
+        sized by `rule`, ``auto_tile_nm`` or ``ac_tile_nm``, for `vectors`
+        states per row, the sample's orbitals per nm^2 in the x-y plane and,
+        for a small sample, at least two tiles per compute unit) when the
+        sample has orbital positions, else (None, None)."""
         pos = self._sample.orb_pos
         if pos is None:
             return None, None
@@ -155,8 +155,7 @@ class Solver:
             density = float(np.median(np.unique(key, return_counts=True)[1])) / 4.0
             import torch
             cus = torch.cuda.get_device_properties(gpu["device"]).multi_processor_count
-            tile = auto_tile_nm(vectors, gpu["precision"], max(density, 1.0),
-                                n_orb=len(xy), cus=cus)
+            tile = rule(vectors, gpu["precision"], max(density, 1.0), n_orb=len(xy), cus=cus)
         tile = float(tile or 0)
         if tile > 0:
             return self._sample.tile_order(tile, return_tiles=True)
@@ -353,14 +352,15 @@ class Solver:
                                  g["Fermi_cheb_precision"])
         coef_omF = fermi_cheb_coef(g["nr_Fermi_fft_steps"], beta_re, mu_re, True,
                                    g["Fermi_cheb_precision"])
-        order, tiles = self._row_order(gpu, "ac_tile_nm")
+        per = max(1, min(21, int(gpu["ac_states"]), int(gpu["max_vectors"])))
+        order, tiles = self._row_order(gpu, "ac_tile_nm", 3 * min(per, num_sample),
+                                       ac_tile_nm)
         H = Hamiltonian(self._sample.num_orb, indptr, indices, hop, dx, dy,
                         cur_scale=self._sample.rescale, precision=gpu["precision"],
                         handle=h, order=order, tiles=tiles)
         psi0 = self._states(num_sample)
         chk, thr = self._chk()
         T = g["nr_time_steps"]
-        per = max(1, min(21, int(gpu["ac_states"]), int(gpu["max_vectors"])))
         tune.apply(H, 3 * min(per, num_sample), bes, gpu["autotune"])
         corr = np.zeros((4, T), dtype=np.complex128)
         for a, b in self._batches(num_sample, per):

@@ -9,9 +9,9 @@ oracle.
   propagation.  Tolerances: complex128 1e-10;
   complex64 (FP32 storage and arithmetic, FP64 sums) 1e-6, about ten times
   the largest error measured over 2048 steps.
-* AC on the same sample (8 states in runs of 4 on 2.5 nm tiles with four
-  terms per pass, complex128 also with five, and in one run on 1.5 nm
-  tiles with three) against exact
+* AC on the same sample (8 states in runs of 4 on the AC rule's tiles,
+  complex64 3.8 nm and complex128 2.5 nm, with five terms per pass, on
+  2.5 nm with four, and in one run on 1.5 nm tiles with three) against exact
   k-space propagation with the product's Fermi polynomial over 256 steps:
   complex128 1e-11, complex64 1e-6 (correlations ~4e-2).
 * The disordered, field-threaded DC sample (1% vacancies, Anderson disorder
@@ -130,13 +130,14 @@ def ac2048(graphene2048):
 
 
 @pytest.mark.parametrize("prec,tol,tile,terms,per", [
-    ("fp32", 1e-6, 2.5, 4, 4), ("fp64", 1e-11, 2.5, 4, 4), ("fp64", 1e-11, 2.5, 5, 4),
-    ("fp32", 1e-6, 1.5, 3, 8), ("fp64", 1e-11, 1.5, 3, 8)])
+    ("fp32", 1e-6, 3.8, 5, 4), ("fp32", 1e-6, 2.5, 4, 4), ("fp64", 1e-11, 2.5, 4, 4),
+    ("fp64", 1e-11, 2.5, 5, 4), ("fp32", 1e-6, 1.5, 3, 8), ("fp64", 1e-11, 1.5, 3, 8)])
 def test_ac_benchmark_layout_vs_kspace(gpu, ac2048, prec, tol, tile, terms, per):
     """AC on 8.4M-orbital graphene as the benchmark and the Solver run it
     (runs of 4 states, 12 vectors: J_x psi1, J_y psi1 and psi2 of each, on
-    2.5 nm tiles with four terms per pass, complex128 also with five, which
-    the tuner picks for it) and as one run of 8 states (24
+    the tiles of the library's AC rule with five terms per pass, which the
+    tuner picks there: complex64 3.8 nm, complex128 2.5 nm; both also on
+    2.5 nm with four) and as one run of 8 states (24
     vectors, 1.5 nm tiles, three terms per pass), against exact k-space
     propagation with the same Fermi polynomial, at sampled steps of 256
     (the correlations are ~4e-2)."""
@@ -160,6 +161,8 @@ def test_ac_benchmark_layout_vs_kspace(gpu, ac2048, prec, tol, tile, terms, per)
           f"(max |C| {np.max(np.abs(exact)):.3e})", flush=True)
     assert np.max(np.abs(c[:, :, 0])) == 0.0
     assert err < tol
+    if tile != 1.5 and per == 4:
+        assert tile == 2.5 or tile == round(tb.ac_tile_nm(3 * per, prec), 1)
 
 
 def _cfg3_sample(L, seed=2024):

@@ -98,6 +98,12 @@ def test_auto_tile_size():
     assert abs(tb.auto_tile_nm(1, "fp64", n_orb=131072, cus=256) - 2.589) < 0.001
     assert tb.auto_tile_nm(8, "fp32", n_orb=8388608, cus=256) == tb.auto_tile_nm(8, "fp32")
     assert tb.auto_tile_nm(1, "fp64", n_orb=1000, cus=256) == 1.5
+    # AC: complex64 follows the rule; complex128 stays at 2.5 nm on graphene
+    assert tb.ac_tile_nm(12, "fp32") == tb.auto_tile_nm(12, "fp32")
+    assert abs(tb.ac_tile_nm(12, "fp32") - 3.81) < 0.01
+    assert tb.ac_tile_nm(12, "fp64") == 2.5
+    assert tb.ac_tile_nm(48, "fp64") == tb.auto_tile_nm(48, "fp64") < 2.5
+    assert abs(tb.ac_tile_nm(12, "fp64", density=4 * rho) - 1.5) < 1e-12
 
 
 def test_analyzer_dos_matches_loop_restatement():
