@@ -23,7 +23,9 @@ from . import _lib
 __all__ = ["autotune", "apply", "CANDIDATES"]
 
 CANDIDATES = [(bpc, stage) for bpc in (4, 8, 12, 16) for stage in (True, False)]
-DEFAULT = (16, True, 3)
+# untuned: blocks per CU and staging; the terms per pass are those the
+# tiles of auto_tile_nm are sized for (sample.pass_terms)
+DEFAULT = (16, True)
 
 _lock = threading.Lock()
 _mem = {}
@@ -134,6 +136,9 @@ def apply(H, K, bes, enabled=True):
     Hamiltonian or the list of a batch)."""
     if enabled:
         return autotune(H, K, bes)[0]
-    h = (H[0] if isinstance(H, (list, tuple)) else H).handle
-    _set(h, DEFAULT)
-    return DEFAULT
+    from .sample import pass_terms
+    H0 = H[0] if isinstance(H, (list, tuple)) else H
+    prec = "fp32" if H0.precision == _lib.TBG_FP32 else "fp64"
+    shape = DEFAULT + (pass_terms(K, prec),)
+    _set(H0.handle, shape)
+    return shape
