@@ -996,7 +996,7 @@ def assembly_workload(args):
                                            _lib.ptr(indptr), _lib.ptr(ix), _lib.ptr(dt)))
             return k
 
-        def best(f, reps=3):
+        def best(f, reps=5):
             f()
             torch.cuda.synchronize()
             ts = []
